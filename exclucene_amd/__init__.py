@@ -1,0 +1,356 @@
+"""exclucene_amd -- MI355X-native decode + exact intersection of compressed
+posting lists (LRC, SegLRC, HS_SegLRC, ParaPFD), behind the C ABI declared in
+include/exclucene_amd.h.
+
+This module is the host-side mirror of the reference's per-program stage
+functions (src/gpu/Compressed/src/<codec>/<codec>_Intersection/*.cu:
+CPUPreprocess / htodTransfer / kernelInvoke / dtohTransfer / verify) over
+ctypes.  All decoding and intersecting happens in the HIP kernels of
+libexclucene_amd.so; there is no CPU fallback -- loading fails loudly if the
+library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libexclucene_amd.so")
+
+LRC, SEGLRC, HS_SEGLRC, PARAPFD = 0, 1, 2, 3
+CODECS = {"LRC": LRC, "SegLRC": SEGLRC, "HS_SegLRC": HS_SEGLRC, "ParaPFD": PARAPFD}
+
+OPT_SEARCH, OPT_TPE, OPT_PERSISTENT, OPT_GRAPH, OPT_TIMING = 0, 1, 2, 3, 4
+SEARCH_BINARY, SEARCH_FIT, SEARCH_LEARNED = 0, 1, 2
+
+EXPORTS = [
+    "exc_ctx_create", "exc_ctx_destroy", "exc_last_error", "exc_ctx_set_option", "exc_ctx_last_times",
+    "exc_index_upload", "exc_index_load_files", "exc_index_free", "exc_index_device_bytes",
+    "exc_CPUPreprocess", "exc_htodTransfer", "exc_kernelInvoke", "exc_dtohTransfer", "exc_verify",
+    "exc_intersect", "exc_plan", "exc_decompress",
+    "exc_build_index", "exc_built_view", "exc_built_free", "exc_index_save_files",
+    "exc_synth_lists",
+]
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+
+
+class IndexHost(ctypes.Structure):
+    """exc_index_host (include/exclucene_amd.h)."""
+    _fields_ = [("codec", ctypes.c_int32), ("seg_size", _u32), ("nterms", _u32), ("reserved", _u32),
+                ("ind1", _vp), ("ind2", _vp), ("ind2_words", _u64),
+                ("seg_num", _vp), ("seg_offset", _vp), ("nseg", _u64),
+                ("seg_head", _vp), ("seg_median", _vp), ("lrc_info", _vp), ("seg_info", _vp),
+                ("seg_base", _vp), ("list_info", _vp), ("hash_offset", _vp), ("hash_info", _vp),
+                ("hash_words", _u64)]
+
+
+class BatchInfo(ctypes.Structure):
+    """batchInfo (LRC_Intersection_kernel.cu:68-74)."""
+    _fields_ = [("blockNum", _u32), ("constantUsedInByte", _u32), ("ucntShortest_sum", _u32),
+                ("nTotalQueryNum", _u32), ("nLinearInfoInByte", _u32)]
+
+
+class CheckSum(ctypes.Structure):
+    """checkSum (LRC_Intersection_kernel.cu:79-83)."""
+    _fields_ = [("checkSum1", _u64), ("checkSum2", _u64), ("checkSum3", _u64)]
+
+
+IND1 = np.dtype([("m_urlcount", "<u4"), ("pad", "<u4"), ("m_off", "<u8")])
+LRC_INFO = np.dtype([("fSlope", "<f4"), ("iIntercept", "<i4"), ("fSearchRange", "<f4")])
+SEG_INFO = np.dtype([("fSlope", "<f4"), ("fIntercept", "<f4"), ("nRangeLeft", "<u4"), ("nRangeRight", "<u4")])
+
+_lib = None
+
+
+class ExcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("exclucene_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libexclucene_amd.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libexclucene_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name).restype = ctypes.c_int
+        L.exc_last_error.restype = ctypes.c_char_p
+        L.exc_ctx_destroy.restype = None
+        L.exc_index_free.restype = None
+        L.exc_built_free.restype = None
+        L.exc_index_device_bytes.restype = _u64
+        L.exc_ctx_create.argtypes = [ctypes.c_int, _u64, ctypes.POINTER(_vp)]
+        L.exc_ctx_destroy.argtypes = [_vp]
+        L.exc_ctx_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
+        L.exc_ctx_last_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.exc_index_upload.argtypes = [_vp, ctypes.POINTER(IndexHost), ctypes.POINTER(_vp)]
+        L.exc_index_load_files.argtypes = [_vp, ctypes.c_int, _u32, ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.POINTER(_vp)]
+        L.exc_index_free.argtypes = [_vp]
+        L.exc_index_device_bytes.argtypes = [_vp]
+        L.exc_CPUPreprocess.argtypes = [_vp, _vp, ctypes.POINTER(_vp), _vp, _u64, ctypes.POINTER(BatchInfo)]
+        L.exc_htodTransfer.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
+        L.exc_kernelInvoke.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
+        L.exc_dtohTransfer.argtypes = [_vp, ctypes.POINTER(BatchInfo), _vp, _vp, _u64]
+        L.exc_verify.argtypes = [ctypes.POINTER(BatchInfo), _vp, _vp, ctypes.POINTER(CheckSum)]
+        L.exc_intersect.argtypes = [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+        L.exc_plan.argtypes = [_vp, _vp, _vp, _vp, _u32, ctypes.POINTER(BatchInfo)]
+        L.exc_decompress.argtypes = [_vp, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]
+        L.exc_build_index.argtypes = [ctypes.c_int, _u32, ctypes.c_float, _u32, _u32, _vp, _vp, _u32,
+                                      ctypes.c_int, ctypes.POINTER(_vp)]
+        L.exc_built_view.argtypes = [_vp, ctypes.POINTER(IndexHost)]
+        L.exc_built_free.argtypes = [_vp]
+        L.exc_index_save_files.argtypes = [ctypes.POINTER(IndexHost), ctypes.c_char_p, ctypes.c_char_p]
+        L.exc_synth_lists.argtypes = [_u64, _u32, _vp, _u32, ctypes.c_int, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise ExcError(rc, lib().exc_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+def _arr(ptr, count, dtype):
+    if not ptr or count == 0:
+        return None
+    dt = np.dtype(dtype)
+    buf = (ctypes.c_char * (count * dt.itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dt, count=count)
+
+
+# --------------------------------------------------------------- corpus + build
+
+def synth_lists(df, universe, seed, nthreads=0):
+    """Sorted uniform samples without replacement, list t of length df[t]."""
+    df = np.ascontiguousarray(df, np.uint32)
+    off = np.zeros(len(df) + 1, np.uint64)
+    off[1:] = np.cumsum(df, dtype=np.uint64)
+    docs = np.empty(max(int(off[-1]), 1), np.uint32)
+    _check(lib().exc_synth_lists(seed, universe, _p(df), len(df), nthreads, _p(docs), _p(off)))
+    return docs[:int(off[-1])], off
+
+
+class BuiltIndex:
+    """A compressed index built on the host (the *_Generator + *_Compression
+    programs' output), viewed as numpy arrays without copies."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.view = IndexHost()
+        _check(lib().exc_built_view(handle, ctypes.byref(self.view)))
+        v = self.view
+        nt, ns = v.nterms, v.nseg
+        self.codec, self.seg, self.nterms, self.nseg = v.codec, v.seg_size, nt, ns
+        self.ind1 = _arr(v.ind1, nt, IND1)
+        self.ind2 = _arr(v.ind2, v.ind2_words, np.uint32)
+        self.seg_num = _arr(v.seg_num, nt, np.uint32)
+        self.seg_offset = _arr(v.seg_offset, ns, np.uint32)
+        self.seg_head = _arr(v.seg_head, ns, np.uint32)
+        self.seg_median = _arr(v.seg_median, ns, np.uint32)
+        self.lrc_info = _arr(v.lrc_info, nt, LRC_INFO)
+        self.seg_info = _arr(v.seg_info, ns, SEG_INFO)
+        self.seg_base = _arr(v.seg_base, ns, np.int32)
+        self.list_info = _arr(v.list_info, nt, SEG_INFO)
+        self.hash_offset = _arr(v.hash_offset, nt, np.uint32)
+        self.hash_info = _arr(v.hash_info, v.hash_words, np.uint32)
+
+    @property
+    def ucnt(self):
+        return self.ind1["m_urlcount"]
+
+    def compressed_bytes(self):
+        tot = 0
+        for name in ("ind1", "ind2", "seg_num", "seg_offset", "seg_head", "seg_median", "lrc_info",
+                     "seg_info", "seg_base", "hash_offset", "hash_info"):
+            a = getattr(self, name)
+            if a is not None:
+                tot += a.nbytes
+        return tot
+
+    def save(self, directory, name):
+        _check(lib().exc_index_save_files(ctypes.byref(self.view), directory.encode(), name.encode()))
+
+    def close(self):
+        if self._h:
+            lib().exc_built_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_index(docs, offsets, codec, seg=128, frac=0.0, hs_target=256, hs_shift=0, nthreads=0):
+    """Encode concatenated sorted lists (docs, offsets[nterms+1]) into one index."""
+    docs = np.ascontiguousarray(docs, np.uint32)
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    h = _vp()
+    _check(lib().exc_build_index(codec, seg, frac, hs_target, hs_shift, _p(docs) or docs.ctypes.data,
+                                 _p(offsets), len(offsets) - 1, nthreads, ctypes.byref(h)))
+    return BuiltIndex(h)
+
+
+def lists_to_flat(lists):
+    lists = [np.ascontiguousarray(x, np.uint32) for x in lists]
+    off = np.zeros(len(lists) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    docs = np.concatenate(lists) if off[-1] else np.zeros(1, np.uint32)
+    return docs, off
+
+
+def queries_to_flat(queries):
+    qoff = np.zeros(len(queries) + 1, np.uint64)
+    qoff[1:] = np.cumsum([len(q) for q in queries])
+    qterms = np.array([t for q in queries for t in q], np.uint32) if qoff[-1] else np.zeros(1, np.uint32)
+    return qterms, qoff
+
+
+def queries_to_stream(queries):
+    """The reference's .query byte stream: {u32 tnum; u32 tno[tnum]}..."""
+    out = []
+    for q in queries:
+        out.append(len(q))
+        out.extend(q)
+    return np.array(out, np.uint32).tobytes()
+
+
+# --------------------------------------------------------------- device side
+
+class DeviceIndex:
+    def __init__(self, ctx, handle, built=None):
+        self.ctx, self._h, self.built = ctx, handle, built
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self):
+        return int(lib().exc_index_device_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            lib().exc_index_free(self._h)
+            self._h = None
+
+
+class Context:
+    """One HIP stream and its batch buffers (one per host thread)."""
+
+    def __init__(self, device=0, budget=0):
+        h = _vp()
+        _check(lib().exc_ctx_create(device, budget, ctypes.byref(h)))
+        self._h = h
+        self._bi = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_option(self, key, value):
+        _check(lib().exc_ctx_set_option(self._h, key, int(value)))
+
+    def last_times(self):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        _check(lib().exc_ctx_last_times(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def upload(self, built):
+        h = _vp()
+        _check(lib().exc_index_upload(self._h, ctypes.byref(built.view), ctypes.byref(h)))
+        return DeviceIndex(self, h, built)
+
+    def load_files(self, codec, seg, directory, name):
+        h = _vp()
+        _check(lib().exc_index_load_files(self._h, codec, seg, directory.encode(), name.encode(), ctypes.byref(h)))
+        return DeviceIndex(self, h)
+
+    # -- reference stage API
+    def CPUPreprocess(self, index, stream_bytes, pos, threshold):
+        """Plan one batch from the .query stream starting at byte pos; returns
+        (batchInfo, new_pos)."""
+        buf = ctypes.create_string_buffer(stream_bytes, len(stream_bytes)) if isinstance(stream_bytes, bytes) else stream_bytes
+        base = ctypes.addressof(buf)
+        p = _vp(base + pos)
+        bi = BatchInfo()
+        _check(lib().exc_CPUPreprocess(self._h, index.handle, ctypes.byref(p), base + len(stream_bytes),
+                                       threshold, ctypes.byref(bi)))
+        self._bi = bi
+        return bi, p.value - base
+
+    def htodTransfer(self, bi):
+        _check(lib().exc_htodTransfer(self._h, ctypes.byref(bi)))
+
+    def kernelInvoke(self, bi):
+        _check(lib().exc_kernelInvoke(self._h, ctypes.byref(bi)))
+
+    def dtohTransfer(self, bi, cap=None):
+        nq = bi.nTotalQueryNum
+        cap = bi.ucntShortest_sum if cap is None else cap
+        cum = np.zeros(max(nq, 1), np.uint32)
+        res = np.zeros(max(cap, 1), np.uint32)
+        _check(lib().exc_dtohTransfer(self._h, ctypes.byref(bi), _p(cum), _p(res), cap))
+        n = int(cum[nq - 1]) if nq else 0
+        return cum[:nq], res[:n]
+
+    @staticmethod
+    def verify(bi, cum, res):
+        cs = CheckSum()
+        c = np.ascontiguousarray(cum, np.uint32)
+        r = np.ascontiguousarray(res, np.uint32) if len(res) else np.zeros(1, np.uint32)
+        _check(lib().exc_verify(ctypes.byref(bi), _p(c), _p(r), ctypes.byref(cs)))
+        return cs.checkSum1, cs.checkSum2, cs.checkSum3
+
+    # -- direct API
+    def plan(self, index, queries):
+        qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
+        bi = BatchInfo()
+        _check(lib().exc_plan(self._h, index.handle, _p(qterms), _p(qoff), len(qoff) - 1, ctypes.byref(bi)))
+        self._bi = bi
+        return bi
+
+    def intersect(self, index, queries):
+        """Exact results of every query: (cum_counts, results)."""
+        qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
+        nq = len(qoff) - 1
+        ucnt = index.built.ucnt if index.built is not None else None
+        cap = 0
+        if ucnt is not None:
+            for q in range(nq):
+                ts = qterms[int(qoff[q]):int(qoff[q + 1])]
+                if len(ts):
+                    cap += int(ucnt[ts].min())
+        else:
+            cap = 1 << 26
+        cum = np.zeros(max(nq, 1), np.uint32)
+        res = np.zeros(max(cap, 1), np.uint32)
+        n = _u64()
+        _check(lib().exc_intersect(self._h, index.handle, _p(qterms), _p(qoff), nq, _p(cum), _p(res), cap,
+                                   ctypes.byref(n)))
+        return cum[:nq], res[:int(n.value)]
+
+    def decompress(self, index, terms):
+        terms = np.ascontiguousarray(terms, np.uint32)
+        ucnt = index.built.ucnt
+        cap = int(sum(int(ucnt[t]) for t in terms))
+        out = np.zeros(max(cap, 1), np.uint32)
+        n = _u64()
+        _check(lib().exc_decompress(self._h, index.handle, _p(terms), len(terms), _p(out), cap, ctypes.byref(n)))
+        return out[:int(n.value)]
+
+    def close(self):
+        if self._h:
+            lib().exc_ctx_destroy(self._h)
+            self._h = None

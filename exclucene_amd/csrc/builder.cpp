@@ -1,0 +1,478 @@
+// builder.cpp -- CPU index builder and synthetic corpus generator.
+//
+// Replaces the reference's two offline programs per codec: the *_Generator
+// (double-precision fits + residual kernel + min shift) and the *_Compression
+// (ParaPFor packing + side files).  Output arrays follow the reference's file
+// layouts exactly (GC = src/gpu/Compressed/src, GI = src/gpu/Compressed/include):
+//   fits        GC/LRC/LRC_Generator/LRC_Generator.cu:221-287,
+//               GC/SegLRC/SegLRC_Generator/SegLRC_Generator.cu:264-331
+//   residuals   GC/LRC/LRC_Generator/LRC_Generator_kernel.cu:55-91,
+//               GC/SegLRC/SegLRC_Generator/SegLRC_Generator_kernel.cu:62-100
+//   min shift   GC/LRC/LRC_Generator/LRC_Generator.cu:165-180,
+//               GC/SegLRC/SegLRC_Generator/SegLRC_Generator.cu:185-233
+//   HS table    GC/HS_SegLRC/HS_SegLRC_Generator/HS_SegLRC_Generator.cu:503-594
+//   packing     GC/LRC/LRC_Compression/main.cpp:163-199, GI/ParaPFor.cpp:56-94,
+//               GC/ParaPFD/ParaPFD_Compression/main.cpp:151-189,
+//               GC/HS_SegLRC/HS_SegLRC_Compression/main.cpp:149-178
+// Lists are encoded in parallel (static partition over std::thread workers);
+// the result is identical for any thread count.
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/exclucene_amd.h"
+#include "exc_format.h"
+#include "exc_internal.h"
+
+using namespace exc;
+
+struct exc_built {
+    int32_t codec = 0;
+    uint32_t seg = 256;
+    std::vector<exc_term_ind1> ind1;
+    std::vector<uint32_t> ind2;
+    std::vector<uint32_t> seg_num, seg_offset, seg_head, seg_median;
+    std::vector<exc_lrc_info> lrc;
+    std::vector<exc_seg_info> seg_info, list_info;
+    std::vector<int32_t> base;
+    std::vector<uint32_t> hash_offset, hash_info;
+};
+
+namespace {
+
+// ---------------------------------------------------------------- ParaPFor
+
+// Smallest b with #(v >= 2^b) <= frac * n, compared in float (GI/ParaPFor.cpp:31-53).
+uint32_t pfor_best_b(const uint32_t* v, uint32_t n, float frac)
+{
+    uint32_t hist[33] = {0};
+    for (uint32_t i = 0; i < n; ++i) hist[bits_of(v[i])]++;
+    uint32_t above = n - hist[0];  // values needing more than b bits, b = 0
+    for (uint32_t b = 0; b < 32; ++b) {
+        if ((float)above <= frac * (float)n) return b;
+        above -= hist[b + 1];
+    }
+    return 32;
+}
+
+void put_bits(uint32_t* out, const uint32_t* v, uint32_t n, uint32_t b)
+{
+    uint32_t nw = words_for(n, b);
+    std::memset(out, 0, (size_t)nw * 4);
+    if (!b) return;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t bit = (uint64_t)i * b;
+        uint32_t w = (uint32_t)(bit >> 5), s = (uint32_t)(bit & 31);
+        out[w] |= v[i] << s;
+        if (s + b > 32) out[w + 1] |= v[i] >> (32 - s);
+    }
+}
+
+// One ParaPFor block appended to out; false if the descriptor cannot hold it.
+bool pfor_block(const uint32_t* v, uint32_t n, float frac, std::vector<uint32_t>& out)
+{
+    uint32_t lb = pfor_best_b(v, n, frac);
+    if (lb > 31) return false;
+    std::vector<uint32_t> low(v, v + n), pos, hi;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (bits_of(v[i]) > lb) {
+            pos.push_back(i);
+            hi.push_back(v[i] >> lb);
+            low[i] = v[i] & ((1u << lb) - 1u);
+        }
+    }
+    uint32_t en = (uint32_t)pos.size();
+    uint32_t eb = en ? bits_of(pos.back()) : 0, hb = 0;
+    for (uint32_t h : hi) hb = std::max(hb, bits_of(h));
+    if (en > 65535 || hb > 31) return false;
+    size_t at = out.size();
+    out.resize(at + 1 + words_for(n, lb) + words_for(en, eb) + words_for(en, hb));
+    uint32_t* o = out.data() + at;
+    o[0] = en ? (lb << 26) | (eb << 21) | (hb << 16) | en : (lb << 26);
+    o += 1;
+    put_bits(o, low.data(), n, lb);
+    o += words_for(n, lb);
+    if (en) {
+        put_bits(o, pos.data(), en, eb);
+        o += words_for(en, eb);
+        put_bits(o, hi.data(), en, hb);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------- fits
+
+exc_lrc_info lrc_fit(const uint32_t* L, uint32_t n)
+{
+    exc_lrc_info r{1.0f, 0, 0.0f};
+    if (n == 0) return r;
+    if (n == 1) { r.iIntercept = (int32_t)L[0]; return r; }
+    double xa = 0, ya = 0;
+    for (uint32_t i = 0; i < n; ++i) { xa += i; ya += L[i]; }
+    xa /= n; ya /= n;
+    double sx = 0, sxy = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double dx = i - xa;
+        sx += dx * dx;
+        sxy += dx * ((double)L[i] - ya);
+    }
+    double slope = sxy / sx, icpt = ya - slope * xa, rl = 0, rr = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double d = (L[i] - icpt) / slope - i;
+        if (d > rl) rl = d;
+        else if (d < rr) rr = d;
+    }
+    r.fSlope = (float)slope;
+    r.iIntercept = (int32_t)icpt;
+    r.fSearchRange = (float)(rl - rr);
+    return r;
+}
+
+exc_seg_info seg_fit(const uint32_t* L, uint32_t n)
+{
+    exc_seg_info r{1.0f, 0.0f, 0, 0};
+    if (n == 0) return r;
+    if (n == 1) { r.fIntercept = (float)L[0]; return r; }
+    double xa = 0, ya = 0;
+    for (uint32_t i = 0; i < n; ++i) { xa += i; ya += L[i]; }
+    xa /= n; ya /= n;
+    double sx = 0, sxy = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double dx = i - xa;
+        sx += dx * dx;
+        sxy += dx * ((double)L[i] - ya);
+    }
+    r.fSlope = (float)(sxy / sx);
+    r.fIntercept = (float)(ya - (double)r.fSlope * xa);
+    float rl = 0, rr = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float d = ((float)L[i] - r.fIntercept) / r.fSlope - (float)i;
+        if (d > rl) rl = d;
+        else if (d < rr) rr = d;
+    }
+    r.nRangeLeft = (uint32_t)rl + 1;
+    r.nRangeRight = (uint32_t)(0 - rr) + 1;
+    return r;
+}
+
+// residuals against a segment fit, shifted by their minimum; returns the base
+int32_t seg_residuals(const uint32_t* L, uint32_t n, const exc_seg_info& f, uint32_t* vd)
+{
+    int32_t mn = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        int32_t d = (int32_t)(L[j] - (uint32_t)seg_y(j, f.fSlope, f.fIntercept));
+        vd[j] = (uint32_t)d;
+        if (j == 0 || d < mn) mn = d;
+    }
+    for (uint32_t j = 0; j < n; ++j) vd[j] = (uint32_t)((int32_t)vd[j] - mn);
+    return n ? mn : 0;
+}
+
+void hs_table(const uint32_t* L, uint32_t n, uint32_t target, uint32_t fixed_shift, std::vector<uint32_t>& t)
+{
+    uint32_t maxb = n ? bits_of(L[n - 1]) : 0;
+    if (!fixed_shift && n <= target) { t = {maxb, 1u, 0u, n}; return; }
+    uint32_t shift = fixed_shift ? fixed_shift : maxb - bits_of((n + target - 1) / target);
+    uint32_t nb = n ? (L[n - 1] >> shift) + 1 : 1;
+    t.assign(nb + 3, 0);
+    t[0] = shift;
+    t[1] = nb;
+    uint32_t cur = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t b = L[i] >> shift;
+        while (cur < b) t[2 + (++cur)] = i;
+    }
+    t[2 + nb] = n;
+}
+
+// Encoded form of one list before it is placed in the index.
+struct ListOut {
+    std::vector<uint32_t> words, seg_offset, seg_head, seg_median, hash;
+    std::vector<exc_seg_info> seg_info;
+    std::vector<int32_t> base;
+    exc_lrc_info lrc{1.0f, 0, 0.0f};
+    exc_seg_info list{1.0f, 0.0f, 0, 0};
+    bool ok = true;
+};
+
+void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
+                 const uint32_t* L, uint32_t n, ListOut& o)
+{
+    std::vector<uint32_t> vd(n ? n : 1);
+    if (codec == EXC_LRC) {
+        o.lrc = lrc_fit(L, n);
+        if (n > 1) {
+            int32_t mn = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                int32_t d = (int32_t)(L[i] - (uint32_t)(lrc_y(i, o.lrc.fSlope) + o.lrc.iIntercept));
+                vd[i] = (uint32_t)d;
+                if (i == 0 || d < mn) mn = d;
+            }
+            for (uint32_t i = 0; i < n; ++i) vd[i] = (uint32_t)((int32_t)vd[i] - mn);
+            o.lrc.iIntercept += mn;
+        } else {
+            vd[0] = 0;
+        }
+    }
+    if (codec == EXC_PARAPFD && n) {
+        vd[0] = L[0];
+        for (uint32_t j = 1; j < n; ++j) vd[j] = L[j] - L[j - 1];
+    }
+    if (codec == EXC_SEGLRC) o.list = seg_fit(L, n);
+    if (codec == EXC_HS_SEGLRC) {
+        hs_table(L, n, hs_target, hs_shift, o.hash);
+        uint32_t nb = o.hash[1];
+        o.seg_offset.resize(nb);
+        o.seg_info.resize(nb);
+        o.base.resize(nb);
+        for (uint32_t b = 0; b < nb; ++b) {
+            uint32_t st = o.hash[2 + b], len = o.hash[3 + b] - st;
+            o.seg_info[b] = seg_fit(L + st, len);
+            o.base[b] = seg_residuals(L + st, len, o.seg_info[b], vd.data() + st);
+            o.seg_offset[b] = (uint32_t)o.words.size();
+            if (len && !pfor_block(vd.data() + st, len, frac, o.words)) { o.ok = false; return; }
+        }
+        return;
+    }
+    uint32_t nseg = (n + seg - 1) / seg;
+    o.seg_offset.resize(nseg);
+    o.seg_head.resize(nseg);
+    if (codec == EXC_PARAPFD) o.seg_median.resize(nseg);
+    if (codec == EXC_SEGLRC) { o.seg_info.resize(nseg); o.base.resize(nseg); }
+    for (uint32_t s = 0; s < nseg; ++s) {
+        uint32_t st = s * seg, len = std::min(seg, n - st);
+        o.seg_head[s] = L[st];
+        if (codec == EXC_PARAPFD) o.seg_median[s] = L[st + len / 2];
+        if (codec == EXC_SEGLRC) {
+            o.seg_info[s] = seg_fit(L + st, len);
+            o.base[s] = seg_residuals(L + st, len, o.seg_info[s], vd.data() + st);
+        }
+        o.seg_offset[s] = (uint32_t)o.words.size();
+        if (!pfor_block(vd.data() + st, len, frac, o.words)) { o.ok = false; return; }
+    }
+}
+
+template <class F>
+void parallel_for(uint32_t n, int nthreads, F f)
+{
+    if (nthreads <= 1 || n < 2) { for (uint32_t i = 0; i < n; ++i) f(i, 0); return; }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t)
+        th.emplace_back([&, t] {
+            for (;;) {
+                uint32_t i = next.fetch_add(64);
+                if (i >= n) break;
+                for (uint32_t k = i; k < std::min(n, i + 64); ++k) f(k, t);
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
+                               const uint32_t* docs, const uint64_t* list_off, uint32_t nterms,
+                               int nthreads, exc_built** out)
+{
+    if (!out || !list_off || (nterms && !docs)) return set_error(EXC_ERR_ARG, "exc_build_index: null argument");
+    if (codec < EXC_LRC || codec > EXC_PARAPFD) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
+    if (codec != EXC_HS_SEGLRC && (seg < 64 || seg > 1024 || (seg & 63)))
+        return set_error(EXC_ERR_ARG, "exc_build_index: segment size must be a multiple of 64 in [64,1024]");
+    if (hs_target == 0) hs_target = 256;
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    for (uint32_t t = 0; t < nterms; ++t) {
+        uint64_t n = list_off[t + 1] - list_off[t];
+        if (list_off[t + 1] < list_off[t] || n > 0xffffffffull)
+            return set_error(EXC_ERR_ARG, "exc_build_index: bad list offsets");
+        for (uint64_t i = 1; i < n; ++i)
+            if (docs[list_off[t] + i] <= docs[list_off[t] + i - 1])
+                return set_error(EXC_ERR_ARG, "exc_build_index: list %u is not strictly increasing", t);
+        if (n && docs[list_off[t + 1] - 1] > 0x7fffffffu)
+            return set_error(EXC_ERR_ARG, "exc_build_index: docIDs must be < 2^31");
+    }
+    std::vector<ListOut> lo(nterms);
+    parallel_for(nterms, nthreads, [&](uint32_t t, int) {
+        encode_list(codec, seg, frac, hs_target, hs_shift, docs + list_off[t],
+                    (uint32_t)(list_off[t + 1] - list_off[t]), lo[t]);
+    });
+    for (uint32_t t = 0; t < nterms; ++t)
+        if (!lo[t].ok) return set_error(EXC_ERR_FORMAT, "exc_build_index: list %u has an unrepresentable block", t);
+
+    auto* b = new exc_built();
+    b->codec = codec;
+    b->seg = seg;
+    b->ind1.resize(nterms);
+    b->seg_num.resize(nterms);
+    std::vector<uint64_t> woff(nterms + 1, 0), soff(nterms + 1, 0), hoff(nterms + 1, 0);
+    for (uint32_t t = 0; t < nterms; ++t) {
+        woff[t + 1] = woff[t] + lo[t].words.size();
+        soff[t + 1] = soff[t] + lo[t].seg_offset.size();
+        hoff[t + 1] = hoff[t] + lo[t].hash.size();
+    }
+    if (soff[nterms] > 0xffffffffull || hoff[nterms] > 0xffffffffull) {
+        delete b;
+        return set_error(EXC_ERR_CAPACITY, "exc_build_index: more than 2^32 segments");
+    }
+    uint64_t S = soff[nterms];
+    b->ind2.resize(woff[nterms]);
+    b->seg_offset.resize(S);
+    if (codec != EXC_HS_SEGLRC) b->seg_head.resize(S);
+    if (codec == EXC_PARAPFD) b->seg_median.resize(S);
+    if (codec == EXC_SEGLRC || codec == EXC_HS_SEGLRC) { b->seg_info.resize(S); b->base.resize(S); }
+    if (codec == EXC_SEGLRC) b->list_info.resize(nterms);
+    if (codec == EXC_LRC) b->lrc.resize(nterms);
+    if (codec == EXC_HS_SEGLRC) { b->hash_offset.resize(nterms); b->hash_info.resize(hoff[nterms]); }
+    parallel_for(nterms, nthreads, [&](uint32_t t, int) {
+        ListOut& o = lo[t];
+        b->ind1[t].m_urlcount = (uint32_t)(list_off[t + 1] - list_off[t]);
+        b->ind1[t].m_off = woff[t] * 4;
+        b->seg_num[t] = (uint32_t)soff[t];
+        std::copy(o.words.begin(), o.words.end(), b->ind2.begin() + woff[t]);
+        std::copy(o.seg_offset.begin(), o.seg_offset.end(), b->seg_offset.begin() + soff[t]);
+        if (!o.seg_head.empty()) std::copy(o.seg_head.begin(), o.seg_head.end(), b->seg_head.begin() + soff[t]);
+        if (!o.seg_median.empty()) std::copy(o.seg_median.begin(), o.seg_median.end(), b->seg_median.begin() + soff[t]);
+        if (!o.seg_info.empty()) std::copy(o.seg_info.begin(), o.seg_info.end(), b->seg_info.begin() + soff[t]);
+        if (!o.base.empty()) std::copy(o.base.begin(), o.base.end(), b->base.begin() + soff[t]);
+        if (codec == EXC_SEGLRC) b->list_info[t] = o.list;
+        if (codec == EXC_LRC) b->lrc[t] = o.lrc;
+        if (codec == EXC_HS_SEGLRC) {
+            b->hash_offset[t] = (uint32_t)hoff[t];
+            std::copy(o.hash.begin(), o.hash.end(), b->hash_info.begin() + hoff[t]);
+        }
+        ListOut().words.swap(o.words);
+    });
+    *out = b;
+    return EXC_OK;
+}
+
+template <class T>
+static const T* ptr_or_null(const std::vector<T>& v) { return v.empty() ? nullptr : v.data(); }
+
+extern "C" int exc_built_view(const exc_built* b, exc_index_host* v)
+{
+    if (!b || !v) return set_error(EXC_ERR_ARG, "exc_built_view: null argument");
+    std::memset(v, 0, sizeof(*v));
+    v->codec = b->codec;
+    v->seg_size = b->seg;
+    v->nterms = (uint32_t)b->ind1.size();
+    v->ind1 = ptr_or_null(b->ind1);
+    v->ind2 = ptr_or_null(b->ind2);
+    v->ind2_words = b->ind2.size();
+    v->seg_num = ptr_or_null(b->seg_num);
+    v->seg_offset = ptr_or_null(b->seg_offset);
+    v->nseg = b->seg_offset.size();
+    v->seg_head = ptr_or_null(b->seg_head);
+    v->seg_median = ptr_or_null(b->seg_median);
+    v->lrc_info = ptr_or_null(b->lrc);
+    v->seg_info = ptr_or_null(b->seg_info);
+    v->seg_base = ptr_or_null(b->base);
+    v->list_info = ptr_or_null(b->list_info);
+    v->hash_offset = ptr_or_null(b->hash_offset);
+    v->hash_info = ptr_or_null(b->hash_info);
+    v->hash_words = b->hash_info.size();
+    return EXC_OK;
+}
+
+extern "C" void exc_built_free(exc_built* b) { delete b; }
+
+// ------------------------------------------------------------ index files
+
+static int write_file(const std::string& p, const void* data, size_t bytes)
+{
+    FILE* f = std::fopen(p.c_str(), "wb");
+    if (!f) return set_error(EXC_ERR_IO, "cannot create %s", p.c_str());
+    size_t w = bytes ? std::fwrite(data, 1, bytes, f) : 0;
+    std::fclose(f);
+    return w == bytes ? EXC_OK : set_error(EXC_ERR_IO, "short write on %s", p.c_str());
+}
+
+extern "C" int exc_index_save_files(const exc_index_host* h, const char* dir, const char* name)
+{
+    if (!h || !dir || !name) return set_error(EXC_ERR_ARG, "exc_index_save_files: null argument");
+    std::string p = std::string(dir) + "/" + name;
+    int rc;
+    uint32_t nt = h->nterms;
+    if ((rc = write_file(p + ".ind1", h->ind1, (size_t)nt * sizeof(exc_term_ind1)))) return rc;
+    if ((rc = write_file(p + ".ind2", h->ind2, h->ind2_words * 4))) return rc;
+    if ((rc = write_file(p + ".segNum", h->seg_num, (size_t)nt * 4))) return rc;
+    if ((rc = write_file(p + ".segOffset", h->seg_offset, h->nseg * 4))) return rc;
+    if (h->seg_head && (rc = write_file(p + ".segHead", h->seg_head, h->nseg * 4))) return rc;
+    if (h->seg_median && (rc = write_file(p + ".segMedian", h->seg_median, h->nseg * 4))) return rc;
+    if (h->lrc_info && (rc = write_file(p + ".regression_info", h->lrc_info, (size_t)nt * sizeof(exc_lrc_info)))) return rc;
+    if (h->seg_info && (rc = write_file(p + ".regression_info", h->seg_info, h->nseg * sizeof(exc_seg_info)))) return rc;
+    if (h->list_info && (rc = write_file(p + ".regression_info_list", h->list_info, (size_t)nt * sizeof(exc_seg_info)))) return rc;
+    if (h->seg_base && (rc = write_file(p + ".ind2_distance_base", h->seg_base, h->nseg * 4))) return rc;
+    if (h->hash_offset && (rc = write_file(p + ".HashSeg_offset", h->hash_offset, (size_t)nt * 4))) return rc;
+    if (h->hash_info && (rc = write_file(p + ".HashSeg_info", h->hash_info, h->hash_words * 4))) return rc;
+    return EXC_OK;
+}
+
+// ------------------------------------------------------------ synthetic lists
+
+namespace {
+
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    // uniform in [0, bound) by 64x64 multiply-high (Lemire)
+    uint32_t below(uint32_t bound) { return (uint32_t)(((unsigned __int128)next() * bound) >> 64); }
+};
+
+void synth_one(uint64_t seed, uint32_t U, uint32_t t, uint32_t df, uint32_t* out,
+               std::vector<uint64_t>& bitmap, std::vector<uint32_t>& tmp)
+{
+    SplitMix r{seed * 0x100000001B3ull ^ (0x9E3779B97F4A7C15ull * (uint64_t)(t + 1))};
+    if (df == 0) return;
+    if ((uint64_t)df * 256 >= U) {
+        // dense: Floyd's sampling over a bitmap, read back in order
+        bitmap.assign((U + 63) / 64, 0);
+        for (uint32_t j = U - df; j < U; ++j) {
+            uint32_t x = r.below(j + 1);
+            if (bitmap[x >> 6] >> (x & 63) & 1) x = j;
+            bitmap[x >> 6] |= 1ull << (x & 63);
+        }
+        uint32_t k = 0;
+        for (uint32_t w = 0; w < bitmap.size(); ++w)
+            for (uint64_t m = bitmap[w]; m; m &= m - 1) out[k++] = w * 64 + (uint32_t)__builtin_ctzll(m);
+        return;
+    }
+    // sparse: draw, sort, dedupe, top up until df distinct values
+    tmp.clear();
+    while (tmp.size() < df) {
+        size_t need = df - tmp.size();
+        for (size_t i = 0; i < need; ++i) tmp.push_back(r.below(U));
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    }
+    std::copy(tmp.begin(), tmp.end(), out);
+}
+
+}  // namespace
+
+extern "C" int exc_synth_lists(uint64_t seed, uint32_t universe, const uint32_t* df, uint32_t nterms,
+                               int nthreads, uint32_t* docs, const uint64_t* list_off)
+{
+    if (!df || !docs || !list_off || universe == 0) return set_error(EXC_ERR_ARG, "exc_synth_lists: bad argument");
+    for (uint32_t t = 0; t < nterms; ++t)
+        if (df[t] > universe || list_off[t + 1] - list_off[t] != df[t])
+            return set_error(EXC_ERR_ARG, "exc_synth_lists: df/list_off mismatch at term %u", t);
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::vector<uint64_t>> bm(nthreads);
+    std::vector<std::vector<uint32_t>> tmp(nthreads);
+    parallel_for(nterms, nthreads, [&](uint32_t t, int w) {
+        synth_one(seed, universe, t, df[t], docs + list_off[t], bm[w], tmp[w]);
+    });
+    return EXC_OK;
+}

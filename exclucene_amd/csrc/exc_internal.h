@@ -1,0 +1,122 @@
+// exc_internal.h -- device-side index layout and kernel launch interface shared
+// by the runtime (runtime.cpp) and the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/exclucene_amd.h"
+
+int set_error(int code, const char* fmt, ...);
+
+namespace exc {
+
+// One record per term, built at upload from .ind1/.segNum/.regression_info(_list)
+// /.HashSeg_offset, so a wave fetches everything it needs about a list with one
+// 64-byte scalar load.
+struct alignas(16) TermRec {
+    uint64_t woff;    // word offset of the list's first block in ind2
+    uint32_t n;       // list length (ind1.m_urlcount)
+    uint32_t segnum;  // first segment / bucket (.segNum)
+    uint32_t nseg;    // segments, or buckets (HS)
+    uint32_t aux;     // HS: index of start_0 in hash_info
+    float slope;      // LRC fSlope | SegLRC list-level slope
+    int32_t icpt;     // LRC iIntercept | SegLRC list-level intercept bits | HS shift
+    float range_l;    // LRC fSearchRange | SegLRC list-level nRangeLeft
+    float range_r;    // SegLRC list-level nRangeRight
+    uint32_t model;   // learned model slot, or kNoModel
+    uint32_t pad0;
+    uint64_t pad1;
+    uint64_t pad2;
+};
+static_assert(sizeof(TermRec) == 64, "TermRec is one 64-byte record");
+
+constexpr uint32_t kNoModel = 0xffffffffu;
+constexpr uint32_t kPlaPieces = 256;
+constexpr uint32_t kMlpHidden = 32;
+
+// Learned position model of one list (docID -> position), fp32.
+//   PLA: 256 pieces; key[p] = first docID of piece p, pos = a[p] * (x - key[p]) + b[p]
+//   MLP: 1-32-1 ReLU on x = docID * xscale; pos = (w2 . relu(w1 x + b1) + b2) * yscale
+// err = max |pos - true position| over the list, measured with the same
+// inference code the search uses.
+struct alignas(16) Model {
+    uint32_t kind;     // 1 PLA, 2 MLP
+    uint32_t n;        // list length the model was fitted to
+    float err;         // recorded max error (positions)
+    float xscale;
+    float yscale;
+    uint32_t param_off;  // float offset into the model parameter pool
+    uint32_t pad[2];
+};
+
+struct DevIndex {
+    const uint32_t* words;
+    const TermRec* terms;
+    const uint32_t* seg_off;
+    const uint32_t* seg_head;
+    const float4* fit;      // exc_seg_info rows (slope, icpt, rangeL bits, rangeR bits)
+    const int32_t* base;
+    const uint32_t* hash;   // HashSeg_info
+    const Model* models;
+    const float* mparams;
+    int32_t codec;
+    uint32_t seg;
+};
+
+struct QueryRec {
+    uint32_t toff;        // first sorted term of the query in qterms
+    uint32_t tc;          // term count
+    uint32_t first_unit;  // first work unit of the query
+    uint32_t nunits;      // work units (shortest-list chunks)
+    uint64_t short_off;   // offset of the query's shortest list in the stage buffer
+    uint64_t pad;
+};
+static_assert(sizeof(QueryRec) == 32, "QueryRec is 32 bytes");
+
+enum { SEARCH_BINARY = 0, SEARCH_FIT = 1, SEARCH_LEARNED = 2 };
+
+struct ProbeArgs {
+    DevIndex ix;
+    const QueryRec* queries;
+    const uint32_t* qterms;
+    const uint32_t* unit_q;
+    uint32_t nunits;
+    int32_t search;
+    int32_t tpe;
+    int32_t persistent;
+    uint32_t* stage;
+    uint32_t* ucount;
+    uint32_t* counter;
+};
+
+struct ScanArgs {
+    const QueryRec* queries;
+    const uint32_t* unit_q;
+    const uint32_t* ucount;
+    const uint32_t* stage;
+    uint32_t nunits;
+    uint32_t nq;
+    uint32_t seg;
+    uint32_t* uoff;     // per unit exclusive offset inside its scan block
+    uint32_t* bsum;     // per scan block sum, then exclusive block offsets (+ total)
+    uint32_t* result;
+    uint32_t* qcum;     // per query inclusive cumulative count
+};
+
+constexpr uint32_t kScanBlockItems = 4096;  // units per scan block (256 threads x 16)
+
+// launchers (kernels.hip); all return hipError_t of the launch
+hipError_t launch_reset(uint32_t* counter, hipStream_t s);
+hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s);
+hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
+hipError_t launch_decompress(const DevIndex& ix, const uint32_t* terms, const uint64_t* out_off,
+                             const uint32_t* unit_term, const uint32_t* unit_k, uint32_t nunits,
+                             uint32_t* out, hipStream_t s);
+uint32_t probe_waves_per_block();
+
+}  // namespace exc
+
+namespace exc {
+// persistent-launch sizing: resident probe workgroups per CU for this codec / segment size
+int probe_blocks_per_cu(int codec, uint32_t seg);
+}  // namespace exc

@@ -1,0 +1,775 @@
+// runtime.cpp -- host runtime behind the C ABI: contexts, device-resident
+// indexes, batch planning (CPUPreprocess), launches, result transfer.
+//
+// Replaces the reference's per-program globals and host stages
+// (GC = src/gpu/Compressed/src):
+//   allocateResource / releaseResource  GC/LRC/LRC_Intersection/LRC_Intersection.cu:77-122,382-446
+//   CPUPreprocess                       :126-246
+//   htodTransfer / kernelInvoke         :249-345
+//   dtohTransfer / verify               :348-379
+// Device memory: the compressed index is uploaded once and stays resident
+// (index arrays + one 64-byte record per term); per-batch buffers grow
+// geometrically and are reused, so a steady stream of batches allocates nothing.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "exc_format.h"
+#include "exc_internal.h"
+
+using namespace exc;
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char* exc_last_error(void) { return g_err; }
+
+#define HIP_TRY(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_error(EXC_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct exc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int cus = 0;
+    // options
+    int search = SEARCH_FIT;
+    int tpe = 0;
+    int persistent = 1;
+    int graph = 0;
+    int timing = 1;
+    // batch plan (host)
+    exc_index* ix = nullptr;
+    std::vector<QueryRec> queries;
+    std::vector<uint32_t> qterms, unit_q;
+    uint64_t short_sum = 0;
+    uint32_t nq = 0, nunits = 0, seg = 0;
+    int state = 0;  // 0 none, 1 planned, 2 on device, 3 invoked
+    // device batch buffers
+    DevBuf d_queries, d_qterms, d_unit_q, d_stage, d_ucount, d_uoff, d_bsum, d_result, d_qcum, d_counter;
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float probe_ms = 0, invoke_ms = 0;
+    // graph of the current plan
+    hipGraphExec_t gexec = nullptr;
+    int gtpe = -1;
+};
+
+struct exc_index {
+    exc_ctx* ctx = nullptr;
+    int codec = 0;
+    uint32_t seg = 256;
+    uint32_t nterms = 0;
+    uint64_t nseg = 0;
+    std::vector<uint32_t> n;       // list lengths (host copy for planning)
+    std::vector<DevBuf> bufs;
+    uint64_t bytes = 0;
+    DevIndex dix{};
+};
+
+namespace {
+
+int dev_alloc(exc_ctx* c, DevBuf& b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.p) return EXC_OK;
+    size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    if (want < 256) want = 256;
+    if (c->budget && c->used - b.bytes + want > c->budget) {
+        want = bytes < 256 ? 256 : bytes;
+        if (c->used - b.bytes + want > c->budget)
+            return set_error(EXC_ERR_BUDGET, "device-memory budget of %llu bytes exceeded",
+                             (unsigned long long)c->budget);
+    }
+    if (b.p) {
+        (void)hipFree(b.p);
+        c->used -= b.bytes;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    HIP_TRY(hipMalloc(&b.p, want));
+    b.bytes = want;
+    c->used += want;
+    return EXC_OK;
+}
+
+void dev_free(exc_ctx* c, DevBuf& b)
+{
+    if (b.p) {
+        (void)hipFree(b.p);
+        c->used -= b.bytes;
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void drop_graph(exc_ctx* c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+}
+
+template <class T>
+T* dptr(DevBuf& b) { return static_cast<T*>(b.p); }
+
+// upload `bytes` of src followed by kPadWords zero words
+int upload_padded(exc_index* ix, const void* src, size_t bytes, const void** out)
+{
+    DevBuf b;
+    size_t total = bytes + kPadWords * 4;
+    if (int rc = dev_alloc(ix->ctx, b, total)) return rc;
+    ix->bufs.push_back(b);
+    ix->bytes += b.bytes;
+    if (bytes) HIP_TRY(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(static_cast<char*>(b.p) + bytes, 0, total - bytes));
+    *out = b.p;
+    return EXC_OK;
+}
+
+uint32_t block_need_words(const uint32_t* blk, uint32_t len)
+{
+    uint32_t d = blk[0];
+    uint32_t lb = desc_lb(d), eb = desc_eb(d), hb = desc_hb(d), en = desc_en(d);
+    return 1 + words_for(len, lb) + (en ? words_for(en, eb) + words_for(en, hb) : 0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- context
+
+extern "C" int exc_ctx_create(int device, uint64_t budget, exc_ctx** out)
+{
+    if (!out) return set_error(EXC_ERR_ARG, "exc_ctx_create: null out");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return set_error(EXC_ERR_ARG, "exc_ctx_create: no device %d", device);
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new exc_ctx();
+    c->device = device;
+    c->budget = budget;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    if (c->cus <= 0) c->cus = 256;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_error(EXC_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    for (auto& e : c->ev) (void)hipEventCreate(&e);
+    *out = c;
+    return EXC_OK;
+}
+
+extern "C" void exc_ctx_destroy(exc_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (DevBuf* b : {&c->d_queries, &c->d_qterms, &c->d_unit_q, &c->d_stage, &c->d_ucount, &c->d_uoff,
+                      &c->d_bsum, &c->d_result, &c->d_qcum, &c->d_counter})
+        dev_free(c, *b);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
+{
+    if (!c) return set_error(EXC_ERR_ARG, "exc_ctx_set_option: null ctx");
+    switch (key) {
+    case EXC_OPT_SEARCH:
+        if (v < 0 || v > 2) return set_error(EXC_ERR_ARG, "search mode must be 0, 1 or 2");
+        c->search = (int)v;
+        break;
+    case EXC_OPT_TPE:
+        if (v != 0 && v != 1 && v != 64) return set_error(EXC_ERR_ARG, "tpe must be 0, 1 or 64");
+        c->tpe = (int)v;
+        break;
+    case EXC_OPT_PERSISTENT: c->persistent = v != 0; break;
+    case EXC_OPT_GRAPH: c->graph = v != 0; break;
+    case EXC_OPT_TIMING: c->timing = v != 0; break;
+    default: return set_error(EXC_ERR_ARG, "unknown option %d", key);
+    }
+    drop_graph(c);
+    return EXC_OK;
+}
+
+extern "C" int exc_ctx_last_times(exc_ctx* c, float* probe_ms, float* invoke_ms)
+{
+    if (!c) return set_error(EXC_ERR_ARG, "null ctx");
+    if (probe_ms) *probe_ms = c->probe_ms;
+    if (invoke_ms) *invoke_ms = c->invoke_ms;
+    return EXC_OK;
+}
+
+// ---------------------------------------------------------------- index
+
+extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index** out)
+{
+    if (!c || !h || !out) return set_error(EXC_ERR_ARG, "exc_index_upload: null argument");
+    const int codec = h->codec;
+    if (codec < EXC_LRC || codec > EXC_PARAPFD) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
+    const uint32_t seg = h->seg_size;
+    const bool hs = codec == EXC_HS_SEGLRC;
+    if (hs) {
+        // the HS kernels walk the shortest list in chunks of 128 elements
+    } else if (seg != 128 && seg != 256) {
+        return set_error(EXC_ERR_ARG, "segment size %u unsupported (128 or 256)", seg);
+    }
+    const uint32_t nt = h->nterms;
+    if (nt && (!h->ind1 || !h->seg_num || !h->seg_offset || !h->ind2))
+        return set_error(EXC_ERR_FORMAT, "index is missing .ind1/.ind2/.segNum/.segOffset");
+    if (!hs && !h->seg_head) return set_error(EXC_ERR_FORMAT, "index is missing .segHead");
+    if (codec == EXC_LRC && !h->lrc_info) return set_error(EXC_ERR_FORMAT, "LRC index is missing .regression_info");
+    if ((codec == EXC_SEGLRC || hs) && (!h->seg_info || !h->seg_base))
+        return set_error(EXC_ERR_FORMAT, "index is missing .regression_info/.ind2_distance_base");
+    if (hs && (!h->hash_offset || !h->hash_info))
+        return set_error(EXC_ERR_FORMAT, "HS index is missing .HashSeg_offset/.HashSeg_info");
+
+    // ---- validate the layout so that every device read stays inside its array
+    std::vector<TermRec> recs(nt);
+    const uint64_t W = h->ind2_words;
+    for (uint32_t t = 0; t < nt; ++t) {
+        const exc_term_ind1& r = h->ind1[t];
+        if (r.m_off % 4) return set_error(EXC_ERR_FORMAT, "term %u: .ind1 offset not word aligned", t);
+        const uint64_t w0 = r.m_off / 4;
+        const uint64_t w1 = (t + 1 < nt) ? h->ind1[t + 1].m_off / 4 : W;
+        if (w0 > w1 || w1 > W) return set_error(EXC_ERR_FORMAT, "term %u: list words outside .ind2", t);
+        const uint32_t n = r.m_urlcount;
+        TermRec& R = recs[t];
+        std::memset(&R, 0, sizeof R);
+        R.woff = w0;
+        R.n = n;
+        R.segnum = h->seg_num[t];
+        R.model = kNoModel;
+        uint32_t nseg;
+        std::vector<uint32_t> blen;
+        if (hs) {
+            const uint64_t ho = h->hash_offset[t];
+            if (ho + 3 > h->hash_words) return set_error(EXC_ERR_FORMAT, "term %u: hash table outside .HashSeg_info", t);
+            const uint32_t* ht = h->hash_info + ho;
+            nseg = ht[1];
+            if (nseg == 0 || ho + 3 + (uint64_t)nseg > h->hash_words + 1 || ht[0] > 31)
+                return set_error(EXC_ERR_FORMAT, "term %u: bad hash table header", t);
+            if (ho + 2 + (uint64_t)nseg >= h->hash_words) return set_error(EXC_ERR_FORMAT, "term %u: hash table truncated", t);
+            if (ht[2] != 0 || ht[2 + nseg] != n) return set_error(EXC_ERR_FORMAT, "term %u: bucket starts do not span the list", t);
+            for (uint32_t b = 0; b < nseg; ++b) {
+                if (ht[3 + b] < ht[2 + b]) return set_error(EXC_ERR_FORMAT, "term %u: bucket starts decrease", t);
+                blen.push_back(ht[3 + b] - ht[2 + b]);
+            }
+            R.aux = (uint32_t)(ho + 2);
+            R.icpt = (int32_t)ht[0];
+        } else {
+            nseg = (n + seg - 1) / seg;
+            for (uint32_t s = 0; s < nseg; ++s) blen.push_back(std::min(seg, n - s * seg));
+        }
+        R.nseg = nseg;
+        if ((uint64_t)R.segnum + nseg > h->nseg) return set_error(EXC_ERR_FORMAT, "term %u: segments outside .segOffset", t);
+        const uint64_t lw = w1 - w0;
+        for (uint32_t s = 0; s < nseg; ++s) {
+            const uint64_t o = h->seg_offset[R.segnum + s];
+            if (blen[s] == 0) { if (o > lw) return set_error(EXC_ERR_FORMAT, "term %u: bad offset of empty bucket", t); continue; }
+            if (o >= lw) return set_error(EXC_ERR_FORMAT, "term %u segment %u: offset outside the list", t, s);
+            const uint32_t* blk = h->ind2 + w0 + o;
+            uint32_t need = block_need_words(blk, blen[s]);
+            if (o + need > lw) return set_error(EXC_ERR_FORMAT, "term %u segment %u: block overruns the list", t, s);
+            if (codec != EXC_PARAPFD && desc_en(blk[0]))
+                return set_error(EXC_ERR_FORMAT,
+                                 "term %u segment %u: LRC-family blocks must have no exceptions (FRAC = 0)", t, s);
+        }
+        if (codec == EXC_LRC) {
+            R.slope = h->lrc_info[t].fSlope;
+            R.icpt = h->lrc_info[t].iIntercept;
+            R.range_l = R.range_r = h->lrc_info[t].fSearchRange;
+        } else if (codec == EXC_SEGLRC) {
+            exc_seg_info li{1.0f, 0.0f, 0xffffffffu, 0xffffffffu};
+            if (h->list_info) li = h->list_info[t];
+            R.slope = li.fSlope;
+            std::memcpy(&R.icpt, &li.fIntercept, 4);
+            R.range_l = (float)li.nRangeLeft;
+            R.range_r = (float)li.nRangeRight;
+            if (!h->list_info || !(li.fSlope > 0.0f)) { R.slope = 1.0f; R.range_l = R.range_r = 4.0e9f; }
+        }
+        if (codec == EXC_LRC && !(R.slope > 0.0f)) { R.slope = 1.0f; R.range_l = R.range_r = 4.0e9f; }
+    }
+
+    auto* ix = new exc_index();
+    ix->ctx = c;
+    ix->codec = codec;
+    ix->seg = hs ? 128 : seg;
+    ix->nterms = nt;
+    ix->nseg = h->nseg;
+    ix->n.resize(nt);
+    for (uint32_t t = 0; t < nt; ++t) ix->n[t] = recs[t].n;
+    HIP_TRY(hipSetDevice(c->device));
+    int rc = EXC_OK;
+    const void* p;
+    auto up = [&](const void* src, size_t bytes) -> const void* {
+        if (rc) return nullptr;
+        rc = upload_padded(ix, src, bytes, &p);
+        return rc ? nullptr : p;
+    };
+    ix->dix.codec = codec;
+    ix->dix.seg = ix->seg;
+    ix->dix.words = (const uint32_t*)up(h->ind2, W * 4);
+    ix->dix.terms = (const TermRec*)up(recs.data(), recs.size() * sizeof(TermRec));
+    ix->dix.seg_off = (const uint32_t*)up(h->seg_offset, h->nseg * 4);
+    if (!hs) ix->dix.seg_head = (const uint32_t*)up(h->seg_head, h->nseg * 4);
+    if (codec == EXC_SEGLRC || hs) {
+        ix->dix.fit = (const float4*)up(h->seg_info, h->nseg * sizeof(exc_seg_info));
+        ix->dix.base = (const int32_t*)up(h->seg_base, h->nseg * 4);
+    }
+    if (hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
+    if (rc) {
+        exc_index_free(ix);
+        return rc;
+    }
+    *out = ix;
+    return EXC_OK;
+}
+
+extern "C" void exc_index_free(exc_index* ix)
+{
+    if (!ix) return;
+    if (ix->ctx) {
+        (void)hipStreamSynchronize(ix->ctx->stream);
+        if (ix->ctx->ix == ix) {
+            ix->ctx->ix = nullptr;
+            ix->ctx->state = 0;
+            drop_graph(ix->ctx);
+        }
+        for (auto& b : ix->bufs) dev_free(ix->ctx, b);
+    }
+    delete ix;
+}
+
+extern "C" uint64_t exc_index_device_bytes(const exc_index* ix) { return ix ? ix->bytes : 0; }
+
+static int read_file(const std::string& p, std::vector<char>& out, bool optional)
+{
+    FILE* f = std::fopen(p.c_str(), "rb");
+    if (!f) {
+        if (optional) { out.clear(); return EXC_OK; }
+        return set_error(EXC_ERR_IO, "cannot open %s", p.c_str());
+    }
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    out.resize(sz > 0 ? (size_t)sz : 0);
+    size_t r = sz > 0 ? std::fread(out.data(), 1, (size_t)sz, f) : 0;
+    std::fclose(f);
+    return r == out.size() ? EXC_OK : set_error(EXC_ERR_IO, "short read on %s", p.c_str());
+}
+
+extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const char* dir, const char* name,
+                                    exc_index** out)
+{
+    if (!c || !dir || !name || !out) return set_error(EXC_ERR_ARG, "exc_index_load_files: null argument");
+    std::string p = std::string(dir) + "/" + name;
+    std::vector<char> ind1, ind2, segnum, segoff, head, median, reg, regl, base, hoff, hinfo;
+    int rc;
+    const bool hs = codec == EXC_HS_SEGLRC;
+    if ((rc = read_file(p + ".ind1", ind1, false)) || (rc = read_file(p + ".ind2", ind2, false)) ||
+        (rc = read_file(p + ".segNum", segnum, false)) || (rc = read_file(p + ".segOffset", segoff, false)))
+        return rc;
+    if (!hs && (rc = read_file(p + ".segHead", head, false))) return rc;
+    if ((rc = read_file(p + ".segMedian", median, true))) return rc;
+    if (codec != EXC_PARAPFD && (rc = read_file(p + ".regression_info", reg, false))) return rc;
+    if ((rc = read_file(p + ".regression_info_list", regl, true))) return rc;
+    if ((codec == EXC_SEGLRC || hs) && (rc = read_file(p + ".ind2_distance_base", base, false))) return rc;
+    if (hs && ((rc = read_file(p + ".HashSeg_offset", hoff, false)) || (rc = read_file(p + ".HashSeg_info", hinfo, false))))
+        return rc;
+    if (ind1.size() % sizeof(exc_term_ind1)) return set_error(EXC_ERR_FORMAT, ".ind1 size is not a multiple of 16");
+    exc_index_host h;
+    std::memset(&h, 0, sizeof h);
+    h.codec = codec;
+    h.seg_size = seg;
+    h.nterms = (uint32_t)(ind1.size() / sizeof(exc_term_ind1));
+    h.ind1 = (const exc_term_ind1*)ind1.data();
+    h.ind2 = (const uint32_t*)ind2.data();
+    h.ind2_words = ind2.size() / 4;
+    h.seg_num = (const uint32_t*)segnum.data();
+    h.seg_offset = (const uint32_t*)segoff.data();
+    h.nseg = segoff.size() / 4;
+    if (segnum.size() / 4 != h.nterms) return set_error(EXC_ERR_FORMAT, ".segNum length differs from .ind1");
+    if (!hs) {
+        if (head.size() != segoff.size()) return set_error(EXC_ERR_FORMAT, ".segHead length differs from .segOffset");
+        h.seg_head = (const uint32_t*)head.data();
+    }
+    if (median.size() == segoff.size() && !median.empty()) h.seg_median = (const uint32_t*)median.data();
+    if (codec == EXC_LRC) {
+        if (reg.size() != (size_t)h.nterms * sizeof(exc_lrc_info)) return set_error(EXC_ERR_FORMAT, "LRC .regression_info size mismatch");
+        h.lrc_info = (const exc_lrc_info*)reg.data();
+    } else if (codec != EXC_PARAPFD) {
+        if (reg.size() != h.nseg * sizeof(exc_seg_info) || base.size() != h.nseg * 4)
+            return set_error(EXC_ERR_FORMAT, ".regression_info / .ind2_distance_base size mismatch");
+        h.seg_info = (const exc_seg_info*)reg.data();
+        h.seg_base = (const int32_t*)base.data();
+        if (regl.size() == (size_t)h.nterms * sizeof(exc_seg_info)) h.list_info = (const exc_seg_info*)regl.data();
+    }
+    if (hs) {
+        if (hoff.size() / 4 != h.nterms) return set_error(EXC_ERR_FORMAT, ".HashSeg_offset length differs from .ind1");
+        h.hash_offset = (const uint32_t*)hoff.data();
+        h.hash_info = (const uint32_t*)hinfo.data();
+        h.hash_words = hinfo.size() / 4;
+    }
+    return exc_index_upload(c, &h, out);
+}
+
+// ---------------------------------------------------------------- planning
+
+namespace {
+
+// Terms ordered by ascending list length with a stable insertion sort
+// (LRC_Intersection.cu:157-179), work units = shortest-list segments
+// (:182-213; one wave per segment here, one 256-thread block there).
+int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq)
+{
+    c->ix = ix;
+    c->seg = ix->seg;
+    c->queries.resize(nq);
+    c->qterms.clear();
+    c->unit_q.clear();
+    c->short_sum = 0;
+    c->nq = nq;
+    drop_graph(c);
+    uint64_t units = 0;
+    std::vector<uint32_t> tmp;
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (qoff[q + 1] < qoff[q]) return set_error(EXC_ERR_ARG, "query %u: bad offsets", q);
+        const uint32_t tc = (uint32_t)(qoff[q + 1] - qoff[q]);
+        tmp.assign(qterms + qoff[q], qterms + qoff[q + 1]);
+        for (uint32_t t : tmp)
+            if (t >= ix->nterms) return set_error(EXC_ERR_ARG, "query %u: term %u out of range", q, t);
+        for (uint32_t i = 1; i < tc; ++i) {
+            uint32_t x = tmp[i], k = i;
+            while (k && ix->n[x] < ix->n[tmp[k - 1]]) { tmp[k] = tmp[k - 1]; --k; }
+            tmp[k] = x;
+        }
+        QueryRec& Q = c->queries[q];
+        std::memset(&Q, 0, sizeof Q);
+        Q.toff = (uint32_t)c->qterms.size();
+        Q.tc = tc;
+        Q.first_unit = (uint32_t)units;
+        const uint32_t n0 = tc ? ix->n[tmp[0]] : 0;
+        Q.nunits = (n0 + ix->seg - 1) / ix->seg;
+        Q.short_off = c->short_sum;
+        c->short_sum += n0;
+        c->qterms.insert(c->qterms.end(), tmp.begin(), tmp.end());
+        for (uint32_t k = 0; k < Q.nunits; ++k) c->unit_q.push_back(q);
+        units += Q.nunits;
+        if (units > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 work units in one batch");
+    }
+    c->nunits = (uint32_t)units;
+    if (c->short_sum > 0xffffffffull)
+        return set_error(EXC_ERR_CAPACITY, "summed shortest-list length %llu exceeds 2^32",
+                         (unsigned long long)c->short_sum);
+    c->state = 1;
+    return EXC_OK;
+}
+
+int upload_plan(exc_ctx* c)
+{
+    if (c->state < 1) return set_error(EXC_ERR_STATE, "htodTransfer before CPUPreprocess");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint32_t nblk = (c->nunits + kScanBlockItems - 1) / kScanBlockItems;
+    int rc;
+    if ((rc = dev_alloc(c, c->d_queries, std::max<size_t>(1, c->nq) * sizeof(QueryRec))) ||
+        (rc = dev_alloc(c, c->d_qterms, std::max<size_t>(1, c->qterms.size()) * 4)) ||
+        (rc = dev_alloc(c, c->d_unit_q, std::max<size_t>(1, c->nunits) * 4)) ||
+        (rc = dev_alloc(c, c->d_stage, std::max<uint64_t>(1, c->short_sum) * 4)) ||
+        (rc = dev_alloc(c, c->d_ucount, std::max<size_t>(1, c->nunits) * 4)) ||
+        (rc = dev_alloc(c, c->d_uoff, std::max<size_t>(1, c->nunits) * 4)) ||
+        (rc = dev_alloc(c, c->d_bsum, (size_t)(nblk + 1) * 4)) ||
+        (rc = dev_alloc(c, c->d_result, std::max<uint64_t>(1, c->short_sum) * 4)) ||
+        (rc = dev_alloc(c, c->d_qcum, std::max<size_t>(1, c->nq) * 4)) ||
+        (rc = dev_alloc(c, c->d_counter, 64)))
+        return rc;
+    drop_graph(c);
+    if (c->nq) HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice, c->stream));
+    if (!c->qterms.empty()) HIP_TRY(hipMemcpyAsync(c->d_qterms.p, c->qterms.data(), c->qterms.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (c->nunits) HIP_TRY(hipMemcpyAsync(c->d_unit_q.p, c->unit_q.data(), (size_t)c->nunits * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->state = 2;
+    return EXC_OK;
+}
+
+// Threads per short-list element when autotuned: random access (1 lane per
+// candidate) for the random-access codecs, wave-cooperative decode for ParaPFD.
+int pick_tpe(exc_ctx* c)
+{
+    if (c->tpe) return c->tpe;
+    return c->ix->codec == EXC_PARAPFD ? 64 : 1;
+}
+
+int enqueue_invoke(exc_ctx* c, int tpe, bool timed)
+{
+    exc_index* ix = c->ix;
+    ProbeArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.ix = ix->dix;
+    a.queries = dptr<QueryRec>(c->d_queries);
+    a.qterms = dptr<uint32_t>(c->d_qterms);
+    a.unit_q = dptr<uint32_t>(c->d_unit_q);
+    a.nunits = c->nunits;
+    a.search = c->search;
+    a.tpe = tpe;
+    a.persistent = c->persistent;
+    a.stage = dptr<uint32_t>(c->d_stage);
+    a.ucount = dptr<uint32_t>(c->d_ucount);
+    a.counter = dptr<uint32_t>(c->d_counter);
+    const uint32_t wpb = probe_waves_per_block();
+    uint32_t grid;
+    if (c->persistent) {
+        uint32_t per = (uint32_t)probe_blocks_per_cu(ix->codec, ix->seg);
+        grid = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)c->cus * per, (c->nunits + wpb - 1) / wpb));
+        HIP_TRY(launch_reset(a.counter, c->stream));
+    } else {
+        grid = std::max<uint32_t>(1, (c->nunits + wpb - 1) / wpb);
+    }
+    if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    if (c->nunits) HIP_TRY(launch_probe(a, grid, c->stream));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    ScanArgs s;
+    std::memset(&s, 0, sizeof s);
+    s.queries = a.queries;
+    s.unit_q = a.unit_q;
+    s.ucount = a.ucount;
+    s.stage = a.stage;
+    s.nunits = c->nunits;
+    s.nq = c->nq;
+    s.seg = ix->seg;
+    s.uoff = dptr<uint32_t>(c->d_uoff);
+    s.bsum = dptr<uint32_t>(c->d_bsum);
+    s.result = dptr<uint32_t>(c->d_result);
+    s.qcum = dptr<uint32_t>(c->d_qcum);
+    HIP_TRY(launch_scan(s, c->stream));
+    return EXC_OK;
+}
+
+int invoke(exc_ctx* c)
+{
+    if (c->state < 2 || !c->ix) return set_error(EXC_ERR_STATE, "kernelInvoke before htodTransfer");
+    HIP_TRY(hipSetDevice(c->device));
+    const int tpe = pick_tpe(c);
+    const bool timed = c->timing && !c->graph;
+    if (c->graph) {
+        if (!c->gexec || c->gtpe != tpe) {
+            drop_graph(c);
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_invoke(c, tpe, false);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            c->gtpe = tpe;
+        }
+        if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+        if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    } else {
+        if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        if (int rc = enqueue_invoke(c, tpe, timed)) return rc;
+        if (timed) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->probe_ms = c->invoke_ms = 0;
+    if (timed) {
+        (void)hipEventElapsedTime(&c->probe_ms, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&c->invoke_ms, c->ev[0], c->ev[3]);
+    } else if (c->timing) {
+        (void)hipEventElapsedTime(&c->invoke_ms, c->ev[0], c->ev[3]);
+    }
+    c->state = 3;
+    return EXC_OK;
+}
+
+int fetch(exc_ctx* c, uint32_t* cum, uint32_t* res, uint64_t cap, uint64_t* nres)
+{
+    if (c->state < 3) return set_error(EXC_ERR_STATE, "dtohTransfer before kernelInvoke");
+    HIP_TRY(hipSetDevice(c->device));
+    uint64_t total = 0;
+    if (c->nq) {
+        HIP_TRY(hipMemcpy(cum, c->d_qcum.p, c->nq * 4, hipMemcpyDeviceToHost));
+        total = cum[c->nq - 1];
+    }
+    if (nres) *nres = total;
+    if (total > cap) return set_error(EXC_ERR_CAPACITY, "result buffer holds %llu, need %llu",
+                                      (unsigned long long)cap, (unsigned long long)total);
+    if (total) HIP_TRY(hipMemcpy(res, c->d_result.p, total * 4, hipMemcpyDeviceToHost));
+    return EXC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- stages
+
+extern "C" int exc_CPUPreprocess(exc_ctx* c, exc_index* ix, const uint8_t** ptr_in, const uint8_t* ptr_end,
+                                 uint64_t threshold, exc_batchInfo* bi)
+{
+    if (!c || !ix || !ptr_in || !*ptr_in || !ptr_end || !bi) return set_error(EXC_ERR_ARG, "exc_CPUPreprocess: null argument");
+    const uint8_t* p = *ptr_in;
+    std::vector<uint32_t> terms;
+    std::vector<uint64_t> off{0};
+    uint64_t sum = 0;
+    // LRC_Intersection.cu:140: keep adding queries while the running shortest
+    // sum is <= threshold (the query that crosses it joins the batch).
+    while (sum <= threshold && p < ptr_end) {
+        if (ptr_end - p < 4) return set_error(EXC_ERR_ARG, "truncated query header");
+        uint32_t tn;
+        std::memcpy(&tn, p, 4);
+        p += 4;
+        if ((uint64_t)(ptr_end - p) < (uint64_t)tn * 4) return set_error(EXC_ERR_ARG, "truncated query terms");
+        uint32_t mn = 0xffffffffu;
+        for (uint32_t i = 0; i < tn; ++i) {
+            uint32_t t;
+            std::memcpy(&t, p + 4 * i, 4);
+            if (t >= ix->nterms) return set_error(EXC_ERR_ARG, "query term %u out of range", t);
+            terms.push_back(t);
+            mn = std::min(mn, ix->n[t]);
+        }
+        p += 4ull * tn;
+        off.push_back(terms.size());
+        if (tn) sum += mn;
+    }
+    const uint32_t nq = (uint32_t)(off.size() - 1);
+    if (int rc = plan_queries(c, ix, terms.data(), off.data(), nq)) return rc;
+    *ptr_in = p;
+    bi->blockNum = c->nunits;
+    bi->constantUsedInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, nq * sizeof(QueryRec) + c->qterms.size() * 4 + (uint64_t)c->nunits * 4);
+    bi->ucntShortest_sum = (uint32_t)c->short_sum;
+    bi->nTotalQueryNum = nq;
+    bi->nLinearInfoInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, c->qterms.size() * sizeof(TermRec));
+    return EXC_OK;
+}
+
+extern "C" int exc_htodTransfer(exc_ctx* c, const exc_batchInfo* bi)
+{
+    if (!c || !bi) return set_error(EXC_ERR_ARG, "exc_htodTransfer: null argument");
+    if (bi->nTotalQueryNum != c->nq) return set_error(EXC_ERR_STATE, "batchInfo does not match the planned batch");
+    return upload_plan(c);
+}
+
+extern "C" int exc_kernelInvoke(exc_ctx* c, const exc_batchInfo* bi)
+{
+    if (!c || !bi) return set_error(EXC_ERR_ARG, "exc_kernelInvoke: null argument");
+    if (bi->nTotalQueryNum != c->nq) return set_error(EXC_ERR_STATE, "batchInfo does not match the planned batch");
+    return invoke(c);
+}
+
+extern "C" int exc_dtohTransfer(exc_ctx* c, const exc_batchInfo* bi, uint32_t* cum, uint32_t* res, uint64_t cap)
+{
+    if (!c || !bi || (!cum && bi->nTotalQueryNum)) return set_error(EXC_ERR_ARG, "exc_dtohTransfer: null argument");
+    if (bi->nTotalQueryNum != c->nq) return set_error(EXC_ERR_STATE, "batchInfo does not match the planned batch");
+    return fetch(c, cum, res, cap, nullptr);
+}
+
+// LRC_Intersection.cu:354-379
+extern "C" int exc_verify(const exc_batchInfo* bi, const uint32_t* cum, const uint32_t* res, exc_checkSum* cs)
+{
+    if (!bi || !cs || (bi->nTotalQueryNum && !cum)) return set_error(EXC_ERR_ARG, "exc_verify: null argument");
+    for (uint32_t i = 0; i < bi->nTotalQueryNum; ++i) {
+        const uint32_t b = i ? cum[i - 1] : 0, n = cum[i] - b;
+        if (n) {
+            cs->checkSum2 += n;
+            cs->checkSum3 ^= res[b];
+            cs->checkSum3 ^= res[b + n - 1];
+            cs->checkSum3 ^= res[b + n / 2];
+        } else {
+            ++cs->checkSum1;
+        }
+    }
+    return EXC_OK;
+}
+
+extern "C" int exc_plan(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq,
+                        exc_batchInfo* bi)
+{
+    if (!c || !ix || !qoff || (nq && !qterms && qoff[nq]) || !bi) return set_error(EXC_ERR_ARG, "exc_plan: null argument");
+    if (int rc = plan_queries(c, ix, qterms, qoff, nq)) return rc;
+    bi->blockNum = c->nunits;
+    bi->constantUsedInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, nq * sizeof(QueryRec) + c->qterms.size() * 4 + (uint64_t)c->nunits * 4);
+    bi->ucntShortest_sum = (uint32_t)c->short_sum;
+    bi->nTotalQueryNum = nq;
+    bi->nLinearInfoInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, c->qterms.size() * sizeof(TermRec));
+    return upload_plan(c);
+}
+
+extern "C" int exc_intersect(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq,
+                             uint32_t* cum, uint32_t* res, uint64_t cap, uint64_t* nres)
+{
+    exc_batchInfo bi;
+    if (int rc = exc_plan(c, ix, qterms, qoff, nq, &bi)) return rc;
+    if (int rc = invoke(c)) return rc;
+    return fetch(c, cum, res, cap, nres);
+}
+
+// ---------------------------------------------------------------- decompress
+
+extern "C" int exc_decompress(exc_ctx* c, exc_index* ix, const uint32_t* terms, uint32_t nt, uint32_t* out,
+                              uint64_t cap, uint64_t* n_out)
+{
+    if (!c || !ix || (nt && !terms)) return set_error(EXC_ERR_ARG, "exc_decompress: null argument");
+    std::vector<uint64_t> off(nt + 1, 0);
+    std::vector<uint32_t> ut, uk;
+    for (uint32_t i = 0; i < nt; ++i) {
+        if (terms[i] >= ix->nterms) return set_error(EXC_ERR_ARG, "term %u out of range", terms[i]);
+        const uint32_t n = ix->n[terms[i]];
+        off[i + 1] = off[i] + n;
+        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; ++k) { ut.push_back(i); uk.push_back(k); }
+    }
+    if (n_out) *n_out = off[nt];
+    if (off[nt] > cap) return set_error(EXC_ERR_CAPACITY, "output holds %llu, need %llu",
+                                        (unsigned long long)cap, (unsigned long long)off[nt]);
+    if (!off[nt]) return EXC_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    DevBuf dt, doff, dut, duk, dout;
+    int rc;
+    if ((rc = dev_alloc(c, dt, nt * 4)) || (rc = dev_alloc(c, doff, (nt + 1) * 8)) ||
+        (rc = dev_alloc(c, dut, ut.size() * 4)) || (rc = dev_alloc(c, duk, uk.size() * 4)) ||
+        (rc = dev_alloc(c, dout, off[nt] * 4))) {
+        for (DevBuf* b : {&dt, &doff, &dut, &duk, &dout}) dev_free(c, *b);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(dt.p, terms, nt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(doff.p, off.data(), (nt + 1) * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dut.p, ut.data(), ut.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(duk.p, uk.data(), uk.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_decompress(ix->dix, dptr<uint32_t>(dt), dptr<uint64_t>(doff), dptr<uint32_t>(dut), dptr<uint32_t>(duk),
+                              (uint32_t)ut.size(), dptr<uint32_t>(dout), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dout.p, off[nt] * 4, hipMemcpyDeviceToHost);
+    for (DevBuf* b : {&dt, &doff, &dut, &duk, &dout}) dev_free(c, *b);
+    if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "exc_decompress: %s", hipGetErrorString(e));
+    return EXC_OK;
+}

@@ -1,0 +1,173 @@
+/*
+ * exclucene_amd.h -- C ABI of the MI355X (gfx950) posting-list decode +
+ * intersection library.
+ *
+ * The reference (ExCLucene src/gpu) has no library: each codec is a standalone
+ * program whose host functions share file-scope globals.  This header replaces
+ * those per-program functions with one context-handle API over the same
+ * compressed index layouts.  Every entry point names the reference interface it
+ * replaces (paths relative to the reference root; GC = src/gpu/Compressed/src).
+ *
+ * Conventions: plain pointers and sizes, no exit() -- every call returns an
+ * EXC_* status and exc_last_error() holds a message for the calling thread.
+ * One exc_ctx per host thread (a context owns one HIP stream and its buffers).
+ */
+#ifndef EXCLUCENE_AMD_H
+#define EXCLUCENE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference calls exit(); LRC_Intersection_kernel.cu:7-14) */
+#define EXC_OK            0
+#define EXC_ERR_ARG      (-1)  /* bad argument / malformed query stream */
+#define EXC_ERR_DEVICE   (-2)  /* HIP runtime error */
+#define EXC_ERR_CAPACITY (-3)  /* batch or output larger than the buffers (LRC_Intersection.cu:218-226) */
+#define EXC_ERR_FORMAT   (-4)  /* index arrays inconsistent with the codec's layout */
+#define EXC_ERR_IO       (-5)  /* index file missing / short */
+#define EXC_ERR_BUDGET   (-6)  /* device-memory budget exceeded */
+#define EXC_ERR_STATE    (-7)  /* call out of order (e.g. dtoh before kernelInvoke) */
+#define EXC_ERR_KERNEL   (-8)  /* a kernel reported an internal limit (bounded wait expired) */
+
+/* ---- codecs: one per reference program family */
+enum {
+    EXC_LRC = 0,       /* GC/LRC       : one global fit per list          */
+    EXC_SEGLRC = 1,    /* GC/SegLRC    : per-segment fit + base           */
+    EXC_HS_SEGLRC = 2, /* GC/HS_SegLRC : hash buckets on top docID bits   */
+    EXC_PARAPFD = 3    /* GC/ParaPFD   : d-gaps in ParaPFor blocks        */
+};
+
+/* ---- reference structs, byte-identical */
+/* .ind1 row (GC/LRC/LRC_Compression/main.cpp:34-37), 16 bytes with padding. */
+typedef struct { uint32_t m_urlcount; uint64_t m_off; } exc_term_ind1;
+/* LRC .regression_info row (GC/LRC/LRC_Intersection/LRC_Intersection_kernel.cu:89-93). */
+typedef struct { float fSlope; int32_t iIntercept; float fSearchRange; } exc_lrc_info;
+/* SegLRC/HS .regression_info row (GC/SegLRC/SegLRC_Intersection/SegLRC_Intersection_kernel.cu:89-94). */
+typedef struct { float fSlope; float fIntercept; uint32_t nRangeLeft; uint32_t nRangeRight; } exc_seg_info;
+/* batchInfo (GC/LRC/LRC_Intersection/LRC_Intersection_kernel.cu:68-74). */
+typedef struct {
+    uint32_t blockNum;           /* work units (query, shortest-list segment) in the batch */
+    uint32_t constantUsedInByte; /* bytes of batch descriptor sent to the device           */
+    uint32_t ucntShortest_sum;   /* sum of shortest-list lengths                           */
+    uint32_t nTotalQueryNum;     /* queries in the batch                                   */
+    uint32_t nLinearInfoInByte;  /* bytes of per-term records referenced by the batch      */
+} exc_batchInfo;
+/* checkSum (LRC_Intersection_kernel.cu:79-83). */
+typedef struct { uint64_t checkSum1, checkSum2, checkSum3; } exc_checkSum;
+
+/* Host view of one compressed index, array for array the reference's files:
+ * .ind1 .ind2 .segNum .segOffset .segHead .segMedian .regression_info
+ * .regression_info_list .ind2_distance_base .HashSeg_offset .HashSeg_info
+ * (GC/<codec>/<codec>_Compression/main.cpp, GC/<codec>/<codec>_Generator).  Arrays a codec does not
+ * use are NULL.  seg_size is the segment length (the reference fixes 256). */
+typedef struct {
+    int32_t codec;
+    uint32_t seg_size;
+    uint32_t nterms;
+    uint32_t reserved;
+    const exc_term_ind1 *ind1;     /* nterms; m_off in bytes into ind2            */
+    const uint32_t *ind2;          /* compressed words                            */
+    uint64_t ind2_words;
+    const uint32_t *seg_num;       /* nterms: first segment / bucket of the term  */
+    const uint32_t *seg_offset;    /* nseg: block word offset inside its list     */
+    uint64_t nseg;
+    const uint32_t *seg_head;      /* nseg: first docID (LRC, SegLRC, ParaPFD)    */
+    const uint32_t *seg_median;    /* nseg: ParaPFD only (optional)               */
+    const exc_lrc_info *lrc_info;  /* nterms: LRC                                 */
+    const exc_seg_info *seg_info;  /* nseg: SegLRC / HS                           */
+    const int32_t *seg_base;       /* nseg: SegLRC / HS (.ind2_distance_base)     */
+    const exc_seg_info *list_info; /* nterms: SegLRC list-level fit (optional)    */
+    const uint32_t *hash_offset;   /* nterms: HS                                  */
+    const uint32_t *hash_info;     /* HS tables [shift][nb][start_0..start_nb]    */
+    uint64_t hash_words;
+} exc_index_host;
+
+typedef struct exc_ctx exc_ctx;
+typedef struct exc_index exc_index;
+typedef struct exc_built exc_built;
+
+/* ---- context (replaces allocateResource/releaseResource globals,
+ *      GC/LRC/LRC_Intersection/LRC_Intersection.cu:77-122,382-446) */
+int exc_ctx_create(int device, uint64_t mem_budget_bytes, exc_ctx **out);
+void exc_ctx_destroy(exc_ctx *ctx);
+const char *exc_last_error(void);
+
+/* options (no reference counterpart: the reference fixes these with #define) */
+enum {
+    EXC_OPT_SEARCH = 0,     /* 0 plain binary search, 1 fit window + fallback, 2 learned window + fallback */
+    EXC_OPT_TPE = 1,        /* threads per short-list element: 0 autotuned, 1 random access, 64 wave-cooperative decode */
+    EXC_OPT_PERSISTENT = 2, /* 1: persistent work-queue launch, 0: one wave per work unit */
+    EXC_OPT_GRAPH = 3,      /* 1: replay kernelInvoke as a captured hipGraph */
+    EXC_OPT_TIMING = 4      /* 1: hipEvent timing around each kernel of kernelInvoke */
+};
+int exc_ctx_set_option(exc_ctx *ctx, int key, int64_t value);
+/* Times (ms) of the last kernelInvoke: probe kernel and the whole invoke. */
+int exc_ctx_last_times(exc_ctx *ctx, float *probe_ms, float *invoke_ms);
+
+/* ---- index residency (replaces as_load_atind + the whole-.ind2 cudaMemcpy,
+ *      GC/LRC/LRC_Intersection/LRC_Intersection.cu:50-74,94-106) */
+int exc_index_upload(exc_ctx *ctx, const exc_index_host *h, exc_index **out);
+int exc_index_load_files(exc_ctx *ctx, int codec, uint32_t seg_size, const char *dir,
+                         const char *name, exc_index **out);
+void exc_index_free(exc_index *ix);
+uint64_t exc_index_device_bytes(const exc_index *ix);
+
+/* ---- batch stages, one call per reference stage
+ *      (GC/LRC/LRC_Intersection/LRC_Intersection.cu:126,249,256,348,354).
+ * CPUPreprocess parses the .query stream {u32 tnum; u32 tno[tnum]}... from
+ * *ptr_in, stops once the summed shortest-list length exceeds threshold (as the
+ * reference does), sorts each query's terms by ascending length and plans the
+ * work units.  tnum is the full 32-bit word (the reference reads its low byte). */
+int exc_CPUPreprocess(exc_ctx *ctx, exc_index *ix, const uint8_t **ptr_in,
+                      const uint8_t *ptr_end, uint64_t threshold, exc_batchInfo *bi);
+int exc_htodTransfer(exc_ctx *ctx, const exc_batchInfo *bi);
+int exc_kernelInvoke(exc_ctx *ctx, const exc_batchInfo *bi);
+/* ucntResult[q] = inclusive cumulative result count (the reference's h_ucntResult);
+ * result = per-query results, ascending, concatenated in query order. */
+int exc_dtohTransfer(exc_ctx *ctx, const exc_batchInfo *bi, uint32_t *ucntResult,
+                     uint32_t *result, uint64_t result_cap);
+int exc_verify(const exc_batchInfo *bi, const uint32_t *ucntResult, const uint32_t *result,
+               exc_checkSum *cs);
+
+/* All stages in one call for an in-memory query list (terms of query q are
+ * qterms[qoff[q] .. qoff[q+1])). */
+int exc_intersect(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
+                  uint32_t nq, uint32_t *ucntResult, uint32_t *result, uint64_t result_cap,
+                  uint64_t *n_results);
+/* Plan + upload a query list without running it (replay with exc_kernelInvoke). */
+int exc_plan(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
+             uint32_t nq, exc_batchInfo *bi);
+
+/* ---- decompression programs (GC/<codec>/<codec>_Decompression):
+ * decode whole lists into out (concatenated in the given order). */
+int exc_decompress(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, uint32_t nterms,
+                   uint32_t *out, uint64_t out_cap, uint64_t *n_out);
+
+/* ---- index builder (replaces the *_Generator and *_Compression programs,
+ *      GC/LRC/LRC_Generator/LRC_Generator.cu:93-287, GC/LRC/LRC_Compression/main.cpp:86-238
+ *      and their SegLRC / HS_SegLRC / ParaPFD counterparts).
+ * docs: concatenated sorted distinct docID lists, list t = docs[list_off[t] .. list_off[t+1]).
+ * frac: ParaPFor exception fraction (FRAC); hs_target: expected bucket size (256);
+ * hs_shift: 0 = adaptive shift, else a fixed shift for every list. */
+int exc_build_index(int codec, uint32_t seg_size, float frac, uint32_t hs_target,
+                    uint32_t hs_shift, const uint32_t *docs, const uint64_t *list_off,
+                    uint32_t nterms, int nthreads, exc_built **out);
+int exc_built_view(const exc_built *b, exc_index_host *view);
+void exc_built_free(exc_built *b);
+/* Write / read the reference's file set <dir>/<name>.{ind1,ind2,segNum,...}. */
+int exc_index_save_files(const exc_index_host *h, const char *dir, const char *name);
+
+/* ---- synthetic corpus (stands in for the reference's GP/ preprocessing tools):
+ * list t = df[t] distinct docIDs drawn uniformly from [0, universe), sorted,
+ * written to docs[list_off[t] ..].  Deterministic in (seed, t). */
+int exc_synth_lists(uint64_t seed, uint32_t universe, const uint32_t *df, uint32_t nterms,
+                    int nthreads, uint32_t *docs, const uint64_t *list_off);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

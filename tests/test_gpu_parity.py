@@ -1,0 +1,77 @@
+"""GPU parity: the HIP decode + intersect path through the C ABI against the
+CPU oracle (bit-exact docID sets, cumulative counts and checksums)."""
+import numpy as np
+import pytest
+
+import exclucene_amd as ea
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [1, 2, 7, 64, 127, 128, 129, 255, 256, 257, 1000, 4000, 20000, 150000, 600000]
+
+
+def _lists(seed, universe=1 << 24):
+    rng = np.random.default_rng(seed)
+    out = [np.sort(rng.choice(universe, n, replace=False)).astype(np.uint32) for n in LENGTHS]
+    # a dense run and a list entirely above / below the others
+    out.append(np.arange(5000, 9000, dtype=np.uint32))
+    out.append(np.arange(universe - 300, universe, dtype=np.uint32))
+    out.append(np.arange(0, 50, dtype=np.uint32))
+    return out
+
+
+def _queries(nl, seed):
+    rng = np.random.default_rng(seed)
+    qs = [[i] for i in range(nl)]
+    qs += [[i, j] for i in range(nl) for j in range(nl) if i != j and (i + j) % 3 == 0]
+    qs += [[int(x) for x in rng.choice(nl, int(rng.integers(3, 6)), replace=False)] for _ in range(40)]
+    qs += [[13, 13], [14, 13, 12, 11], [nl - 1, nl - 2], [nl - 3, 14], [], [0, 14, 14]]
+    return qs
+
+
+CASES = [(ea.LRC, 256, 0.0, 0), (ea.LRC, 128, 0.0, 0), (ea.SEGLRC, 128, 0.0, 0), (ea.SEGLRC, 256, 0.0, 0),
+         (ea.HS_SEGLRC, 128, 0.0, 0), (ea.HS_SEGLRC, 128, 0.0, 13), (ea.PARAPFD, 128, 0.1, 0),
+         (ea.PARAPFD, 256, 0.1, 0), (ea.PARAPFD, 128, 0.0, 0)]
+
+
+@pytest.mark.parametrize("codec,seg,frac,shift", CASES)
+def test_intersect_matches_oracle(ctx, codec, seg, frac, shift):
+    lists = _lists(11)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, codec, seg=seg, frac=frac, hs_shift=shift)
+    ref = oracle.build_index(lists, codec, seg=seg, frac=frac, hs_shift=shift)
+    qs = _queries(len(lists), 3)
+    ocum, ores = oracle.eval_queries(ref, qs)
+    dix = ctx.upload(built)
+    try:
+        tpes = [1, 64] if codec in (ea.LRC, ea.SEGLRC) else [0]
+        for search in (0, 1):
+            for tpe in tpes:
+                for persistent in (0, 1):
+                    ctx.set_option(ea.OPT_SEARCH, search)
+                    ctx.set_option(ea.OPT_TPE, tpe)
+                    ctx.set_option(ea.OPT_PERSISTENT, persistent)
+                    print("case codec=%d seg=%d search=%d tpe=%d persistent=%d" % (codec, seg, search, tpe, persistent), flush=True)
+                    cum, res = ctx.intersect(dix, qs)
+                    assert np.array_equal(cum, ocum), (search, tpe, persistent)
+                    assert np.array_equal(res, ores), (search, tpe, persistent)
+    finally:
+        ctx.set_option(ea.OPT_TPE, 0)
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        dix.close()
+
+
+@pytest.mark.parametrize("codec,seg,frac,shift", CASES)
+def test_decompress_roundtrip(ctx, codec, seg, frac, shift):
+    lists = _lists(12)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, codec, seg=seg, frac=frac, hs_shift=shift)
+    dix = ctx.upload(built)
+    try:
+        terms = list(range(len(lists)))[::-1]
+        out = ctx.decompress(dix, terms)
+        assert np.array_equal(out, np.concatenate([lists[t] for t in terms]))
+    finally:
+        dix.close()
