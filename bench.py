@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Benchmark: exact k-term intersection queries/s on one MI355X.
+
+Workload (C2): universe 2^25, 100,000 terms with df_k = min(2^23, 2^25/k),
+lists uniform without replacement, encoded as SegLRC-128 (headline) and
+ParaPFD-128 (FRAC 0.1); a batch of 1,000 queries of 2-5 terms drawn
+proportionally to df.  One step = one kernelInvoke of the whole batch with the
+compressed index and the planned batch resident in HBM (probe kernel + scan +
+scatter).  Prints one JSON line.
+
+Also measured: the plain single-query GPU path (one query per launch, binary
+search, no work queue, host planning and PCIe included), the ParaPFD batch,
+the probe kernel's roofline against 8 TB/s, and the CPU decode-and-intersect
+baseline (the oracle, on the host cores) on a bounded sample of the batch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+META = {0: 8, 1: 28, 2: 32, 3: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(built, docs, off, queries):
+    """Bytes an exact block-decoding algorithm must move per batch: compressed
+    words + side data of the shortest list, of every longer-list block that
+    holds a probe's target position, and 4 B per result."""
+    ucnt = built.ind1["m_urlcount"].astype(np.int64)
+    wstart = (built.ind1["m_off"] // 4).astype(np.int64)
+    wend = np.append(wstart[1:], len(built.ind2))
+    segnum = built.seg_num.astype(np.int64)
+    codec = built.codec
+    meta = META[codec]
+    segoff = built.seg_offset.astype(np.int64)
+
+    def nseg(t):
+        if codec == 2:
+            return int(built.hash_info[built.hash_offset[t] + 1])
+        return (int(ucnt[t]) + built.seg - 1) // built.seg
+
+    def block_words(t, blocks):
+        s0, ns = int(segnum[t]), nseg(t)
+        o = segoff[s0:s0 + ns]
+        ends = np.append(o[1:], wend[t] - wstart[t])
+        return int((ends[blocks] - o[blocks]).sum())
+
+    tot = full = 0
+    for q in queries:
+        if not q:
+            continue
+        ts = sorted(q, key=lambda t: ucnt[t])  # stable: ascending length
+        for t in ts:
+            full += int(wend[t] - wstart[t]) * 4 + nseg(t) * meta + (12 if codec == 0 else 0)
+        t0 = ts[0]
+        tot += int(wend[t0] - wstart[t0]) * 4 + nseg(t0) * meta + (12 if codec == 0 else 0)
+        surv = docs[int(off[t0]):int(off[t0 + 1])]
+        for t in ts[1:]:
+            if surv.size == 0:
+                break
+            L = docs[int(off[t]):int(off[t + 1])]
+            if codec == 0:
+                tot += 12
+            if codec == 2:
+                h = int(built.hash_offset[t])
+                b = (surv >> built.hash_info[h]).astype(np.int64)
+                b = b[b < int(built.hash_info[h + 1])]
+            else:
+                s0 = int(segnum[t])
+                heads = built.seg_head[s0:s0 + nseg(t)]
+                b = np.searchsorted(heads, surv, side="right").astype(np.int64) - 1
+                b = b[b >= 0]
+            ub = np.unique(b)
+            tot += block_words(t, ub) * 4 + len(ub) * meta
+            pos = np.searchsorted(L, surv)
+            ok = pos < len(L)
+            ok[ok] = L[pos[ok]] == surv[ok]
+            surv = surv[ok]
+        tot += 4 * len(surv)
+        full += 4 * len(surv)
+    return tot, full
+
+
+def time_batch(ctx, dix, queries, steps, warmup):
+    ctx.plan(dix, queries)
+    bi = ctx._bi
+    for _ in range(warmup):
+        ctx.kernelInvoke(bi)
+    probe, invoke = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.kernelInvoke(bi)
+        p, i = ctx.last_times()
+        probe.append(p)
+        invoke.append(i)
+    dt = time.perf_counter() - t0
+    return dt, np.array(probe), np.array(invoke)
+
+
+def cpu_baseline(built, queries, budget_s=12.0):
+    """The oracle's block-skipping decode + intersect on the host cores over a
+    bounded prefix of the batch (the cpu_baseline leg)."""
+    import oracle
+    ix = oracle.Index(codec=built.codec, seg=built.seg, ucnt=np.ascontiguousarray(built.ind1["m_urlcount"]),
+                      uwlist=np.ascontiguousarray(built.ind1["m_off"] // 4), segnum=built.seg_num,
+                      words=built.ind2, total_words=len(built.ind2), seg_off=built.seg_offset,
+                      seg_head=built.seg_head, seg_median=built.seg_median, lrc=built.lrc_info,
+                      fit=built.seg_info, base=built.seg_base, hash_off=built.hash_offset,
+                      hash_info=built.hash_info)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    done, t_used = 0, 0.0
+    chunk = max(threads, 16)
+    while done < len(queries) and t_used < budget_s:
+        part = queries[done:done + chunk]
+        t0 = time.perf_counter()
+        oracle.eval_queries(ix, part, nthreads=threads)
+        t_used += time.perf_counter() - t0
+        done += len(part)
+        chunk = min(chunk * 2, 256)
+    qps = done / t_used if t_used > 0 else 0.0
+    # single-thread figure on a smaller prefix
+    n1 = max(1, min(done, 64))
+    t0 = time.perf_counter()
+    oracle.eval_queries(ix, queries[:n1], nthreads=1)
+    qps1 = n1 / (time.perf_counter() - t0)
+    return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": "first %d of the %d C2 queries, SegLRC-128, oracle block-skipping decode + "
+                      "intersect, %d threads, %.1f s; 1 thread: %.1f queries/s on %d queries"
+                      % (done, len(queries), threads, t_used, qps1, n1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain single-query leg")
+    ap.add_argument("--no-pfd", action="store_true", help="skip the ParaPFD leg")
+    ap.add_argument("--nterms", type=int, default=100_000)
+    ap.add_argument("--nq", type=int, default=1000)
+    ap.add_argument("--traffic-kib", type=float, default=None,
+                    help="rocprofv3 FETCH_SIZE+WRITE_SIZE KiB per probe launch (scaled), if measured")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard by "
+              "replicating the index or by docID range; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import exclucene_amd as ea
+    from exclucene_amd import workloads as W
+
+    t0 = time.perf_counter()
+    df = W.zipf_df(args.nterms)
+    docs, off = ea.synth_lists(df, W.C2_UNIVERSE, 2)
+    queries = W.df_weighted_queries(df, args.nq, 2, 5, 22)
+    log("corpus: %d terms, %d postings, %.1f s" % (len(df), len(docs), time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    seg = ea.build_index(docs, off, ea.SEGLRC, seg=128, frac=0.0)
+    log("SegLRC-128 built: %.1f MB compressed, %.1f s" % (seg.compressed_bytes() / 1e6, time.perf_counter() - t0))
+
+    ctx = ea.Context(0)
+    dix = ctx.upload(seg)
+    ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_FIT)
+    ctx.set_option(ea.OPT_PERSISTENT, 1)
+    ctx.set_option(ea.OPT_TPE, 0)
+    dt, probe, invoke = time_batch(ctx, dix, queries, args.steps, args.warmup)
+    nq = len(queries)
+    value = nq * args.steps / dt
+    bi = ctx._bi
+    cum, res = ctx.dtohTransfer(bi)
+    cs = ctx.verify(bi, cum, res)
+    log("SegLRC batched: %.0f q/s, step %.3f ms, probe %.3f ms, results %d, checksum %s"
+        % (value, 1e3 * dt / args.steps, probe.mean(), len(res), cs))
+
+    t0 = time.perf_counter()
+    alg, full = algorithmic_bytes(seg, docs, off, queries)
+    log("algorithmic bytes per launch %.1f MB (decode-everything %.1f MB), %.1f s"
+        % (alg / 1e6, full / 1e6, time.perf_counter() - t0))
+    achieved = alg / (probe.mean() * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_plain:
+        ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_BINARY)
+        ctx.set_option(ea.OPT_PERSISTENT, 0)
+        ctx.set_option(ea.OPT_TPE, 1)
+        ctx.set_option(ea.OPT_TIMING, 0)
+        nplain = min(nq, 300)
+        lat = []
+        t0 = time.perf_counter()
+        for q in queries[:nplain]:
+            t1 = time.perf_counter()
+            ctx.intersect(dix, [q])
+            lat.append(time.perf_counter() - t1)
+        plain_dt = time.perf_counter() - t0
+        extra["plain_single_query_qps"] = nplain / plain_dt
+        extra["plain_p50_ms"] = float(np.percentile(lat, 50) * 1e3)
+        extra["plain_p99_ms"] = float(np.percentile(lat, 99) * 1e3)
+        ctx.set_option(ea.OPT_TIMING, 1)
+        log("plain single-query: %.0f q/s" % extra["plain_single_query_qps"])
+        # batched end to end (plan + htod + kernels + dtoh)
+        ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_FIT)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        ctx.set_option(ea.OPT_TPE, 0)
+        t0 = time.perf_counter()
+        ctx.intersect(dix, queries)
+        extra["batched_end_to_end_qps"] = nq / (time.perf_counter() - t0)
+
+    if not args.no_pfd:
+        pfd = ea.build_index(docs, off, ea.PARAPFD, seg=128, frac=0.1)
+        pix = ctx.upload(pfd)
+        pdt, pprobe, _ = time_batch(ctx, pix, queries, max(3, args.steps // 2), 1)
+        extra["parapfd_batched_qps"] = nq * max(3, args.steps // 2) / pdt
+        extra["parapfd_probe_ms"] = float(pprobe.mean())
+        pcum, pres = ctx.dtohTransfer(ctx._bi)
+        extra["parapfd_matches_seglrc"] = bool(np.array_equal(pcum, cum) and np.array_equal(pres, res))
+        log("ParaPFD batched: %.0f q/s" % extra["parapfd_batched_qps"])
+        pix.close()
+        del pfd
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(seg, queries)
+        log("cpu baseline: %s" % cpu)
+
+    out = {
+        "metric": "exact k-term intersect queries/s (batched vs plain); % HBM peak; p50/p99 ms",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "C2: 1,000 queries of 2-5 terms, 100k-term Zipf(1.0) index over 2^25 docIDs, "
+                               "SegLRC-128 resident in HBM",
+                   "queries": nq, "postings": int(len(docs)), "codec": "SegLRC-128",
+                   "compressed_bytes": int(seg.compressed_bytes())},
+        "step_p50_ms": float(np.percentile(invoke, 50)),
+        "step_p99_ms": float(np.percentile(invoke, 99)),
+        "results": int(len(res)),
+        "checksum": [int(x) for x in cs],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if args.traffic_kib is None else args.traffic_kib * 1024.0,
+                     "kernel": "k_probe<SegLRC,128>", "probe_ms": float(probe.mean()),
+                     "algorithmic_bytes": int(alg), "decode_everything_bytes": int(full)},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    dix.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

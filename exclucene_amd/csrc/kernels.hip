@@ -298,6 +298,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, uint32_t* ca
     // ---- segment of every candidate
     uint32_t hreg = kNone;
     if (small && ln < rng) hreg = head[s_lo + ln];
+    const uint32_t h63 = uni(__builtin_amdgcn_readlane(hreg, 63));
     uint32_t v[V], s[V];
     bool live[V];
 #pragma unroll
@@ -312,6 +313,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, uint32_t* ca
                 const uint32_t h = __shfl(hreg, (int)(pos + st - 1));
                 if (h <= v[i]) pos += st;
             }
+            if (pos == 63 && h63 <= v[i]) pos = 64;  // the lifting steps sum to 63
         } else if (live[i]) {
             uint32_t lo = 0, hi = rng;
             while (lo < hi) {
@@ -402,6 +404,7 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
     const uint32_t nb = R.nseg;
     const uint32_t b0 = uni(wave_upper_bound(starts, nb + 1, e0) - 1);
     const uint32_t hreg = (b0 + 1 + ln <= nb) ? starts[b0 + 1 + ln] : kNone;
+    const uint32_t h63 = uni(__builtin_amdgcn_readlane(hreg, 63));
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         const uint32_t j = ln + 64 * i;
@@ -412,6 +415,7 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
             const uint32_t h = __shfl(hreg, (int)(pos + st - 1));
             if (h <= e) pos += st;
         }
+        if (pos == 63 && h63 <= e) pos = 64;  // the lifting steps sum to 63
         if (j < clen) {
             uint32_t b = b0 + pos;
             if (pos == 64) {

@@ -1,0 +1,78 @@
+"""Seeded synthetic workloads (the reference's own datasets -- Baidu bd* and
+GOV2 -- are not available; these restate the benchmark configurations):
+
+C1  one 2-term query: 1,000 and 1,000,000 docIDs uniform without replacement
+    from [0, 2^24), LRC (one global fit, 256-docID residual blocks).
+C2  universe 2^25, 100,000 terms, df_k = min(2^23, floor(2^25 / k)) (Zipf s=1
+    capped at 2^25/4); 1,000 queries of 2-5 terms drawn proportionally to df;
+    lists encoded as SegLRC-128 and ParaPFD-128 (FRAC 0.1).
+C3  the C2 lists split into hash buckets on the top 12 docID bits (shift 13);
+    10,000 2-term queries with short:long ratios 10^U(1,4).
+C4  the C2 lists with a learned position model on every list longer than
+    65,536; 10,000 queries of 2-4 terms.
+Lists come from exc_synth_lists (splitmix64 per list, Floyd sampling for dense
+lists, sort-unique-refill for sparse ones); query streams from numpy PCG64.
+"""
+import numpy as np
+
+from . import lib as _lib  # noqa: F401  (fail loudly if the library is missing)
+from . import synth_lists
+
+C2_UNIVERSE = 1 << 25
+C2_TERMS = 100_000
+C2_CAP = 1 << 23
+
+
+def zipf_df(nterms=C2_TERMS, universe=C2_UNIVERSE, cap=C2_CAP, s=1.0, scale=None):
+    k = np.arange(1, nterms + 1, dtype=np.float64)
+    c = float(universe if scale is None else scale)
+    df = np.floor(c / k ** s)
+    return np.minimum(df, cap).astype(np.uint32)
+
+
+def c1_lists(seed=1):
+    rng = np.random.default_rng(seed)
+    a = np.sort(rng.choice(1 << 24, 1000, replace=False)).astype(np.uint32)
+    b = np.sort(rng.choice(1 << 24, 1_000_000, replace=False)).astype(np.uint32)
+    return [a, b]
+
+
+def c2_corpus(seed=2, nterms=C2_TERMS, nthreads=0):
+    df = zipf_df(nterms)
+    docs, off = synth_lists(df, C2_UNIVERSE, seed, nthreads)
+    return df, docs, off
+
+
+def df_weighted_queries(df, nq, tmin, tmax, seed):
+    """nq queries; term count uniform in [tmin, tmax]; distinct terms per query,
+    drawn with probability proportional to df."""
+    rng = np.random.default_rng(seed)
+    p = df.astype(np.float64) / float(df.sum())
+    out = []
+    for _ in range(nq):
+        tc = int(rng.integers(tmin, tmax + 1))
+        out.append([int(t) for t in rng.choice(len(df), tc, replace=False, p=p)])
+    return out
+
+
+def c2_queries(df, nq=1000, seed=22):
+    return df_weighted_queries(df, nq, 2, 5, seed)
+
+
+def ratio_queries(df, nq=10_000, seed=3, lo=1.0, hi=4.0):
+    """2-term queries whose long:short length ratio is 10^U(lo, hi) (df_k ~ C/k,
+    so the long term index is the short one divided by the ratio)."""
+    rng = np.random.default_rng(seed)
+    n = len(df)
+    out = []
+    for _ in range(nq):
+        r = 10.0 ** rng.uniform(lo, hi)
+        ks = int(np.exp(rng.uniform(np.log(r), np.log(n))))
+        kl = max(1, int(round(ks / r)))
+        out.append([ks - 1, kl - 1])
+    return out
+
+
+def flat(queries):
+    from . import queries_to_flat
+    return queries_to_flat(queries)

@@ -8,7 +8,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-LENGTHS = [1, 2, 7, 64, 127, 128, 129, 255, 256, 257, 1000, 4000, 20000, 150000, 600000]
+LENGTHS = [1, 2, 7, 64, 127, 128, 129, 255, 256, 257, 1000, 4000, 20000, 150000, 600000, 64000, 16384]
 
 
 def _lists(seed, universe=1 << 24):
@@ -26,7 +26,7 @@ def _queries(nl, seed):
     qs = [[i] for i in range(nl)]
     qs += [[i, j] for i in range(nl) for j in range(nl) if i != j and (i + j) % 3 == 0]
     qs += [[int(x) for x in rng.choice(nl, int(rng.integers(3, 6)), replace=False)] for _ in range(40)]
-    qs += [[13, 13], [14, 13, 12, 11], [nl - 1, nl - 2], [nl - 3, 14], [], [0, 14, 14]]
+    qs += [[13, 13], [14, 13, 12, 11], [nl - 1, nl - 2], [nl - 3, 14], [], [0, 14, 14], [10, 15], [5, 16], [4, 16], [6, 15, 14]]
     return qs
 
 
