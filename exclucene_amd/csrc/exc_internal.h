@@ -49,14 +49,28 @@ struct alignas(16) Model {
     uint32_t pad[2];
 };
 
+// Per-segment (per-bucket) record derived at upload so one 32-byte load gives
+// a probe everything about a block: its word offset, ParaPFor descriptor,
+// fit, base and first docID (the .segOffset/.regression_info/
+// .ind2_distance_base/.segHead rows of the reference, interleaved).
+struct alignas(32) SegRec {
+    uint32_t off;    // word offset of the block inside its list
+    uint32_t desc;   // the block's ParaPFor descriptor word
+    float slope;     // SegLRC/HS fit slope (1 otherwise)
+    float icpt;      // SegLRC/HS fit intercept
+    uint32_t rl;     // nRangeLeft
+    uint32_t rr;     // nRangeRight
+    int32_t base;    // SegLRC/HS residual base
+    uint32_t head;   // first docID (segment codecs) | first element index (HS)
+};
+static_assert(sizeof(SegRec) == 32, "SegRec is 32 bytes");
+
 struct DevIndex {
+    const SegRec* rec;
     const uint32_t* words;
     const TermRec* terms;
-    const uint32_t* seg_off;
-    const uint32_t* seg_head;
-    const float4* fit;      // exc_seg_info rows (slope, icpt, rangeL bits, rangeR bits)
-    const int32_t* base;
-    const uint32_t* hash;   // HashSeg_info
+    const uint32_t* seg_head;  // .segHead (dense, for the head searches)
+    const uint32_t* hash;      // .HashSeg_info
     const Model* models;
     const float* mparams;
     int32_t codec;
@@ -84,6 +98,7 @@ struct ProbeArgs {
     int32_t search;
     int32_t tpe;
     int32_t persistent;
+    int32_t coop_max;   // decode runs of up to this many segments cooperatively (tpe 0)
     uint32_t* stage;
     uint32_t* ucount;
     uint32_t* counter;
@@ -96,7 +111,7 @@ struct ScanArgs {
     const uint32_t* stage;
     uint32_t nunits;
     uint32_t nq;
-    uint32_t seg;
+    uint32_t pad_;
     uint32_t* uoff;     // per unit exclusive offset inside its scan block
     uint32_t* bsum;     // per scan block sum, then exclusive block offsets (+ total)
     uint32_t* result;
@@ -104,6 +119,7 @@ struct ScanArgs {
 };
 
 constexpr uint32_t kScanBlockItems = 4096;  // units per scan block (256 threads x 16)
+constexpr uint32_t kChunk = 512;            // shortest-list docIDs per work unit
 
 // launchers (kernels.hip); all return hipError_t of the launch
 hipError_t launch_reset(uint32_t* counter, hipStream_t s);

@@ -9,22 +9,25 @@
 //   GC/ParaPFD/ParaPFD_Decompression/ParaPFD_Decompression_kernel.cu:117-272
 //
 // Execution model (CDNA4, wave64):
-//  * one wave owns one work unit = (query, one segment of its shortest list);
+//  * one wave owns one work unit = (query, 512 consecutive docIDs of its
+//    shortest list, i.e. 4 segments of 128 or 2 of 256);
 //    a 256-thread workgroup hosts 4 independent waves, each with a private LDS
-//    slice, synchronised with wave barriers only.
+//    slice, synchronised with wave barriers only.  Persistent launches claim
+//    units in chunks from one device counter.
 //  * the unit's docIDs are decoded cooperatively into LDS and stay there as the
 //    candidate set; each longer list (ascending length) filters it.
-//  * per list: the wave brackets the segment range with a 64-ary search over
-//    segHead (one gather + ballot per level), each candidate finds its segment
-//    with 6 register shuffles (range <= 64 segments) or a binary search, then
-//    either (tpe = 1) searches its segment by random-access bit extraction, or
-//    (tpe = 64) the wave decodes each distinct needed segment into LDS with
-//    coalesced loads and candidates binary-search LDS.
+//  * per list, one gather of <= 64 segment heads around the window that the
+//    list-level fit predicts (checked against the heads, exact fallback to a
+//    64-ary search) gives every candidate its segment by register shuffles;
+//  * then either the wave decodes the contiguous run of needed segments into
+//    LDS with coalesced loads (low length ratios) and candidates search LDS
+//    branch-free, or each lane fetches its segment record, extracts the value
+//    at the position the segment fit predicts and walks to v (high ratios);
 //  * survivors are compacted with ballot + mbcnt (order preserving), so the
-//    next list only probes live candidates and no 0/1 flag array exists.
-//  * results: each unit writes its survivors into the stage slot of its
-//    shortest-list segment; a block scan over unit counts + a scatter produce
-//    the contiguous per-query output and cumulative counts.
+//    next list only probes live candidates and no 0/1 flag array exists;
+//  * each unit writes its survivors into the stage slot of its shortest-list
+//    segment; a block scan over unit counts + a scatter produce the contiguous
+//    per-query output and cumulative counts.
 #include "exc_format.h"
 #include "exc_internal.h"
 
@@ -33,6 +36,9 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kClaim = 4;        // units per work-queue claim (persistent launch)
+constexpr uint32_t kBufWords = 1024;  // per-wave LDS decode buffer
+constexpr int kV = (int)(kChunk / 64);  // candidates per lane
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -93,21 +99,18 @@ __device__ uint32_t wave_upper_bound(const uint32_t* __restrict__ a, uint32_t n,
     return lo + popc(__ballot(p));
 }
 
-// Same with a predicted window [wa, wb] for the answer; the window is used only
-// if head[wa-1] <= v < head[wb] confirms it (exact fallback otherwise).
-__device__ uint32_t wave_upper_bound_win(const uint32_t* __restrict__ a, uint32_t n, uint32_t v,
-                                         uint32_t wa, uint32_t wb)
+// Count of lanes of the sorted register array H (kNone padded) that are <= v,
+// v per lane: 6 shuffles of binary lifting plus the 64th lane.
+__device__ __forceinline__ uint32_t lane_count_le(uint32_t H, uint32_t h63, uint32_t v)
 {
-    const uint32_t ln = lane_id();
-    wa = min(wa, n);
-    wb = min(max(wb, wa), n);
-    uint32_t x = 0;
-    if (ln == 0 && wa > 0) x = a[wa - 1];
-    if (ln == 1 && wb < n) x = a[wb];
-    const uint32_t left = uni(__shfl(x, 0)), right = uni(__shfl(x, 1));
-    const bool ok = (wa == 0 || left <= v) && (wb == n || right > v);
-    if (!ok) return wave_upper_bound(a, n, v);
-    return wa + wave_upper_bound(a + wa, wb - wa, v);
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t st = 32; st >= 1; st >>= 1) {
+        const uint32_t h = __shfl(H, (int)(pos + st - 1));
+        if (h <= v) pos += st;
+    }
+    if (pos == 63 && h63 <= v) pos = 64;  // the lifting steps sum to 63
+    return pos;
 }
 
 __device__ __forceinline__ uint32_t seg_len(const TermRec& R, uint32_t s, uint32_t seg)
@@ -115,54 +118,60 @@ __device__ __forceinline__ uint32_t seg_len(const TermRec& R, uint32_t s, uint32
     return (s + 1 == R.nseg) ? R.n - s * seg : seg;
 }
 
-// Decode segment s of list R into dst[0..len) with the whole wave (segment
-// codecs).  Returns len.  Caller must wave_sync() before reading other lanes'.
+// ------------------------------------------------------------ block decode
+
+// Raw lb-bit fields of one block for this lane: strided positions
+// (j = lane + 64 t) for the random-access codecs, consecutive (j = lane*V + t)
+// for ParaPFD whose prefix sum runs along a lane first.
 template <int CODEC, int SEG>
-__device__ uint32_t decode_seg_block(const DevIndex& ix, const TermRec& R, uint32_t s, uint32_t* dst)
+__device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, uint32_t lb, uint32_t len,
+                                            uint32_t (&f)[SEG / 64])
 {
+    constexpr int V = SEG / 64;
     const uint32_t ln = lane_id();
-    const uint32_t len = uni(seg_len(R, s, SEG));
-    const uint32_t* blk = ix.words + R.woff + ix.seg_off[R.segnum + s];
-    const uint32_t d = uni(blk[0]);
-    const uint32_t lb = desc_lb(d);
+#pragma unroll
+    for (int t = 0; t < V; ++t) {
+        const uint32_t j = (CODEC == EXC_PARAPFD) ? ln * V + t : ln + 64 * t;
+        f[t] = (j < len) ? bits_at(data, lb, j) : 0u;
+    }
+}
+
+// Finish decoding block s (fields already loaded) into dst[0..len).
+template <int CODEC, int SEG>
+__device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const SegRec& rec, uint32_t len,
+                                             const uint32_t* __restrict__ data, uint32_t (&f)[SEG / 64],
+                                             uint32_t* dst)
+{
+    constexpr int V = SEG / 64;
+    const uint32_t ln = lane_id();
+    const uint32_t lb = desc_lb(rec.desc);
     if constexpr (CODEC == EXC_LRC) {
         // GC/LRC/LRC_Decompression/LRC_Decompression_kernel.cu:206-256
 #pragma unroll
-        for (int i = 0; i < SEG / 64; ++i) {
-            const uint32_t j = ln + 64 * i;
-            if (j < len) dst[j] = bits_at(blk + 1, lb, j) + (uint32_t)(lrc_y(s * SEG + j, R.slope) + R.icpt);
+        for (int t = 0; t < V; ++t) {
+            const uint32_t j = ln + 64 * t;
+            if (j < len) dst[j] = f[t] + (uint32_t)(lrc_y(s * SEG + j, R.slope) + R.icpt);
         }
     } else if constexpr (CODEC == EXC_SEGLRC) {
         // GC/SegLRC/SegLRC_Decompression/SegLRC_Decompression_kernel.cu:200-256
-        const float4 f = ix.fit[R.segnum + s];
-        const int32_t b = ix.base[R.segnum + s];
-#pragma unroll
-        for (int i = 0; i < SEG / 64; ++i) {
-            const uint32_t j = ln + 64 * i;
-            if (j < len) dst[j] = bits_at(blk + 1, lb, j) + (uint32_t)(seg_y(j, f.x, f.y) + b);
-        }
-    } else {
-        // ParaPFD: unpack, patch exceptions, head, inclusive scan
-        // (GC/ParaPFD/ParaPFD_Decompression/ParaPFD_Decompression_kernel.cu:198-264).
-        // Lane holds V consecutive gaps, so the scan is V-1 local adds plus one
-        // wave scan of lane totals.
-        constexpr int V = SEG / 64;
-        uint32_t val[V];
 #pragma unroll
         for (int t = 0; t < V; ++t) {
-            const uint32_t j = ln * V + t;
-            val[t] = (j < len) ? bits_at(blk + 1, lb, j) : 0u;
+            const uint32_t j = ln + 64 * t;
+            if (j < len) dst[j] = f[t] + (uint32_t)(seg_y(j, rec.slope, rec.icpt) + rec.base);
         }
-        const uint32_t en = desc_en(d);
+    } else {
+        // ParaPFD: patch exceptions, segHead, inclusive scan
+        // (GC/ParaPFD/ParaPFD_Decompression/ParaPFD_Decompression_kernel.cu:198-264).
+        const uint32_t en = desc_en(rec.desc);
         if (en) {
 #pragma unroll
             for (int t = 0; t < V; ++t) {
                 const uint32_t j = ln * V + t;
-                if (j < len) dst[j] = val[t];
+                if (j < len) dst[j] = f[t];
             }
             wave_sync();
-            const uint32_t eb = desc_eb(d), hb = desc_hb(d);
-            const uint32_t* pp = blk + 1 + words_for(len, lb);
+            const uint32_t eb = desc_eb(rec.desc), hb = desc_hb(rec.desc);
+            const uint32_t* pp = data + words_for(len, lb);
             const uint32_t* hp = pp + words_for(en, eb);
             for (uint32_t e = ln; e < en; e += 64) {
                 const uint32_t p = bits_at(pp, eb, e);
@@ -173,204 +182,340 @@ __device__ uint32_t decode_seg_block(const DevIndex& ix, const TermRec& R, uint3
 #pragma unroll
             for (int t = 0; t < V; ++t) {
                 const uint32_t j = ln * V + t;
-                if (j < len) val[t] = dst[j];
+                if (j < len) f[t] = dst[j];
             }
         }
-        if (ln == 0) val[0] = ix.seg_head[R.segnum + s];
+        if (ln == 0) f[0] = rec.head;
         uint32_t acc = 0;
 #pragma unroll
-        for (int t = 0; t < V; ++t) { acc += val[t]; val[t] = acc; }
+        for (int t = 0; t < V; ++t) { acc += f[t]; f[t] = acc; }
         const uint32_t excl = wave_incl_scan(acc) - acc;
         wave_sync();
 #pragma unroll
         for (int t = 0; t < V; ++t) {
             const uint32_t j = ln * V + t;
-            if (j < len) dst[j] = val[t] + excl;
+            if (j < len) dst[j] = f[t] + excl;
         }
     }
-    return len;
 }
 
-// exact search of v in a random-access SegLRC / HS block of len values, with
-// the per-block fit window [pivot - L, pivot + R] tried first and a full binary
-// search when v may lie outside it.  The reference uses the window alone
-// (HS_SegLRC_Intersection_kernel.cu:361-369, SegLRC_Intersection_kernel.cu:366-371),
-// which drops results; the fallback keeps the answer exact.
-__device__ __forceinline__ bool fit_block_search(const uint32_t* __restrict__ blk, uint32_t len, float4 f,
-                                                 int32_t base, uint32_t v, bool use_window)
+// Decode segments listed by sid(g), g < r <= NB, of list R into dst + g*SEG.
+// Blocks go in groups (all of them for the random-access codecs, pairs for
+// ParaPFD whose scans hold more registers); every load of a group is issued
+// before any of its values is consumed.
+template <int CODEC, int SEG, int NB, class SegId>
+__device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, SegId sid, uint32_t* dst)
 {
-    const uint32_t lb = desc_lb(blk[0]);
-    const uint32_t* p = blk + 1;
-    int lo = 0, hi = (int)len - 1;
-    if (use_window) {
-        float fp = ((float)v - f.y) / f.x;
-        fp = fminf(fmaxf(fp, -1.0e9f), 1.0e9f);
-        const int64_t piv = (int64_t)fp;
-        const int wl = (int)max(piv - (int64_t)__float_as_uint(f.z), (int64_t)0);
-        const int wh = (int)min(piv + (int64_t)__float_as_uint(f.w), (int64_t)len - 1);
-        if (wl <= wh) {
-            int a = wl, c = wh;
-            while (a <= c) {
-                const int m = (a + c) >> 1;
-                const uint32_t q = bits_at(p, lb, (uint32_t)m) + (uint32_t)(seg_y((uint32_t)m, f.x, f.y) + base);
-                if (q == v) return true;
-                if (q < v) a = m + 1; else c = m - 1;
+    constexpr int V = SEG / 64;
+    constexpr int G = (CODEC == EXC_PARAPFD) ? (NB < 2 ? NB : 2) : NB;
+    for (uint32_t g0 = 0; g0 < r; g0 += G) {
+        SegRec rec[G];
+        uint32_t len[G], sg[G];
+        const uint32_t* data[G];
+        uint32_t f[G][V];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (g0 + g < r) {
+                sg[g] = sid((int)(g0 + g));
+                rec[g] = ix.rec[R.segnum + sg[g]];
+                len[g] = seg_len(R, sg[g], SEG);
+                data[g] = ix.words + R.woff + rec[g].off + 1;
             }
-            if (a == wl && wl > 0) { lo = 0; hi = wl - 1; }
-            else if (a == wh + 1 && wh < (int)len - 1) { lo = wh + 1; hi = (int)len - 1; }
-            else return false;
         }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (g0 + g < r) load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (g0 + g < r)
+                finish_block<CODEC, SEG>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG);
     }
+}
+
+// ------------------------------------------------------------ random access
+
+__device__ __forceinline__ uint32_t seg_val(const uint32_t* data, uint32_t lb, uint32_t j, const SegRec& r)
+{
+    return bits_at(data, lb, j) + (uint32_t)(seg_y(j, r.slope, r.icpt) + r.base);
+}
+
+__device__ __forceinline__ bool seg_bsearch(const uint32_t* data, uint32_t lb, const SegRec& r, int lo, int hi,
+                                            uint32_t v)
+{
     while (lo <= hi) {
         const int m = (lo + hi) >> 1;
-        const uint32_t q = bits_at(p, lb, (uint32_t)m) + (uint32_t)(seg_y((uint32_t)m, f.x, f.y) + base);
+        const uint32_t q = seg_val(data, lb, (uint32_t)m, r);
         if (q == v) return true;
         if (q < v) lo = m + 1; else hi = m - 1;
     }
     return false;
 }
 
-// random-access search of v in segment s (LRC / SegLRC, no exceptions)
-template <int CODEC, int SEG>
-__device__ __forceinline__ bool ra_search(const ProbeArgs& A, const TermRec& R, uint32_t s, uint32_t v)
+// Exact search of v in a random-access SegLRC / HS block: start at the
+// position the block's fit predicts, walk towards v for at most the fit's
+// recorded deviation (rl / rr), and binary-search the rest of the block if v
+// may lie beyond the walk.  The reference searches only the fit window
+// (HS_SegLRC_Intersection_kernel.cu:361-369, SegLRC_Intersection_kernel.cu:366-371),
+// which drops results; the fallback keeps the answer exact.
+__device__ __forceinline__ bool seg_find(const uint32_t* data, uint32_t len, const SegRec& r, uint32_t v, bool window)
 {
-    const uint32_t len = seg_len(R, s, SEG);
-    const uint32_t* blk = A.ix.words + R.woff + A.ix.seg_off[R.segnum + s];
-    if constexpr (CODEC == EXC_SEGLRC) {
-        return fit_block_search(blk, len, A.ix.fit[R.segnum + s], A.ix.base[R.segnum + s], v,
-                                A.search != SEARCH_BINARY);
-    } else {
-        // GC/LRC/LRC_Intersection/LRC_Intersection_kernel.cu:333-355
-        const uint32_t lb = desc_lb(blk[0]);
-        int lo = 0, hi = (int)len - 1;
-        while (lo <= hi) {
-            const int m = (lo + hi) >> 1;
-            const uint32_t q = bits_at(blk + 1, lb, (uint32_t)m) + (uint32_t)(lrc_y(s * SEG + (uint32_t)m, R.slope) + R.icpt);
-            if (q == v) return true;
-            if (q < v) lo = m + 1; else hi = m - 1;
+    const uint32_t lb = desc_lb(r.desc);
+    if (!window) return seg_bsearch(data, lb, r, 0, (int)len - 1, v);
+    const float fp = ((float)v - r.icpt) * __builtin_amdgcn_rcpf(r.slope) + 0.5f;
+    int p = (int)fminf(fmaxf(fp, 0.0f), (float)(len - 1));
+    uint32_t q = seg_val(data, lb, (uint32_t)p, r);
+    if (q == v) return true;
+    if (q < v) {
+        const int lim = min((int)len - 1, p + (int)min(r.rr, 64u) + 1);
+        while (p < lim) {
+            ++p;
+            q = seg_val(data, lb, (uint32_t)p, r);
+            if (q >= v) return q == v;
         }
-        return false;
+        return p < (int)len - 1 && seg_bsearch(data, lb, r, p + 1, (int)len - 1, v);
     }
+    const int lim = max(0, p - (int)min(r.rl, 64u) - 1);
+    while (p > lim) {
+        --p;
+        q = seg_val(data, lb, (uint32_t)p, r);
+        if (q <= v) return q == v;
+    }
+    return p > 0 && seg_bsearch(data, lb, r, 0, p - 1, v);
 }
 
-// Predicted window of "number of segment heads <= v" from the list-level fit.
-template <int CODEC, int SEG>
-__device__ __forceinline__ void fit_window(const TermRec& R, uint32_t v, uint32_t& wa, uint32_t& wb)
+// Branchless search of v in the sorted LDS run buf[0..n) (n uniform).
+__device__ __forceinline__ bool lds_contains(const uint32_t* buf, uint32_t n, uint32_t v)
 {
-    float icpt = (CODEC == EXC_LRC) ? (float)R.icpt : __int_as_float(R.icpt);
-    float p = ((float)v - icpt) / R.slope;
+    uint32_t pos = 0;
+    for (uint32_t step = 1u << (31 - __builtin_clz(n | 1u)); step; step >>= 1)
+        if (pos + step <= n && buf[pos + step - 1] < v) pos += step;
+    return pos < n && buf[pos] == v;
+}
+
+// LRC random access: binary search of the segment (no per-segment fit)
+// (GC/LRC/LRC_Intersection/LRC_Intersection_kernel.cu:333-355).
+__device__ __forceinline__ bool lrc_bsearch(const uint32_t* data, uint32_t lb, uint32_t base_index, uint32_t len,
+                                            const TermRec& R, uint32_t v)
+{
+    int lo = 0, hi = (int)len - 1;
+    while (lo <= hi) {
+        const int m = (lo + hi) >> 1;
+        const uint32_t q = bits_at(data, lb, (uint32_t)m) + (uint32_t)(lrc_y(base_index + (uint32_t)m, R.slope) + R.icpt);
+        if (q == v) return true;
+        if (q < v) lo = m + 1; else hi = m - 1;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ per-list probes
+
+// Window [wa, wb] for "number of segment heads <= v" from the list-level fit.
+template <int CODEC, int SEG>
+__device__ __forceinline__ void fit_window(const TermRec& R, uint32_t v, int64_t& wa, int64_t& wb)
+{
+    const float icpt = (CODEC == EXC_LRC) ? (float)R.icpt : __int_as_float(R.icpt);
+    float p = ((float)v - icpt) * __builtin_amdgcn_rcpf(R.slope);
     p = fminf(fmaxf(p, -1.0e9f), 4.0e9f);
-    const float plo = p - R.range_l - 1.0f, phi = p + R.range_r + 1.0f;
-    wa = plo <= 0.0f ? 0u : (uint32_t)(plo / SEG);
-    wb = phi <= 0.0f ? 1u : (uint32_t)(phi / SEG) + 2u;
+    const float plo = p - R.range_l - 2.0f, phi = p + R.range_r + 2.0f;
+    wa = plo <= 0.0f ? 0 : (int64_t)(plo / SEG);
+    wb = phi <= 0.0f ? 1 : (int64_t)(phi / SEG) + 2;
 }
 
-// Filter cand[0..nlive) against list R (segment codecs).  Returns survivors.
+struct LdsWave {
+    uint32_t* cand;   // [kChunk]    live candidates (sorted)
+    uint32_t* buf;    // [kBufWords] decoded segments
+    uint32_t* dlist;  // [kChunk]    distinct segments (ParaPFD long-span path only)
+};
+
 template <int CODEC, int SEG>
-__device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, uint32_t* cand, uint32_t nlive,
-                              uint32_t* sarr, uint32_t* dlist, uint32_t* buf)
+__device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive)
 {
-    constexpr int V = SEG / 64;
-    constexpr int G = (512 / SEG) > 0 ? (512 / SEG) : 1;
+    constexpr int V = kV;
+    constexpr int NB = (int)(kBufWords / SEG);
     const uint32_t ln = lane_id();
     const uint32_t* head = A.ix.seg_head + R.segnum;
+    uint32_t* cand = W.cand;
     const uint32_t vmin = uni(cand[0]), vmax = uni(cand[nlive - 1]);
+    const int64_t nseg = R.nseg;
 
-    // ---- segment range of the candidate set: s_lo .. ub_hi-1
-    uint32_t ub_lo, ub_hi;
-    if (A.search != SEARCH_BINARY && R.nseg > 64) {
-        uint32_t wa, wb;
-        fit_window<CODEC, SEG>(R, vmin, wa, wb);
-        ub_lo = wave_upper_bound_win(head, R.nseg, vmin, wa, wb);
-        fit_window<CODEC, SEG>(R, vmax, wa, wb);
-        ub_hi = wave_upper_bound_win(head, R.nseg, vmax, wa, wb);
-    } else {
-        ub_lo = wave_upper_bound(head, R.nseg, vmin);
-        ub_hi = wave_upper_bound(head, R.nseg, vmax);
+    // ---- heads around the candidates: H = head[hb0 + lane] (index -1 reads as 0)
+    int64_t hb0 = 0;
+    uint32_t H = kNone;
+    bool small = false;
+    if (A.search != SEARCH_BINARY && nseg > 64) {
+        int64_t wa, t0, t1, wb;
+        fit_window<CODEC, SEG>(R, vmin, wa, t0);
+        fit_window<CODEC, SEG>(R, vmax, t1, wb);
+        wa = min(wa, nseg);
+        wb = min(max(wb, wa), nseg);
+        if (wb - wa + 2 <= 64) {
+            hb0 = wa - 1;
+            const int64_t idx = hb0 + ln;
+            H = (idx < 0) ? 0u : ((idx < nseg && (int64_t)ln <= wb - wa + 1) ? head[idx] : kNone);
+            const uint32_t h0 = uni(__builtin_amdgcn_readlane(H, 0));
+            const uint32_t hend = uni(__builtin_amdgcn_readlane(H, (int)(wb - wa + 1)));
+            small = h0 <= vmin && hend > vmax;
+        }
     }
-    ub_lo = uni(ub_lo);
-    ub_hi = uni(ub_hi);
-    if (ub_hi == 0) return 0;  // every candidate precedes the list
-    const uint32_t s_lo = ub_lo ? ub_lo - 1 : 0;
-    const uint32_t rng = ub_hi - s_lo;
-    const bool small = rng <= 64;
+    int64_t rng = 0;
+    if (!small) {
+        const uint32_t ub_lo = uni(wave_upper_bound(head, (uint32_t)nseg, vmin));
+        const uint32_t ub_hi = uni(wave_upper_bound(head, (uint32_t)nseg, vmax));
+        if (ub_hi == 0) return 0;  // every candidate precedes the list
+        hb0 = ub_lo ? ub_lo - 1 : 0;
+        rng = (int64_t)ub_hi - hb0;
+        small = rng <= 64;
+        H = (small && (int64_t)ln < rng) ? head[hb0 + ln] : kNone;
+    }
+    // segments spanned by the candidate set
+    int32_t s_first, s_last;
+    if (small) {
+        s_first = (int32_t)(hb0 + (int64_t)popc(__ballot(H <= vmin)) - 1);
+        s_last = (int32_t)(hb0 + (int64_t)popc(__ballot(H <= vmax)) - 1);
+    } else {
+        s_first = (int32_t)hb0;
+        s_last = (int32_t)(hb0 + rng - 1);
+    }
+    s_first = max(s_first, 0);
+    if (s_last < 0) return 0;
+    const uint32_t r = uni((uint32_t)(s_last - s_first + 1));
+    const uint32_t sf = uni((uint32_t)s_first);
 
-    // ---- segment of every candidate
-    uint32_t hreg = kNone;
-    if (small && ln < rng) hreg = head[s_lo + ln];
-    const uint32_t h63 = uni(__builtin_amdgcn_readlane(hreg, 63));
-    uint32_t v[V], s[V];
-    bool live[V];
+    uint32_t v[V];
+    bool found[V];
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         const uint32_t e = ln + 64 * i;
-        live[i] = e < nlive;
-        v[i] = live[i] ? cand[e] : kNone;
-        uint32_t pos = 0;
-        if (small) {
-#pragma unroll
-            for (uint32_t st = 32; st >= 1; st >>= 1) {
-                const uint32_t h = __shfl(hreg, (int)(pos + st - 1));
-                if (h <= v[i]) pos += st;
-            }
-            if (pos == 63 && h63 <= v[i]) pos = 64;  // the lifting steps sum to 63
-        } else if (live[i]) {
-            uint32_t lo = 0, hi = rng;
-            while (lo < hi) {
-                const uint32_t m = (lo + hi) >> 1;
-                if (head[s_lo + m] <= v[i]) lo = m + 1; else hi = m;
-            }
-            pos = lo;
-        }
-        s[i] = (live[i] && pos > 0) ? s_lo + pos - 1 : kNone;
+        v[i] = e < nlive ? cand[e] : kNone;
+        found[i] = false;
     }
 
-    bool found[V];
+    const uint32_t cmax = (A.tpe == 1 && CODEC != EXC_PARAPFD) ? 0u
+                          : (A.tpe == 64 || CODEC == EXC_PARAPFD) ? 64u : (uint32_t)A.coop_max;
+    if (r <= cmax) {
+        // ---- the run of segments sf .. sf+r-1 decoded piece by piece into LDS;
+        // one gather brings every segment record of the run, one load round
+        // per piece brings the packed words, sorted candidates resolve in the
+        // piece that reaches their value
+        constexpr int P = (CODEC == EXC_PARAPFD) ? (int)(512 / SEG) : NB;
+        const uint32_t gsel = min(ln, r - 1);
+        const SegRec gr = A.ix.rec[R.segnum + sf + gsel];
+        bool done[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) found[i] = false;
-
-    if (CODEC != EXC_PARAPFD && A.tpe == 1) {
-        // ---- one lane per candidate: random-access search in its segment
+        for (int i = 0; i < V; ++i) done[i] = ln + 64 * i >= nlive;
+        for (uint32_t p0 = 0; p0 < r; p0 += P) {
+            const uint32_t np = min((uint32_t)P, r - p0);
+            SegRec rec[P];
+            uint32_t len[P];
+            const uint32_t* data[P];
+            uint32_t f[P][SEG / 64];
 #pragma unroll
-        for (int i = 0; i < V; ++i)
-            if (s[i] != kNone) found[i] = ra_search<CODEC, SEG>(A, R, s[i], v[i]);
-    } else {
-        // ---- wave-cooperative: decode each distinct needed segment once
-#pragma unroll
-        for (int i = 0; i < V; ++i)
-            if (live[i]) sarr[ln + 64 * i] = s[i];
-        wave_sync();
-        uint32_t nd = 0, di[V];
-#pragma unroll
-        for (int i = 0; i < V; ++i) {
-            const uint32_t e = ln + 64 * i;
-            const bool first = s[i] != kNone && (e == 0 || sarr[e - 1] != s[i]);
-            const uint64_t m = __ballot(first);
-            di[i] = nd + below_me(m) + (first ? 1u : 0u) - 1u;
-            if (first) dlist[di[i]] = s[i];
-            nd += popc(m);
-        }
-        nd = uni(nd);
-        wave_sync();
-        for (uint32_t r = 0; r < nd; r += G) {
-            const uint32_t gn = min((uint32_t)G, nd - r);
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-                if ((uint32_t)g < gn) decode_seg_block<CODEC, SEG>(A.ix, R, uni(dlist[r + g]), buf + g * SEG);
-            wave_sync();
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                if (s[i] != kNone && di[i] >= r && di[i] < r + G) {
-                    const uint32_t* b = buf + (di[i] - r) * SEG;
-                    uint32_t lo = 0, hi = seg_len(R, s[i], SEG);
-                    while (lo < hi) {
-                        const uint32_t m = (lo + hi) >> 1;
-                        if (b[m] < v[i]) lo = m + 1; else hi = m;
-                    }
-                    found[i] = (lo < seg_len(R, s[i], SEG)) && b[lo] == v[i];
+            for (int g = 0; g < P; ++g) {
+                if ((uint32_t)g < np) {
+                    const int src = (int)(p0 + g);
+                    rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
+                    rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, src));
+                    rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), src)));
+                    rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
+                    rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
+                    rec[g].head = uni(__builtin_amdgcn_readlane(gr.head, src));
+                    len[g] = seg_len(R, sf + p0 + g, SEG);
+                    data[g] = A.ix.words + R.woff + rec[g].off + 1;
+                    load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
                 }
             }
+#pragma unroll
+            for (int g = 0; g < P; ++g)
+                if ((uint32_t)g < np)
+                    finish_block<CODEC, SEG>(R, sf + p0 + g, rec[g], len[g], data[g], f[g], W.buf + g * SEG);
+            const uint32_t total = (np - 1) * SEG + seg_len(R, sf + p0 + np - 1, SEG);
             wave_sync();
+            const uint32_t last = uni(W.buf[total - 1]);
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                if (!done[i] && v[i] <= last) {
+                    found[i] = lds_contains(W.buf, total, v[i]);
+                    done[i] = true;
+                }
+                any = any || !done[i];
+            }
+            wave_sync();
+            if (!__ballot(any)) break;
+        }
+    } else {
+        // ---- segment of every candidate (-1: precedes the list, -2: no candidate)
+        const uint32_t h63 = uni(__builtin_amdgcn_readlane(H, 63));
+        int32_t s[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const bool live = ln + 64 * i < nlive;
+            int64_t cnt;
+            if (small) {
+                cnt = lane_count_le(H, h63, v[i]);
+            } else {
+                uint32_t lo = 0, hi = (uint32_t)rng;
+                if (live)
+                    while (lo < hi) {
+                        const uint32_t m = (lo + hi) >> 1;
+                        if (head[hb0 + m] <= v[i]) lo = m + 1; else hi = m;
+                    }
+                cnt = lo;
+            }
+            s[i] = live ? (int32_t)(hb0 + cnt - 1) : -2;
+        }
+        if constexpr (CODEC == EXC_PARAPFD) {
+            // ---- decode each distinct needed segment once, NB per round
+            uint32_t nd = 0, di[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                // segment of the previous candidate: lane - 1, or lane 63 of the previous row
+                int32_t sp = __shfl_up(s[i], 1);
+                if (ln == 0) sp = (i > 0) ? (int32_t)__builtin_amdgcn_readlane((uint32_t)s[i > 0 ? i - 1 : 0], 63) : -2;
+                const bool first = s[i] >= 0 && s[i] != sp;
+                const uint64_t m = __ballot(first);
+                di[i] = nd + below_me(m) + (first ? 1u : 0u) - 1u;
+                if (first) W.dlist[di[i]] = (uint32_t)s[i];
+                nd += popc(m);
+            }
+            nd = uni(nd);
+            wave_sync();
+            for (uint32_t r0 = 0; r0 < nd; r0 += NB) {
+                const uint32_t gn = min((uint32_t)NB, nd - r0);
+                decode_blocks<CODEC, SEG, NB>(A.ix, R, gn, [&](int g) { return uni(W.dlist[r0 + g]); }, W.buf);
+                wave_sync();
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    if (s[i] >= 0 && di[i] >= r0 && di[i] < r0 + NB)
+                        found[i] = lds_contains(W.buf + (di[i] - r0) * SEG, seg_len(R, (uint32_t)s[i], SEG), v[i]);
+                }
+                wave_sync();
+            }
+        } else if constexpr (CODEC == EXC_SEGLRC) {
+            // ---- one lane per candidate: segment record, predicted position, short walk
+#pragma unroll
+            for (int i = 0; i < V; i += 2) {
+                SegRec rec[2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+                    if (s[i + x] >= 0) rec[x] = A.ix.rec[R.segnum + s[i + x]];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+                    if (s[i + x] >= 0)
+                        found[i + x] = seg_find(A.ix.words + R.woff + rec[x].off + 1,
+                                                seg_len(R, (uint32_t)s[i + x], SEG), rec[x], v[i + x],
+                                                A.search != SEARCH_BINARY);
+            }
+        } else {
+            // LRC: binary search inside the segment
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                if (s[i] >= 0) {
+                    const SegRec rc = A.ix.rec[R.segnum + s[i]];
+                    found[i] = lrc_bsearch(A.ix.words + R.woff + rc.off + 1, desc_lb(rc.desc), (uint32_t)s[i] * SEG,
+                                           seg_len(R, (uint32_t)s[i], SEG), R, v[i]);
+                }
+            }
         }
     }
 
@@ -390,16 +535,16 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, uint32_t* ca
     return uni(cnt);
 }
 
-// HS_SegLRC: decode elements [k*SEG, k*SEG+SEG) of list R into dst; returns count.
+// HS_SegLRC: decode elements [k*CH, k*CH+CH) of list R into dst; returns count.
 // Bucket of each element = last bucket start <= its index
 // (HS_SegLRC_Intersection_kernel.cu:274-327).
-template <int SEG>
+template <int CH>
 __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32_t k, uint32_t* dst)
 {
-    constexpr int V = SEG / 64;
+    constexpr int V = CH / 64;
     const uint32_t ln = lane_id();
-    const uint32_t e0 = k * SEG;
-    const uint32_t clen = uni(min((uint32_t)SEG, R.n - e0));
+    const uint32_t e0 = k * CH;
+    const uint32_t clen = uni(min((uint32_t)CH, R.n - e0));
     const uint32_t* starts = ix.hash + R.aux;
     const uint32_t nb = R.nseg;
     const uint32_t b0 = uni(wave_upper_bound(starts, nb + 1, e0) - 1);
@@ -409,13 +554,7 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
     for (int i = 0; i < V; ++i) {
         const uint32_t j = ln + 64 * i;
         const uint32_t e = e0 + j;
-        uint32_t pos = 0;
-#pragma unroll
-        for (uint32_t st = 32; st >= 1; st >>= 1) {
-            const uint32_t h = __shfl(hreg, (int)(pos + st - 1));
-            if (h <= e) pos += st;
-        }
-        if (pos == 63 && h63 <= e) pos = 64;  // the lifting steps sum to 63
+        const uint32_t pos = lane_count_le(hreg, h63, e);
         if (j < clen) {
             uint32_t b = b0 + pos;
             if (pos == 64) {
@@ -426,11 +565,9 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
                 }
                 b = lo - 1;
             }
-            const uint32_t st = starts[b];
-            const uint32_t* blk = ix.words + R.woff + ix.seg_off[R.segnum + b];
-            const float4 f = ix.fit[R.segnum + b];
-            const uint32_t x = e - st;
-            dst[j] = bits_at(blk + 1, desc_lb(blk[0]), x) + (uint32_t)(seg_y(x, f.x, f.y) + ix.base[R.segnum + b]);
+            const SegRec rc = ix.rec[R.segnum + b];
+            const uint32_t* data = ix.words + R.woff + rc.off + 1;
+            dst[j] = seg_val(data, desc_lb(rc.desc), e - rc.head, rc);
         }
     }
     return clen;
@@ -438,10 +575,9 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
 
 // HS_SegLRC probe: O(1) bucket, fit window, exact fallback
 // (HS_SegLRC_Intersection_kernel.cu:334-401 plus the fallback it lacks).
-template <int SEG>
-__device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, uint32_t* cand, uint32_t nlive)
+__device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive)
 {
-    constexpr int V = SEG / 64;
+    constexpr int V = kV;
     const uint32_t ln = lane_id();
     const uint32_t shift = (uint32_t)R.icpt, nb = R.nseg;
     const uint32_t* starts = A.ix.hash + R.aux;
@@ -451,19 +587,28 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, uint32_t* can
     for (int i = 0; i < V; ++i) {
         const uint32_t e = ln + 64 * i;
         found[i] = false;
-        v[i] = 0;
-        if (e < nlive) {
-            v[i] = cand[e];
-            const uint32_t b = v[i] >> shift;
-            if (b < nb) {
-                const uint32_t st = starts[b], len = starts[b + 1] - st;
-                if (len) {
-                    const uint32_t* blk = A.ix.words + R.woff + A.ix.seg_off[R.segnum + b];
-                    found[i] = fit_block_search(blk, len, A.ix.fit[R.segnum + b], A.ix.base[R.segnum + b], v[i],
-                                                A.search != SEARCH_BINARY);
-                }
+        v[i] = e < nlive ? W.cand[e] : kNone;
+    }
+#pragma unroll
+    for (int i = 0; i < V; i += 2) {
+        SegRec rec[2];
+        uint32_t len[2];
+        bool ok[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const uint32_t b = v[i + x] >> shift;
+            ok[x] = ln + 64 * (i + x) < nlive && b < nb;
+            len[x] = 0;
+            if (ok[x]) {
+                rec[x] = A.ix.rec[R.segnum + b];
+                len[x] = starts[b + 1] - rec[x].head;
             }
         }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+            if (ok[x] && len[x] > 0)
+                found[i + x] = seg_find(A.ix.words + R.woff + rec[x].off + 1, len[x], rec[x], v[i + x],
+                                        A.search != SEARCH_BINARY);
     }
     uint32_t cnt = 0, pos[V];
 #pragma unroll
@@ -475,42 +620,44 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, uint32_t* can
     wave_sync();
 #pragma unroll
     for (int i = 0; i < V; ++i)
-        if (found[i]) cand[pos[i]] = v[i];
+        if (found[i]) W.cand[pos[i]] = v[i];
     wave_sync();
     return uni(cnt);
 }
 
-template <int SEG>
-constexpr int wave_lds_words() { return 3 * SEG + 512; }
+template <int CODEC>
+constexpr int wave_lds_words() { return (int)(kChunk + kBufWords + (CODEC == EXC_PARAPFD ? kChunk : 0)); }
 
 template <int CODEC, int SEG>
 __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
 {
-    uint32_t* cand = L;
-    uint32_t* sarr = L + SEG;
-    uint32_t* dlist = L + 2 * SEG;
-    uint32_t* buf = L + 3 * SEG;
+    constexpr uint32_t U = kChunk / SEG;  // shortest-list segments per unit
+    const LdsWave W{L, L + kChunk, L + kChunk + kBufWords};
     const uint32_t ln = lane_id();
     const uint32_t q = uni(A.unit_q[u]);
     const QueryRec Q = A.queries[q];
     const uint32_t k = u - Q.first_unit;
     const TermRec R0 = A.ix.terms[uni(A.qterms[Q.toff])];
     uint32_t nlive;
-    if constexpr (CODEC == EXC_HS_SEGLRC)
-        nlive = decode_hs_chunk<SEG>(A.ix, R0, k, cand);
-    else
-        nlive = decode_seg_block<CODEC, SEG>(A.ix, R0, k, cand);
+    if constexpr (CODEC == EXC_HS_SEGLRC) {
+        nlive = decode_hs_chunk<kChunk>(A.ix, R0, k, W.cand);
+    } else {
+        const uint32_t s0 = k * U;
+        const uint32_t nu = uni(min(U, R0.nseg - s0));
+        decode_blocks<CODEC, SEG, U>(A.ix, R0, nu, [&](int g) { return s0 + (uint32_t)g; }, W.cand);
+        nlive = (nu - 1) * SEG + seg_len(R0, s0 + nu - 1, SEG);
+    }
     nlive = uni(nlive);
     wave_sync();
     for (uint32_t j = 1; j < Q.tc && nlive; ++j) {
         const TermRec R = A.ix.terms[uni(A.qterms[Q.toff + j])];
         if constexpr (CODEC == EXC_HS_SEGLRC)
-            nlive = probe_hs<SEG>(A, R, cand, nlive);
+            nlive = probe_hs(A, R, W, nlive);
         else
-            nlive = probe_seg<CODEC, SEG>(A, R, cand, nlive, sarr, dlist, buf);
+            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive);
     }
-    uint32_t* out = A.stage + Q.short_off + (uint64_t)k * SEG;
-    for (uint32_t e = ln; e < nlive; e += 64) out[e] = cand[e];
+    uint32_t* out = A.stage + Q.short_off + (uint64_t)k * kChunk;
+    for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];
     if (ln == 0) A.ucount[u] = nlive;
     wave_sync();
 }
@@ -518,16 +665,17 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
 template <int CODEC, int SEG>
 __global__ __launch_bounds__(256) void k_probe(ProbeArgs A)
 {
-    __shared__ uint32_t lds[kWaves * wave_lds_words<SEG>()];
+    __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);
-    uint32_t* L = lds + wid * wave_lds_words<SEG>();
+    uint32_t* L = lds + wid * wave_lds_words<CODEC>();
     if (A.persistent) {
         for (;;) {
             uint32_t x = 0;
-            if (lane_id() == 0) x = atomicAdd(A.counter, 1u);
-            const uint32_t u = uni(__shfl(x, 0));
-            if (u >= A.nunits) break;
-            run_unit<CODEC, SEG>(A, u, L);
+            if (lane_id() == 0) x = atomicAdd(A.counter, kClaim);
+            const uint32_t u0 = uni(__shfl(x, 0));
+            if (u0 >= A.nunits) break;
+            const uint32_t u1 = min(u0 + kClaim, A.nunits);
+            for (uint32_t u = u0; u < u1; ++u) run_unit<CODEC, SEG>(A, u, L);
         }
     } else {
         const uint32_t u = blockIdx.x * kWaves + wid;
@@ -606,7 +754,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
         if (cnt) {
             const uint32_t q = a.unit_q[u];
             const QueryRec Q = a.queries[q];
-            const uint32_t* src = a.stage + Q.short_off + (uint64_t)(u - Q.first_unit) * a.seg;
+            const uint32_t* src = a.stage + Q.short_off + (uint64_t)(u - Q.first_unit) * kChunk;
             uint32_t* dst = a.result + unit_excl(a, u);
             for (uint32_t e = ln; e < cnt; e += 64) dst[e] = src[e];
         }
@@ -624,6 +772,7 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const uint32_t*
                                                     const uint32_t* unit_term, const uint32_t* unit_k,
                                                     uint32_t nunits, uint32_t* out)
 {
+    constexpr int V = SEG / 64;
     __shared__ uint32_t lds[kWaves * SEG];
     const uint32_t wid = uni(threadIdx.x >> 6), ln = lane_id();
     const uint32_t u = blockIdx.x * kWaves + wid;
@@ -632,10 +781,16 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const uint32_t*
     const uint32_t ti = uni(unit_term[u]), k = uni(unit_k[u]);
     const TermRec R = ix.terms[uni(terms[ti])];
     uint32_t n;
-    if constexpr (CODEC == EXC_HS_SEGLRC)
-        n = decode_hs_chunk<SEG>(ix, R, k, L);
-    else
-        n = decode_seg_block<CODEC, SEG>(ix, R, k, L);
+    if constexpr (CODEC == EXC_HS_SEGLRC) {
+        n = decode_hs_chunk<SEG>(ix, R, k, L);  // HS decompression works in chunks of SEG elements
+    } else {
+        const SegRec rec = ix.rec[R.segnum + k];
+        n = uni(seg_len(R, k, SEG));
+        const uint32_t* data = ix.words + R.woff + rec.off + 1;
+        uint32_t f[V];
+        load_fields<CODEC, SEG>(data, desc_lb(rec.desc), n, f);
+        finish_block<CODEC, SEG>(R, k, rec, n, data, f, L);
+    }
     wave_sync();
     uint32_t* o = out + out_off[ti] + (uint64_t)k * SEG;
     for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
@@ -663,12 +818,8 @@ hipError_t decomp_for_seg(const DevIndex& ix, const uint32_t* terms, const uint6
     return hipGetLastError();
 }
 
-}  // namespace
-
-uint32_t probe_waves_per_block() { return kWaves; }
-
 template <int CODEC>
-static int occ_for(uint32_t seg)
+int occ_for(uint32_t seg)
 {
     int nb = 0;
     hipError_t e = (seg == 256)
@@ -676,6 +827,10 @@ static int occ_for(uint32_t seg)
         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 128>), 256, 0);
     return (e == hipSuccess && nb > 0) ? nb : 4;
 }
+
+}  // namespace
+
+uint32_t probe_waves_per_block() { return kWaves; }
 
 int probe_blocks_per_cu(int codec, uint32_t seg)
 {

@@ -64,6 +64,7 @@ struct exc_ctx {
     int persistent = 1;
     int graph = 0;
     int timing = 1;
+    int coop_max = 64;
     // batch plan (host)
     exc_index* ix = nullptr;
     std::vector<QueryRec> queries;
@@ -214,6 +215,10 @@ extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
     case EXC_OPT_PERSISTENT: c->persistent = v != 0; break;
     case EXC_OPT_GRAPH: c->graph = v != 0; break;
     case EXC_OPT_TIMING: c->timing = v != 0; break;
+    case EXC_OPT_COOP_MAX:
+        if (v < 0 || v > 64) return set_error(EXC_ERR_ARG, "coop_max must be in [0, 64]");
+        c->coop_max = (int)v;
+        break;
     default: return set_error(EXC_ERR_ARG, "unknown option %d", key);
     }
     drop_graph(c);
@@ -319,6 +324,33 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
         if (codec == EXC_LRC && !(R.slope > 0.0f)) { R.slope = 1.0f; R.range_l = R.range_r = 4.0e9f; }
     }
 
+    // one 32-byte record per segment / bucket: offset, descriptor, fit, base, head
+    std::vector<SegRec> srec(h->nseg);
+    for (uint32_t t = 0; t < nt; ++t) {
+        const TermRec& R = recs[t];
+        const uint32_t* ht = hs ? h->hash_info + h->hash_offset[t] + 2 : nullptr;
+        for (uint32_t s = 0; s < R.nseg; ++s) {
+            const uint64_t g = (uint64_t)R.segnum + s;
+            SegRec& q = srec[g];
+            q.off = h->seg_offset[g];
+            const bool empty = hs ? ht[s + 1] == ht[s] : false;
+            q.desc = empty ? 0u : h->ind2[R.woff + q.off];
+            q.slope = 1.0f;
+            q.icpt = 0.0f;
+            q.rl = q.rr = 0;
+            q.base = 0;
+            if (codec == EXC_SEGLRC || hs) {
+                q.slope = h->seg_info[g].fSlope;
+                q.icpt = h->seg_info[g].fIntercept;
+                q.rl = h->seg_info[g].nRangeLeft;
+                q.rr = h->seg_info[g].nRangeRight;
+                q.base = h->seg_base[g];
+                if (!(q.slope > 0.0f)) { q.slope = 1.0f; q.rl = q.rr = 0x7fffffffu; }
+            }
+            q.head = hs ? ht[s] : h->seg_head[g];
+        }
+    }
+
     auto* ix = new exc_index();
     ix->ctx = c;
     ix->codec = codec;
@@ -339,12 +371,8 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->dix.seg = ix->seg;
     ix->dix.words = (const uint32_t*)up(h->ind2, W * 4);
     ix->dix.terms = (const TermRec*)up(recs.data(), recs.size() * sizeof(TermRec));
-    ix->dix.seg_off = (const uint32_t*)up(h->seg_offset, h->nseg * 4);
+    ix->dix.rec = (const SegRec*)up(srec.data(), srec.size() * sizeof(SegRec));
     if (!hs) ix->dix.seg_head = (const uint32_t*)up(h->seg_head, h->nseg * 4);
-    if (codec == EXC_SEGLRC || hs) {
-        ix->dix.fit = (const float4*)up(h->seg_info, h->nseg * sizeof(exc_seg_info));
-        ix->dix.base = (const int32_t*)up(h->seg_base, h->nseg * 4);
-    }
     if (hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
     if (rc) {
         exc_index_free(ix);
@@ -478,7 +506,7 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         Q.tc = tc;
         Q.first_unit = (uint32_t)units;
         const uint32_t n0 = tc ? ix->n[tmp[0]] : 0;
-        Q.nunits = (n0 + ix->seg - 1) / ix->seg;
+        Q.nunits = (n0 + kChunk - 1) / kChunk;
         Q.short_off = c->short_sum;
         c->short_sum += n0;
         c->qterms.insert(c->qterms.end(), tmp.begin(), tmp.end());
@@ -522,11 +550,7 @@ int upload_plan(exc_ctx* c)
 
 // Threads per short-list element when autotuned: random access (1 lane per
 // candidate) for the random-access codecs, wave-cooperative decode for ParaPFD.
-int pick_tpe(exc_ctx* c)
-{
-    if (c->tpe) return c->tpe;
-    return c->ix->codec == EXC_PARAPFD ? 64 : 1;
-}
+int pick_tpe(exc_ctx* c) { return c->tpe; }
 
 int enqueue_invoke(exc_ctx* c, int tpe, bool timed)
 {
@@ -541,6 +565,7 @@ int enqueue_invoke(exc_ctx* c, int tpe, bool timed)
     a.search = c->search;
     a.tpe = tpe;
     a.persistent = c->persistent;
+    a.coop_max = c->coop_max;
     a.stage = dptr<uint32_t>(c->d_stage);
     a.ucount = dptr<uint32_t>(c->d_ucount);
     a.counter = dptr<uint32_t>(c->d_counter);
@@ -564,7 +589,6 @@ int enqueue_invoke(exc_ctx* c, int tpe, bool timed)
     s.stage = a.stage;
     s.nunits = c->nunits;
     s.nq = c->nq;
-    s.seg = ix->seg;
     s.uoff = dptr<uint32_t>(c->d_uoff);
     s.bsum = dptr<uint32_t>(c->d_bsum);
     s.result = dptr<uint32_t>(c->d_result);

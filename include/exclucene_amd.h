@@ -102,7 +102,9 @@ enum {
     EXC_OPT_TPE = 1,        /* threads per short-list element: 0 autotuned, 1 random access, 64 wave-cooperative decode */
     EXC_OPT_PERSISTENT = 2, /* 1: persistent work-queue launch, 0: one wave per work unit */
     EXC_OPT_GRAPH = 3,      /* 1: replay kernelInvoke as a captured hipGraph */
-    EXC_OPT_TIMING = 4      /* 1: hipEvent timing around each kernel of kernelInvoke */
+    EXC_OPT_TIMING = 4,     /* 1: hipEvent timing around each kernel of kernelInvoke */
+    EXC_OPT_COOP_MAX = 5    /* tpe 0: a list whose needed segments form a run of at most this many
+                               is decoded wave-cooperatively, longer runs use random access */
 };
 int exc_ctx_set_option(exc_ctx *ctx, int key, int64_t value);
 /* Times (ms) of the last kernelInvoke: probe kernel and the whole invoke. */

@@ -45,19 +45,22 @@ def test_intersect_matches_oracle(ctx, codec, seg, frac, shift):
     ocum, ores = oracle.eval_queries(ref, qs)
     dix = ctx.upload(built)
     try:
-        tpes = [1, 64] if codec in (ea.LRC, ea.SEGLRC) else [0]
+        modes = [(1, 0), (64, 0), (0, 0), (0, 2), (0, 16)] if codec in (ea.LRC, ea.SEGLRC) else [(0, 2), (0, 64)]
         for search in (0, 1):
-            for tpe in tpes:
+            for tpe, coop in modes:
                 for persistent in (0, 1):
                     ctx.set_option(ea.OPT_SEARCH, search)
                     ctx.set_option(ea.OPT_TPE, tpe)
+                    ctx.set_option(ea.OPT_COOP_MAX, coop)
                     ctx.set_option(ea.OPT_PERSISTENT, persistent)
-                    print("case codec=%d seg=%d search=%d tpe=%d persistent=%d" % (codec, seg, search, tpe, persistent), flush=True)
+                    print("case codec=%d seg=%d search=%d tpe=%d coop=%d persistent=%d"
+                          % (codec, seg, search, tpe, coop, persistent), flush=True)
                     cum, res = ctx.intersect(dix, qs)
                     assert np.array_equal(cum, ocum), (search, tpe, persistent)
                     assert np.array_equal(res, ores), (search, tpe, persistent)
     finally:
         ctx.set_option(ea.OPT_TPE, 0)
+        ctx.set_option(ea.OPT_COOP_MAX, 64)
         ctx.set_option(ea.OPT_SEARCH, 1)
         ctx.set_option(ea.OPT_PERSISTENT, 1)
         dix.close()
