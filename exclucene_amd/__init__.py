@@ -330,6 +330,7 @@ class Context:
         if ucnt is not None:
             for q in range(nq):
                 ts = qterms[int(qoff[q]):int(qoff[q + 1])]
+                ts = ts[ts < len(ucnt)]  # out-of-range terms are reported by the library
                 if len(ts):
                     cap += int(ucnt[ts].min())
         else:

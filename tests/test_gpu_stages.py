@@ -1,0 +1,136 @@
+"""GPU tests of the drop-in surface: the reference's stage sequence over a
+.query byte stream with threshold batching, hipGraph replays, the persistent
+work queue, index files written and read back in the reference layout, and
+error reporting instead of exit()."""
+import numpy as np
+import pytest
+
+import exclucene_amd as ea
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _corpus(seed=21, nl=40):
+    rng = np.random.default_rng(seed)
+    lens = np.unique(np.concatenate([[1, 5, 300], (10 ** rng.uniform(1, 5.5, nl)).astype(int)]))
+    return [np.sort(rng.choice(1 << 24, int(n), replace=False)).astype(np.uint32) for n in lens]
+
+
+def _queries(nl, nq, seed):
+    rng = np.random.default_rng(seed)
+    return [[int(x) for x in rng.choice(nl, int(rng.integers(1, 6)), replace=False)] for _ in range(nq)]
+
+
+@pytest.mark.parametrize("codec,frac", [(ea.SEGLRC, 0.0), (ea.PARAPFD, 0.1), (ea.HS_SEGLRC, 0.0)])
+def test_stage_sequence_with_thresholds(ctx, codec, frac):
+    """Run()-style loop: CPUPreprocess -> htod -> kernelInvoke -> dtoh -> verify
+    over the whole stream at several thresholds; every threshold must give the
+    same checksums, equal to the oracle's (LRC_Intersection.cu:449-540)."""
+    lists = _corpus()
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, codec, seg=128, frac=frac)
+    ref = oracle.build_index(lists, codec, seg=128, frac=frac)
+    qs = _queries(len(lists), 300, 5)
+    ocum, ores = oracle.eval_queries(ref, qs)
+    ocs = oracle.checksum(ocum, ores)
+    stream = ea.queries_to_stream(qs)
+    dix = ctx.upload(built)
+    try:
+        for threshold in (0, 1000, 50_000, 1 << 30):
+            pos, cs, batches, allres = 0, [0, 0, 0], 0, []
+            while pos < len(stream):
+                bi, pos = ctx.CPUPreprocess(dix, stream, pos, threshold)
+                ctx.htodTransfer(bi)
+                ctx.kernelInvoke(bi)
+                cum, res = ctx.dtohTransfer(bi)
+                c = ctx.verify(bi, cum, res)
+                cs = [cs[0] + c[0], cs[1] + c[1], cs[2] ^ c[2]]
+                allres.append(res)
+                batches += 1
+            assert tuple(cs) == ocs, threshold
+            assert np.array_equal(np.concatenate(allres), ores)
+            if threshold == 0:
+                assert batches == len(qs)  # every query crosses a zero threshold on its own
+    finally:
+        dix.close()
+
+
+def test_graph_replays_match_eager(ctx):
+    lists = _corpus(22)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
+    dix = ctx.upload(built)
+    qs = _queries(len(lists), 200, 6)
+    try:
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        ecum, eres = ctx.intersect(dix, qs)
+        for persistent in (1, 0):
+            ctx.set_option(ea.OPT_PERSISTENT, persistent)
+            ctx.set_option(ea.OPT_GRAPH, 1)
+            bi = ctx.plan(dix, qs)
+            for rep in range(4):
+                ctx.kernelInvoke(bi)  # first call captures, later calls replay the same graph
+                cum, res = ctx.dtohTransfer(bi)
+                assert np.array_equal(cum, ecum) and np.array_equal(res, eres), (persistent, rep)
+                # the program's own eager work and a synchronisation between replays
+                out = ctx.decompress(dix, [0, 3, 7])
+                assert np.array_equal(out, np.concatenate([lists[0], lists[3], lists[7]]))
+    finally:
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        dix.close()
+
+
+@pytest.mark.parametrize("codec,frac,shift", [(ea.LRC, 0.0, 0), (ea.SEGLRC, 0.0, 0), (ea.HS_SEGLRC, 0.0, 13),
+                                              (ea.PARAPFD, 0.1, 0)])
+def test_files_roundtrip_on_device(ctx, tmp_path, codec, frac, shift):
+    lists = _corpus(23, 12)
+    docs, off = ea.lists_to_flat(lists)
+    seg = 256 if codec == ea.LRC else 128
+    built = ea.build_index(docs, off, codec, seg=seg, frac=frac, hs_shift=shift)
+    built.save(str(tmp_path), "idx")
+    dix = ctx.load_files(codec, seg, str(tmp_path), "idx")
+    try:
+        out = ctx.decompress(ea.DeviceIndex(ctx, dix.handle, built), list(range(len(lists))))
+        assert np.array_equal(out, np.concatenate(lists))
+    finally:
+        dix.close()
+
+
+def test_errors_are_reported_not_fatal(ctx):
+    lists = _corpus(24, 5)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
+    dix = ctx.upload(built)
+    try:
+        with pytest.raises(ea.ExcError) as e:
+            ctx.intersect(dix, [[0, 99999]])
+        assert e.value.code == -1
+        with pytest.raises(ea.ExcError):
+            ctx.set_option(ea.OPT_TPE, 7)
+        # capacity: result buffer too small
+        bi = ctx.plan(dix, [[len(lists) - 1]])
+        ctx.kernelInvoke(bi)
+        with pytest.raises(ea.ExcError) as e:
+            ctx.dtohTransfer(bi, cap=1)
+        assert e.value.code == -3
+        # still usable afterwards
+        cum, res = ctx.intersect(dix, [[0, 1]])
+        assert len(cum) == 1
+    finally:
+        dix.close()
+
+
+def test_budget_admission():
+    c = ea.Context(0, budget=1 << 20)
+    try:
+        lists = _corpus(25, 30)
+        docs, off = ea.lists_to_flat(lists)
+        built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
+        if built.compressed_bytes() > (1 << 20):
+            with pytest.raises(ea.ExcError) as e:
+                c.upload(built)
+            assert e.value.code == -6
+    finally:
+        c.close()
