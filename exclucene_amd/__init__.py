@@ -29,8 +29,9 @@ EXPORTS = [
     "exc_CPUPreprocess", "exc_htodTransfer", "exc_kernelInvoke", "exc_dtohTransfer", "exc_verify",
     "exc_intersect", "exc_plan", "exc_decompress",
     "exc_build_index", "exc_built_view", "exc_built_free", "exc_index_save_files",
-    "exc_synth_lists",
+    "exc_synth_lists", "exc_index_train_models", "exc_index_model_info", "exc_mlp_gradient",
 ]
+MODEL_PLA, MODEL_MLP = 1, 2
 
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -45,6 +46,16 @@ class IndexHost(ctypes.Structure):
                 ("seg_head", _vp), ("seg_median", _vp), ("lrc_info", _vp), ("seg_info", _vp),
                 ("seg_base", _vp), ("list_info", _vp), ("hash_offset", _vp), ("hash_info", _vp),
                 ("hash_words", _u64)]
+
+
+class ModelStats(ctypes.Structure):
+    _fields_ = [("nmodels", _u32), ("max_err", ctypes.c_float), ("mean_err", ctypes.c_float),
+                ("train_ms", ctypes.c_float)]
+
+
+class ModelHost(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("n", _u32), ("err", ctypes.c_float), ("x0", ctypes.c_float),
+                ("xs", ctypes.c_float), ("nparams", _u32), ("params", ctypes.c_float * 257)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -108,6 +119,10 @@ def lib():
         L.exc_built_free.argtypes = [_vp]
         L.exc_index_save_files.argtypes = [ctypes.POINTER(IndexHost), ctypes.c_char_p, ctypes.c_char_p]
         L.exc_synth_lists.argtypes = [_u64, _u32, _vp, _u32, ctypes.c_int, _vp, _vp]
+        L.exc_index_train_models.argtypes = [_vp, _vp, ctypes.c_int, _u32, _u32, _u32, ctypes.c_float, _u64,
+                                             ctypes.POINTER(ModelStats)]
+        L.exc_index_model_info.argtypes = [_vp, _vp, _u32, ctypes.POINTER(ModelHost)]
+        L.exc_mlp_gradient.argtypes = [_vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float)]
         _lib = L
     return _lib
 
@@ -341,6 +356,27 @@ class Context:
         _check(lib().exc_intersect(self._h, index.handle, _p(qterms), _p(qoff), nq, _p(cum), _p(res), cap,
                                    ctypes.byref(n)))
         return cum[:nq], res[:int(n.value)]
+
+    # -- learned position models
+    def train_models(self, index, kind=MODEL_PLA, min_len=65536, epochs=300, samples=4096, lr=3e-4, seed=4):
+        st = ModelStats()
+        _check(lib().exc_index_train_models(self._h, index.handle, kind, min_len, epochs, samples, lr, seed,
+                                            ctypes.byref(st)))
+        return {"nmodels": st.nmodels, "max_err": st.max_err, "mean_err": st.mean_err, "train_ms": st.train_ms}
+
+    def model_info(self, index, term):
+        m = ModelHost()
+        _check(lib().exc_index_model_info(self._h, index.handle, term, ctypes.byref(m)))
+        return {"kind": m.kind, "n": m.n, "err": m.err, "x0": m.x0, "xs": m.xs,
+                "params": np.array(m.params[:m.nparams], np.float32)}
+
+    def mlp_gradient(self, params, docs, samples):
+        p = np.ascontiguousarray(params, np.float32)
+        d = np.ascontiguousarray(docs, np.uint32)
+        g = np.zeros(97, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().exc_mlp_gradient(self._h, _p(p), _p(d), len(d), samples, _p(g), ctypes.byref(loss)))
+        return g, loss.value
 
     def decompress(self, index, terms):
         terms = np.ascontiguousarray(terms, np.uint32)

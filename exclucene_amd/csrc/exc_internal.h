@@ -31,23 +31,21 @@ struct alignas(16) TermRec {
 static_assert(sizeof(TermRec) == 64, "TermRec is one 64-byte record");
 
 constexpr uint32_t kNoModel = 0xffffffffu;
-constexpr uint32_t kPlaPieces = 256;
-constexpr uint32_t kMlpHidden = 32;
 
-// Learned position model of one list (docID -> position), fp32.
-//   PLA: 256 pieces; key[p] = first docID of piece p, pos = a[p] * (x - key[p]) + b[p]
-//   MLP: 1-32-1 ReLU on x = docID * xscale; pos = (w2 . relu(w1 x + b1) + b2) * yscale
-// err = max |pos - true position| over the list, measured with the same
-// inference code the search uses.
-struct alignas(16) Model {
-    uint32_t kind;     // 1 PLA, 2 MLP
-    uint32_t n;        // list length the model was fitted to
-    float err;         // recorded max error (positions)
-    float xscale;
-    float yscale;
-    uint32_t param_off;  // float offset into the model parameter pool
-    uint32_t pad[2];
+// Learned position model of one list (docID -> position), fp32; inference in
+// exc_model.h.  err = max |prediction - true position| over the list's own
+// elements, measured on the device with the same inference code the search uses.
+struct alignas(32) Model {
+    uint32_t kind;       // kModelPla | kModelMlp
+    uint32_t n;          // list length
+    float err;           // recorded max error (positions)
+    float x0;            // input offset (first docID)
+    float xs;            // input scale
+    uint32_t param_off;  // float offset into the parameter pool
+    uint32_t term;       // the term the model belongs to
+    uint32_t pad;
 };
+static_assert(sizeof(Model) == 32, "Model is 32 bytes");
 
 // Per-segment (per-bucket) record derived at upload so one 32-byte load gives
 // a probe everything about a block: its word offset, ParaPFor descriptor,
@@ -133,6 +131,24 @@ uint32_t probe_waves_per_block();
 }  // namespace exc
 
 namespace exc {
+// learned models (learned.hip)
+struct TrainArgs {
+    const uint32_t* dec;       // decoded lists, concatenated
+    const uint64_t* dec_off;   // per model: offset of its list in dec
+    Model* models;             // per model (header written by the fit kernels)
+    float* params;             // parameter pool
+    uint32_t* err_bits;        // per model: max error as float bits (atomicMax)
+    uint32_t nmodels;
+    uint32_t epochs;           // MLP: Adam epochs (0: gradient only)
+    uint32_t samples;          // MLP: samples per list
+    float lr;
+    uint64_t seed;
+    float* grad_out;           // MLP gradient-only mode: per model kMlpParams floats + loss
+};
+hipError_t launch_pla_fit(const TrainArgs& a, hipStream_t s);
+hipError_t launch_mlp_train(const TrainArgs& a, hipStream_t s);
+hipError_t launch_model_err(const TrainArgs& a, uint64_t total_elems, hipStream_t s);
+
 // persistent-launch sizing: resident probe workgroups per CU for this codec / segment size
 int probe_blocks_per_cu(int codec, uint32_t seg);
 }  // namespace exc

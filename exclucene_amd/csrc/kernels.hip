@@ -30,6 +30,7 @@
 //    per-query output and cumulative counts.
 #include "exc_format.h"
 #include "exc_internal.h"
+#include "exc_model.h"
 
 namespace exc {
 namespace {
@@ -321,6 +322,28 @@ __device__ __forceinline__ void fit_window(const TermRec& R, uint32_t v, int64_t
     wb = phi <= 0.0f ? 1 : (int64_t)(phi / SEG) + 2;
 }
 
+// Window [wa, wb] for "number of segment heads <= v" over [vmin, vmax] from
+// the list's learned model: prediction +- its recorded max error.
+template <int SEG>
+__device__ __forceinline__ void model_window(const DevIndex& ix, uint32_t mid, uint32_t vmin, uint32_t vmax,
+                                             int64_t& wa, int64_t& wb)
+{
+    const Model M = ix.models[mid];
+    const float* prm = ix.mparams + M.param_off;
+    float plo, phi;
+    if (M.kind == kModelPla) {
+        plo = pla_pred(prm, M.x0, M.xs, vmin);
+        phi = pla_pred(prm, M.x0, M.xs, vmax);
+    } else {
+        plo = mlp_pred(prm, M.x0, M.xs, M.n, vmin);
+        phi = mlp_pred(prm, M.x0, M.xs, M.n, vmax);
+    }
+    plo = fminf(fmaxf(plo - M.err - 2.0f, -1.0e9f), 4.0e9f);
+    phi = fminf(fmaxf(phi + M.err + 2.0f, -1.0e9f), 4.0e9f);
+    wa = plo <= 0.0f ? 0 : (int64_t)(plo / SEG);
+    wb = phi <= 0.0f ? 1 : (int64_t)(phi / SEG) + 2;
+}
+
 struct LdsWave {
     uint32_t* cand;   // [kChunk]    live candidates (sorted)
     uint32_t* buf;    // [kBufWords] decoded segments
@@ -344,8 +367,12 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     bool small = false;
     if (A.search != SEARCH_BINARY && nseg > 64) {
         int64_t wa, t0, t1, wb;
-        fit_window<CODEC, SEG>(R, vmin, wa, t0);
-        fit_window<CODEC, SEG>(R, vmax, t1, wb);
+        if (A.search == SEARCH_LEARNED && R.model != kNoModel) {
+            model_window<SEG>(A.ix, R.model, vmin, vmax, wa, wb);
+        } else {
+            fit_window<CODEC, SEG>(R, vmin, wa, t0);
+            fit_window<CODEC, SEG>(R, vmax, t1, wb);
+        }
         wa = min(wa, nseg);
         wb = min(max(wb, wa), nseg);
         if (wb - wa + 2 <= 64) {

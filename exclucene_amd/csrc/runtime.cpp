@@ -20,6 +20,7 @@
 
 #include "exc_format.h"
 #include "exc_internal.h"
+#include "exc_model.h"
 
 using namespace exc;
 
@@ -89,6 +90,10 @@ struct exc_index {
     uint32_t nterms = 0;
     uint64_t nseg = 0;
     std::vector<uint32_t> n;       // list lengths (host copy for planning)
+    std::vector<TermRec> recs;     // host copy of the device term records
+    TermRec* d_terms = nullptr;
+    std::vector<Model> models;     // learned models (host copy)
+    DevBuf d_models, d_params;
     std::vector<DevBuf> bufs;
     uint64_t bytes = 0;
     DevIndex dix{};
@@ -371,6 +376,8 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->dix.seg = ix->seg;
     ix->dix.words = (const uint32_t*)up(h->ind2, W * 4);
     ix->dix.terms = (const TermRec*)up(recs.data(), recs.size() * sizeof(TermRec));
+    ix->d_terms = const_cast<TermRec*>(ix->dix.terms);
+    ix->recs = recs;
     ix->dix.rec = (const SegRec*)up(srec.data(), srec.size() * sizeof(SegRec));
     if (!hs) ix->dix.seg_head = (const uint32_t*)up(h->seg_head, h->nseg * 4);
     if (hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
@@ -393,6 +400,8 @@ extern "C" void exc_index_free(exc_index* ix)
             drop_graph(ix->ctx);
         }
         for (auto& b : ix->bufs) dev_free(ix->ctx, b);
+        dev_free(ix->ctx, ix->d_models);
+        dev_free(ix->ctx, ix->d_params);
     }
     delete ix;
 }
@@ -759,6 +768,45 @@ extern "C" int exc_intersect(exc_ctx* c, exc_index* ix, const uint32_t* qterms, 
 
 // ---------------------------------------------------------------- decompress
 
+namespace {
+
+// Decode whole lists of `terms` into dec (device), concatenated; off[i] =
+// offset of terms[i].  Temporary buffers are freed before returning.
+int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& terms, DevBuf& dec,
+                     std::vector<uint64_t>& off)
+{
+    const uint32_t nt = (uint32_t)terms.size();
+    off.assign(nt + 1, 0);
+    std::vector<uint32_t> ut, uk;
+    for (uint32_t i = 0; i < nt; ++i) {
+        const uint32_t n = ix->n[terms[i]];
+        off[i + 1] = off[i] + n;
+        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; ++k) { ut.push_back(i); uk.push_back(k); }
+    }
+    if (!off[nt]) return EXC_OK;
+    DevBuf dt, doff, dut, duk;
+    int rc;
+    if ((rc = dev_alloc(c, dec, off[nt] * 4)) || (rc = dev_alloc(c, dt, nt * 4)) ||
+        (rc = dev_alloc(c, doff, (nt + 1) * 8)) || (rc = dev_alloc(c, dut, ut.size() * 4)) ||
+        (rc = dev_alloc(c, duk, uk.size() * 4))) {
+        for (DevBuf* b : {&dt, &doff, &dut, &duk}) dev_free(c, *b);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(dt.p, terms.data(), nt * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(doff.p, off.data(), (nt + 1) * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dut.p, ut.data(), ut.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(duk.p, uk.data(), uk.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_decompress(ix->dix, dptr<uint32_t>(dt), dptr<uint64_t>(doff), dptr<uint32_t>(dut),
+                              dptr<uint32_t>(duk), (uint32_t)ut.size(), dptr<uint32_t>(dec), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    for (DevBuf* b : {&dt, &doff, &dut, &duk}) dev_free(c, *b);
+    if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "decode: %s", hipGetErrorString(e));
+    return EXC_OK;
+}
+
+}  // namespace
+
 extern "C" int exc_decompress(exc_ctx* c, exc_index* ix, const uint32_t* terms, uint32_t nt, uint32_t* out,
                               uint64_t cap, uint64_t* n_out)
 {
@@ -795,5 +843,171 @@ extern "C" int exc_decompress(exc_ctx* c, exc_index* ix, const uint32_t* terms, 
     if (e == hipSuccess) e = hipMemcpy(out, dout.p, off[nt] * 4, hipMemcpyDeviceToHost);
     for (DevBuf* b : {&dt, &doff, &dut, &duk, &dout}) dev_free(c, *b);
     if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "exc_decompress: %s", hipGetErrorString(e));
+    return EXC_OK;
+}
+
+// ---------------------------------------------------------------- learned models
+
+extern "C" int exc_index_train_models(exc_ctx* c, exc_index* ix, int kind, uint32_t min_len, uint32_t epochs,
+                                      uint32_t samples, float lr, uint64_t seed, exc_model_stats* st)
+{
+    if (!c || !ix) return set_error(EXC_ERR_ARG, "exc_index_train_models: null argument");
+    if (kind != EXC_MODEL_PLA && kind != EXC_MODEL_MLP) return set_error(EXC_ERR_ARG, "unknown model kind %d", kind);
+    if (ix->codec == EXC_HS_SEGLRC) return set_error(EXC_ERR_ARG, "HS_SegLRC lists are searched by bucket, not by position");
+    if (kind == EXC_MODEL_MLP && (epochs == 0 || samples == 0)) return set_error(EXC_ERR_ARG, "MLP needs epochs and samples");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<uint32_t> terms;
+    for (uint32_t t = 0; t < ix->nterms; ++t)
+        if (ix->n[t] > min_len && ix->n[t] >= 2) terms.push_back(t);
+    const uint32_t nm = (uint32_t)terms.size();
+    const uint32_t P = kind == EXC_MODEL_PLA ? 260u : 100u;  // floats per model (padded)
+    ix->models.assign(nm, Model{});
+    for (uint32_t m = 0; m < nm; ++m) {
+        Model& M = ix->models[m];
+        std::memset(&M, 0, sizeof M);
+        M.kind = (uint32_t)kind;
+        M.n = ix->n[terms[m]];
+        M.param_off = m * P;
+        M.term = terms[m];
+    }
+    int rc;
+    DevBuf errb;
+    if ((rc = dev_alloc(c, ix->d_models, std::max<size_t>(1, nm) * sizeof(Model))) ||
+        (rc = dev_alloc(c, ix->d_params, std::max<size_t>(1, (size_t)nm * P) * 4)) ||
+        (rc = dev_alloc(c, errb, std::max<size_t>(1, nm) * 4)))
+        return rc;
+    if (nm) {
+        HIP_TRY(hipMemcpy(ix->d_models.p, ix->models.data(), nm * sizeof(Model), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(errb.p, 0, nm * 4));
+    }
+    (void)hipEventRecord(c->ev[0], c->stream);
+    const uint64_t kBatch = 64ull << 20;  // decoded elements per training batch
+    for (uint32_t m0 = 0; m0 < nm;) {
+        uint32_t m1 = m0;
+        uint64_t tot = 0;
+        while (m1 < nm && (m1 == m0 || tot + ix->n[terms[m1]] <= kBatch)) tot += ix->n[terms[m1++]];
+        std::vector<uint32_t> bt(terms.begin() + m0, terms.begin() + m1);
+        DevBuf dec, doff;
+        std::vector<uint64_t> off;
+        if ((rc = decode_to_device(c, ix, bt, dec, off)) || (rc = dev_alloc(c, doff, off.size() * 8))) {
+            dev_free(c, dec);
+            dev_free(c, errb);
+            return rc;
+        }
+        hipError_t e = hipMemcpy(doff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice);
+        TrainArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.dec = dptr<uint32_t>(dec);
+        a.dec_off = dptr<uint64_t>(doff);
+        a.models = dptr<Model>(ix->d_models) + m0;
+        a.params = dptr<float>(ix->d_params);
+        a.err_bits = dptr<uint32_t>(errb) + m0;
+        a.nmodels = m1 - m0;
+        a.epochs = epochs;
+        a.samples = samples;
+        a.lr = lr;
+        a.seed = seed;
+        if (e == hipSuccess) e = kind == EXC_MODEL_PLA ? launch_pla_fit(a, c->stream) : launch_mlp_train(a, c->stream);
+        if (e == hipSuccess) e = launch_model_err(a, tot, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dev_free(c, dec);
+        dev_free(c, doff);
+        if (e != hipSuccess) {
+            dev_free(c, errb);
+            return set_error(EXC_ERR_DEVICE, "model training: %s", hipGetErrorString(e));
+        }
+        m0 = m1;
+    }
+    (void)hipEventRecord(c->ev[3], c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[3]);
+    std::vector<uint32_t> eb(nm);
+    if (nm) {
+        HIP_TRY(hipMemcpy(ix->models.data(), ix->d_models.p, nm * sizeof(Model), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(eb.data(), errb.p, nm * 4, hipMemcpyDeviceToHost));
+    }
+    dev_free(c, errb);
+    double sum = 0;
+    float mx = 0;
+    for (uint32_t m = 0; m < nm; ++m) {
+        float e;
+        std::memcpy(&e, &eb[m], 4);
+        ix->models[m].err = e;
+        sum += e;
+        mx = std::max(mx, e);
+    }
+    for (auto& R : ix->recs) R.model = kNoModel;
+    for (uint32_t m = 0; m < nm; ++m) ix->recs[terms[m]].model = m;
+    if (nm) HIP_TRY(hipMemcpy(ix->d_models.p, ix->models.data(), nm * sizeof(Model), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ix->d_terms, ix->recs.data(), ix->recs.size() * sizeof(TermRec), hipMemcpyHostToDevice));
+    ix->dix.models = dptr<Model>(ix->d_models);
+    ix->dix.mparams = dptr<float>(ix->d_params);
+    if (c->ix == ix) drop_graph(c);
+    if (st) {
+        st->nmodels = nm;
+        st->max_err = mx;
+        st->mean_err = nm ? (float)(sum / nm) : 0.0f;
+        st->train_ms = ms;
+    }
+    return EXC_OK;
+}
+
+extern "C" int exc_index_model_info(exc_ctx* c, exc_index* ix, uint32_t term, exc_model_host* out)
+{
+    if (!c || !ix || !out || term >= ix->nterms) return set_error(EXC_ERR_ARG, "exc_index_model_info: bad argument");
+    const uint32_t m = ix->recs[term].model;
+    if (m == kNoModel || m >= ix->models.size()) return set_error(EXC_ERR_ARG, "term %u has no model", term);
+    const Model& M = ix->models[m];
+    std::memset(out, 0, sizeof *out);
+    out->kind = (int32_t)M.kind;
+    out->n = M.n;
+    out->err = M.err;
+    out->x0 = M.x0;
+    out->xs = M.xs;
+    out->nparams = M.kind == kModelPla ? (uint32_t)kPlaKnots : (uint32_t)kMlpParams;
+    HIP_TRY(hipMemcpy(out->params, dptr<float>(ix->d_params) + M.param_off, out->nparams * 4, hipMemcpyDeviceToHost));
+    return EXC_OK;
+}
+
+extern "C" int exc_mlp_gradient(exc_ctx* c, const float* params, const uint32_t* docs, uint32_t n, uint32_t samples,
+                                float* grad, float* loss)
+{
+    if (!c || !params || !docs || !grad || n < 2 || samples == 0) return set_error(EXC_ERR_ARG, "exc_mlp_gradient: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    DevBuf dl, doff, dm, dp, dg;
+    int rc;
+    if ((rc = dev_alloc(c, dl, (size_t)n * 4)) || (rc = dev_alloc(c, doff, 8)) || (rc = dev_alloc(c, dm, sizeof(Model))) ||
+        (rc = dev_alloc(c, dp, 128 * 4)) || (rc = dev_alloc(c, dg, 128 * 4))) {
+        for (DevBuf* b : {&dl, &doff, &dm, &dp, &dg}) dev_free(c, *b);
+        return rc;
+    }
+    Model M;
+    std::memset(&M, 0, sizeof M);
+    M.kind = kModelMlp;
+    M.n = n;
+    const uint64_t zero = 0;
+    hipError_t e = hipMemcpy(dl.p, docs, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(doff.p, &zero, 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dm.p, &M, sizeof M, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp.p, params, kMlpParams * 4, hipMemcpyHostToDevice);
+    TrainArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.dec = dptr<uint32_t>(dl);
+    a.dec_off = dptr<uint64_t>(doff);
+    a.models = dptr<Model>(dm);
+    a.params = dptr<float>(dp);
+    a.nmodels = 1;
+    a.epochs = 0;
+    a.samples = samples;
+    a.grad_out = dptr<float>(dg);
+    if (e == hipSuccess) e = launch_mlp_train(a, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float g[kMlpParams + 1];
+    if (e == hipSuccess) e = hipMemcpy(g, dg.p, sizeof g, hipMemcpyDeviceToHost);
+    for (DevBuf* b : {&dl, &doff, &dm, &dp, &dg}) dev_free(c, *b);
+    if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "exc_mlp_gradient: %s", hipGetErrorString(e));
+    std::memcpy(grad, g, kMlpParams * 4);
+    if (loss) *loss = g[kMlpParams];
     return EXC_OK;
 }

@@ -149,6 +149,33 @@ int exc_plan(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t
 int exc_decompress(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, uint32_t nterms,
                    uint32_t *out, uint64_t out_cap, uint64_t *n_out);
 
+/* ---- learned position models.  No reference counterpart: the reference's
+ *      precursor is the single linear-regression window of
+ *      src/gpu/Uncompressed/src/cuda_runRegressionAlgorithm/cuda_linearRegression.cu:792-873
+ *      (and the list-level fit of .regression_info_list), used there without
+ *      a fallback.  Every list longer than min_len gets a model trained on the
+ *      GPU from the decoder's (docID, position) pairs; its recorded max error
+ *      bounds the window of the search (EXC_OPT_SEARCH = 2), whose exact
+ *      fallback keeps results identical to plain binary search. */
+enum { EXC_MODEL_PLA = 1, EXC_MODEL_MLP = 2 };
+typedef struct { uint32_t nmodels; float max_err; float mean_err; float train_ms; } exc_model_stats;
+typedef struct {
+    int32_t kind;       /* EXC_MODEL_PLA (257 knots) | EXC_MODEL_MLP (w1[32] b1[32] w2[32] b2) */
+    uint32_t n;         /* list length */
+    float err;          /* recorded max |prediction - position| */
+    float x0, xs;       /* input offset and scale */
+    uint32_t nparams;
+    float params[257];
+} exc_model_host;
+int exc_index_train_models(exc_ctx *ctx, exc_index *ix, int kind, uint32_t min_len, uint32_t epochs,
+                           uint32_t samples, float lr, uint64_t seed, exc_model_stats *stats);
+int exc_index_model_info(exc_ctx *ctx, exc_index *ix, uint32_t term, exc_model_host *out);
+/* One full-batch MLP gradient (the training kernel with zero epochs) on the
+ * sorted list docs[0..n): grad[97] of the mean squared position error at params
+ * over `samples` strided pairs; loss gets the mean squared error. */
+int exc_mlp_gradient(exc_ctx *ctx, const float *params, const uint32_t *docs, uint32_t n, uint32_t samples,
+                     float *grad, float *loss);
+
 /* ---- index builder (replaces the *_Generator and *_Compression programs,
  *      GC/LRC/LRC_Generator/LRC_Generator.cu:93-287, GC/LRC/LRC_Compression/main.cpp:86-238
  *      and their SegLRC / HS_SegLRC / ParaPFD counterparts).

@@ -82,6 +82,13 @@ def lib():
                                             _vp, _vp, ctypes.c_uint64]
         L.orc_intersect2.restype = ctypes.c_uint64
         L.orc_intersect2.argtypes = [_vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp]
+        L.orc_pla_fit.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp]
+        L.orc_pla_pred.restype = ctypes.c_float
+        L.orc_pla_pred.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]
+        L.orc_mlp_pred.restype = ctypes.c_float
+        L.orc_mlp_pred.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_model_max_err.restype = ctypes.c_float
+        L.orc_model_max_err.argtypes = [ctypes.c_int, _vp, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_uint32]
         L.orc_query_bytes.restype = ctypes.c_uint64
         L.orc_query_bytes.argtypes = [ctypes.POINTER(OrcIndex), _vp, _vp, ctypes.c_uint32, _vp]
         _lib = L
@@ -316,3 +323,53 @@ def intersect_all(lists):
     for x in lists[1:]:
         acc = np.intersect1d(acc, _u32(x), assume_unique=True)
     return acc.astype(np.uint32)
+
+
+# ------------------------------------------------------------ learned models
+
+def pla_fit(L):
+    """(x0, xs, knots[257]) of the PLA-256 position model of sorted list L."""
+    a = _u32(L)
+    x0 = np.zeros(1, np.float32)
+    xs = np.zeros(1, np.float32)
+    y = np.zeros(257, np.float32)
+    lib().orc_pla_fit(_p(a), len(a), x0.ctypes.data, xs.ctypes.data, y.ctypes.data)
+    return float(x0[0]), float(xs[0]), y
+
+
+def model_max_err(kind, params, x0, xs, L):
+    a = _u32(L)
+    prm = np.ascontiguousarray(params, np.float32)
+    return float(lib().orc_model_max_err(kind, prm.ctypes.data, x0, xs, _p(a), len(a)))
+
+
+def mlp_scale(L):
+    x0 = np.float32(L[0])
+    xs = np.float32(1.0) / (np.float32(L[-1]) - x0 + np.float32(1.0))
+    return float(x0), float(xs)
+
+
+def mlp_loss_grad(params, L, samples):
+    """Mean squared position error of the 1-32-1 ReLU MLP over `samples`
+    strided pairs of L and its analytic gradient, in float64."""
+    L = _u32(L)
+    n = len(L)
+    S = min(samples, n)
+    idx = (np.arange(S, dtype=np.uint64) * np.uint64(n)) // np.uint64(S)
+    x0, xs = mlp_scale(L)
+    x = ((L[idx].astype(np.float32) - np.float32(x0)) * np.float32(xs)).astype(np.float64)
+    t = idx.astype(np.float64) / n
+    w = np.asarray(params, np.float64)
+    w1, b1, w2, b2 = w[:32], w[32:64], w[64:96], w[96]
+    pre = np.outer(x, w1) + b1
+    h = np.maximum(pre, 0.0)
+    out = h @ w2 + b2
+    e = out - t
+    loss = float(np.mean(e * e))
+    g = 2.0 * e / S
+    gw2 = h.T @ g
+    gb2 = g.sum()
+    dh = np.outer(g, w2) * (pre > 0)
+    gw1 = dh.T @ x
+    gb1 = dh.sum(axis=0)
+    return loss, np.concatenate([gw1, gb1, gw2, [gb2]])

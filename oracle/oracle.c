@@ -752,3 +752,49 @@ uint64_t orc_query_bytes(const orc_index *ix, const uint32_t *qterms, const uint
     if (full_bytes) *full_bytes = full;
     return tot;
 }
+
+/* ----------------------------------------------------------- learned models */
+
+/* PLA-256: knots at t = p of t(v) = ((float)v - x0) * xs, xs = 256 / span;
+ * y[p] = number of elements with t < p (the empirical CDF at the knots). */
+void orc_pla_fit(const uint32_t *L, uint32_t n, float *x0, float *xs, float *y)
+{
+    *x0 = (float)L[0];
+    *xs = 256.0f / ((float)L[n - 1] - *x0 + 1.0f);
+    for (int p = 0; p < 257; ++p) {
+        uint32_t c = 0;
+        while (c < n && ((float)L[c] - *x0) * *xs < (float)p) ++c;
+        y[p] = (float)c;
+    }
+}
+
+float orc_pla_pred(const float *y, float x0, float xs, uint32_t v)
+{
+    float t = ((float)v - x0) * xs;
+    t = fminf(fmaxf(t, 0.0f), 256.0f);
+    int p = (int)t;
+    if (p > 255) p = 255;
+    const float f = t - (float)p;
+    return fmaf(f, y[p + 1] - y[p], y[p]);
+}
+
+float orc_mlp_pred(const float *w, float x0, float xs, uint32_t n, uint32_t v)
+{
+    const float x = ((float)v - x0) * xs;
+    float acc = w[96];
+    for (int j = 0; j < 32; ++j) {
+        const float h = fmaxf(fmaf(w[j], x, w[32 + j]), 0.0f);
+        acc = fmaf(w[64 + j], h, acc);
+    }
+    return acc * (float)n;
+}
+
+float orc_model_max_err(int kind, const float *params, float x0, float xs, const uint32_t *L, uint32_t n)
+{
+    float e = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float p = kind == 1 ? orc_pla_pred(params, x0, xs, L[i]) : orc_mlp_pred(params, x0, xs, n, L[i]);
+        e = fmaxf(e, fabsf(p - (float)i));
+    }
+    return e;
+}

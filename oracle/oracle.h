@@ -134,6 +134,14 @@ uint64_t orc_intersect2(const uint32_t *a, uint64_t na, const uint32_t *b, uint6
 uint64_t orc_query_bytes(const orc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
                          uint32_t nq, uint64_t *full_bytes);
 
+/* ---- learned position models (no reference counterpart; checker for the
+ * GPU-trained models).  Same fp32 operations, in the same order, as the
+ * device inference, so recorded max errors must agree exactly. */
+void orc_pla_fit(const uint32_t *L, uint32_t n, float *x0, float *xs, float *y /* 257 */);
+float orc_pla_pred(const float *y, float x0, float xs, uint32_t v);
+float orc_mlp_pred(const float *w /* 97 */, float x0, float xs, uint32_t n, uint32_t v);
+float orc_model_max_err(int kind, const float *params, float x0, float xs, const uint32_t *L, uint32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,103 @@
+"""Learned position models trained on the GPU: PLA knots, recorded max errors
+and MLP gradients against the CPU oracle, the gain of training, and exactness
+of the learned-window search (zero mismatches against plain binary search)."""
+import numpy as np
+import pytest
+
+import exclucene_amd as ea
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [500, 3000, 70000, 150000, 400000, 1_000_000, 2_500_000]
+
+
+def _lists(seed=31, universe=1 << 25):
+    rng = np.random.default_rng(seed)
+    out = [np.sort(rng.choice(universe, n, replace=False)).astype(np.uint32) for n in LENGTHS]
+    # a skewed list: dense head, sparse tail
+    a = np.arange(0, 100000, dtype=np.uint32) * 3
+    b = np.sort(rng.choice(np.arange(400000, universe), 100000, replace=False)).astype(np.uint32)
+    out.append(np.concatenate([a, b]))
+    return out
+
+
+@pytest.fixture(scope="module")
+def indexed(ctx):
+    lists = _lists()
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
+    dix = ctx.upload(built)
+    yield lists, built, dix
+    dix.close()
+
+
+def test_pla_knots_and_error_match_oracle(ctx, indexed):
+    lists, built, dix = indexed
+    st = ctx.train_models(dix, ea.MODEL_PLA, min_len=65536)
+    modelled = [t for t, L in enumerate(lists) if len(L) > 65536]
+    assert st["nmodels"] == len(modelled)
+    for t in modelled:
+        m = ctx.model_info(dix, t)
+        x0, xs, y = oracle.pla_fit(lists[t])
+        assert m["kind"] == ea.MODEL_PLA and m["n"] == len(lists[t])
+        assert m["x0"] == x0 and m["xs"] == xs
+        assert np.array_equal(m["params"], y)
+        assert m["err"] == oracle.model_max_err(1, y, x0, xs, lists[t])
+        # the recorded error really bounds every element
+        assert m["err"] < len(lists[t])
+
+
+def test_mlp_gradient_matches_oracle(ctx):
+    rng = np.random.default_rng(5)
+    L = np.sort(rng.choice(1 << 25, 200000, replace=False)).astype(np.uint32)
+    for trial in range(3):
+        w = np.concatenate([np.ones(32), -np.arange(32) / 32, rng.normal(0, 0.05, 32), [0.01]]).astype(np.float32)
+        w[64] += 1.0
+        w[:32] += rng.normal(0, 0.05, 32).astype(np.float32)
+        g, loss = ctx.mlp_gradient(w, L, 4096)
+        oloss, og = oracle.mlp_loss_grad(w.astype(np.float64), L, 4096)
+        scale = np.max(np.abs(og))
+        assert np.allclose(g, og, rtol=2e-3, atol=2e-3 * scale), (trial, np.max(np.abs(g - og)) / scale)
+        assert abs(loss - oloss) <= 1e-3 * oloss + 1e-9
+
+
+def test_mlp_training_gain_and_error(ctx, indexed):
+    lists, built, dix = indexed
+    st = ctx.train_models(dix, ea.MODEL_MLP, min_len=65536, epochs=400, samples=4096, lr=1e-3, seed=4)
+    assert st["nmodels"] > 0
+    gains = []
+    for t, L in enumerate(lists):
+        if len(L) <= 65536:
+            continue
+        m = ctx.model_info(dix, t)
+        assert m["kind"] == ea.MODEL_MLP
+        x0, xs = oracle.mlp_scale(L)
+        assert m["x0"] == x0 and m["xs"] == xs
+        assert m["err"] == oracle.model_max_err(2, m["params"], x0, xs, L)
+        # untrained network (the initial point, output = input) for comparison
+        ident = np.concatenate([np.ones(32), -np.arange(32) / 32, [1.0] + [0.0] * 31, [0.0]]).astype(np.float32)
+        e0 = oracle.model_max_err(2, ident, x0, xs, L)
+        gains.append(e0 / max(m["err"], 1.0))
+    # training must shrink the error bound of every list (>= 1.6x at this budget)
+    assert min(gains) > 1.6, gains
+
+
+@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP])
+def test_learned_search_is_exact(ctx, indexed, kind):
+    lists, built, dix = indexed
+    ctx.train_models(dix, kind, min_len=65536, epochs=200, samples=4096, lr=1e-3)
+    rng = np.random.default_rng(8)
+    qs = [[int(x) for x in rng.choice(len(lists), int(rng.integers(2, 5)), replace=False)] for _ in range(400)]
+    ref = oracle.build_index(lists, oracle.SEGLRC, seg=128)
+    ocum, ores = oracle.eval_queries(ref, qs)
+    try:
+        for search in (0, 1, 2):
+            for tpe in (0, 1):
+                ctx.set_option(ea.OPT_SEARCH, search)
+                ctx.set_option(ea.OPT_TPE, tpe)
+                cum, res = ctx.intersect(dix, qs)
+                assert np.array_equal(cum, ocum) and np.array_equal(res, ores), (kind, search, tpe)
+    finally:
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        ctx.set_option(ea.OPT_TPE, 0)
