@@ -119,7 +119,9 @@ def cpu_baseline(built, queries, budget_s=12.0):
                       hash_info=built.hash_info)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(threads, 16))
-    done, t_used = 0, 0.0
+    # whole-batch repetitions until the budget is spent (at least 5 when they fit);
+    # a batch too slow for one repetition is cut to a prefix
+    done, t_used, reps = 0, 0.0, []
     chunk = max(threads, 16)
     while done < len(queries) and t_used < budget_s:
         part = queries[done:done + chunk]
@@ -128,16 +130,65 @@ def cpu_baseline(built, queries, budget_s=12.0):
         t_used += time.perf_counter() - t0
         done += len(part)
         chunk = min(chunk * 2, 256)
-    qps = done / t_used if t_used > 0 else 0.0
+    if done == len(queries):
+        reps.append(t_used)
+        while sum(reps) < budget_s or len(reps) < 5:
+            t0 = time.perf_counter()
+            oracle.eval_queries(ix, queries, nthreads=threads)
+            reps.append(time.perf_counter() - t0)
+            if sum(reps) > 3 * budget_s:
+                break
+        t_med = float(np.median(reps))
+        qps = done / t_med
+        t_used = sum(reps)
+    else:
+        qps = done / t_used if t_used > 0 else 0.0
     # single-thread figure on a smaller prefix
     n1 = max(1, min(done, 64))
     t0 = time.perf_counter()
     oracle.eval_queries(ix, queries[:n1], nthreads=1)
     qps1 = n1 / (time.perf_counter() - t0)
     return {"value": qps, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": "first %d of the %d C2 queries, SegLRC-128, oracle block-skipping decode + "
-                      "intersect, %d threads, %.1f s; 1 thread: %.1f queries/s on %d queries"
-                      % (done, len(queries), threads, t_used, qps1, n1)}
+            "sample": "%d of the %d C2 queries, SegLRC-128, oracle block-skipping decode + intersect, "
+                      "%d threads, median of %d repetitions, %.1f s in all; 1 thread: %.1f queries/s on %d queries"
+                      % (done, len(queries), threads, max(1, len(reps)), t_used, qps1, n1)}
+
+
+def serving_leg(ea, ctx, dix, queries, batch_res, batch_cum, qps=20000.0, tiles=20, nslots=3, max_batch=256):
+    """Poisson arrivals at `qps` of the batch's queries repeated `tiles` times
+    through the continuous-batching server; p50/p99 latency, achieved rate and
+    whether every repetition returns the batch's results."""
+    rng = np.random.default_rng(55)
+    allq = queries * tiles
+    arrival = np.cumsum(rng.exponential(1.0 / qps, len(allq)))
+    cap = len(batch_res) * tiles
+    srv = ea.Server(ctx, dix, nslots=nslots, max_batch=max_batch, mem_budget=4 << 30)
+    try:
+        t0 = time.perf_counter()
+        lat, cnt, res = srv.replay(allq, arrival, cap)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    counts = np.diff(np.concatenate([[0], batch_cum])).astype(np.uint32)
+    same = bool(np.array_equal(cnt, np.tile(counts, tiles)) and np.array_equal(res, np.tile(batch_res, tiles)))
+    return {"offered_qps": qps, "queries": len(allq), "achieved_qps": len(allq) / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "slots": nslots, "max_batch": max_batch, "mem_budget": 4 << 30, "batches": st["batches"],
+            "overlapped_launches": st["overlapped_launches"], "results_match_batch": same}
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings of the probe kernel."""
+    import torch
+    n = mib << 18
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def main():
@@ -150,9 +201,20 @@ def main():
     ap.add_argument("--no-pfd", action="store_true", help="skip the ParaPFD leg")
     ap.add_argument("--nterms", type=int, default=100_000)
     ap.add_argument("--nq", type=int, default=1000)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: pick knobs per ratio class on a separate query sample (seed 23) first")
+    ap.add_argument("--search", type=int, default=1, help="0 binary, 1 fit window, 2 learned")
+    ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--coop-max", type=int, default=16)
     ap.add_argument("--traffic-kib", type=float, default=None,
                     help="rocprofv3 FETCH_SIZE+WRITE_SIZE KiB per probe launch (scaled), if measured")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run a plain device copy of this size (rocprofv3 counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (queries shard by "
               "replicating the index or by docID range; see DESIGN.md)", file=sys.stderr)
@@ -172,9 +234,22 @@ def main():
 
     ctx = ea.Context(0)
     dix = ctx.upload(seg)
-    ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_FIT)
-    ctx.set_option(ea.OPT_PERSISTENT, 1)
-    ctx.set_option(ea.OPT_TPE, 0)
+
+    tune = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune = ctx.autotune(dix, W.df_weighted_queries(df, 2000, 2, 5, 23), reps=3)
+        log("autotune %.1f s: %s" % (time.perf_counter() - t0, tune))
+
+    def headline_opts():
+        ctx.set_option(ea.OPT_SEARCH, args.search)
+        ctx.set_option(ea.OPT_PERSISTENT, tune["persistent"] if tune else args.persistent)
+        ctx.set_option(ea.OPT_TPE, 0)
+        ctx.set_option(ea.OPT_COOP_MAX, args.coop_max)
+        if tune:
+            ctx.set_tuning(tune)
+
+    headline_opts()
     dt, probe, invoke = time_batch(ctx, dix, queries, args.steps, args.warmup)
     nq = len(queries)
     value = nq * args.steps / dt
@@ -192,6 +267,7 @@ def main():
 
     extra = {}
     if not args.no_plain:
+        ctx.set_tuning(None)
         ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_BINARY)
         ctx.set_option(ea.OPT_PERSISTENT, 0)
         ctx.set_option(ea.OPT_TPE, 1)
@@ -210,16 +286,34 @@ def main():
         ctx.set_option(ea.OPT_TIMING, 1)
         log("plain single-query: %.0f q/s" % extra["plain_single_query_qps"])
         # batched end to end (plan + htod + kernels + dtoh)
-        ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_FIT)
-        ctx.set_option(ea.OPT_PERSISTENT, 1)
-        ctx.set_option(ea.OPT_TPE, 0)
+        headline_opts()
         t0 = time.perf_counter()
         ctx.intersect(dix, queries)
         extra["batched_end_to_end_qps"] = nq / (time.perf_counter() - t0)
 
+    # the same batch replayed from a hipGraph (persistent work queue)
+    ctx.set_option(ea.OPT_PERSISTENT, 1)
+    ctx.set_option(ea.OPT_GRAPH, 1)
+    gdt, _, ginv = time_batch(ctx, dix, queries, args.steps, args.warmup)
+    gcum, gres = ctx.dtohTransfer(ctx._bi)
+    extra["graph_persistent_qps"] = nq * args.steps / gdt
+    extra["graph_persistent_step_ms"] = float(np.median(ginv))
+    extra["graph_matches"] = bool(np.array_equal(gcum, cum) and np.array_equal(gres, res))
+    ctx.set_option(ea.OPT_GRAPH, 0)
+    headline_opts()
+    log("graph + persistent: %.0f q/s" % extra["graph_persistent_qps"])
+
+    if not args.no_serve:
+        extra["serving"] = serving_leg(ea, ctx, dix, queries, res, cum)
+        log("serving: %s" % extra["serving"])
+
     if not args.no_pfd:
         pfd = ea.build_index(docs, off, ea.PARAPFD, seg=128, frac=0.1)
         pix = ctx.upload(pfd)
+        ctx.set_tuning(None)
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        if args.autotune:
+            extra["parapfd_tune"] = ctx.autotune(pix, W.df_weighted_queries(df, 2000, 2, 5, 23), reps=2)
         pdt, pprobe, _ = time_batch(ctx, pix, queries, max(3, args.steps // 2), 1)
         extra["parapfd_batched_qps"] = nq * max(3, args.steps // 2) / pdt
         extra["parapfd_probe_ms"] = float(pprobe.mean())
@@ -250,7 +344,11 @@ def main():
         "config": {"workload": "C2: 1,000 queries of 2-5 terms, 100k-term Zipf(1.0) index over 2^25 docIDs, "
                                "SegLRC-128 resident in HBM",
                    "queries": nq, "postings": int(len(docs)), "codec": "SegLRC-128",
-                   "compressed_bytes": int(seg.compressed_bytes())},
+                   "compressed_bytes": int(seg.compressed_bytes()),
+                   "knobs": ({"autotuned": True, "coop_max": tune["coop_max"], "search": tune["search"],
+                              "persistent": tune["persistent"]} if tune else
+                             {"autotuned": False, "search": args.search, "persistent": args.persistent,
+                              "coop_max": args.coop_max})},
         "step_p50_ms": float(np.percentile(invoke, 50)),
         "step_p99_ms": float(np.percentile(invoke, 99)),
         "results": int(len(res)),

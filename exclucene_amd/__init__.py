@@ -30,6 +30,8 @@ EXPORTS = [
     "exc_intersect", "exc_plan", "exc_decompress",
     "exc_build_index", "exc_built_view", "exc_built_free", "exc_index_save_files",
     "exc_synth_lists", "exc_index_train_models", "exc_index_model_info", "exc_mlp_gradient",
+    "exc_server_create", "exc_server_submit", "exc_server_wait", "exc_server_replay", "exc_server_stats",
+    "exc_server_destroy", "exc_autotune", "exc_ctx_set_tuning",
 ]
 MODEL_PLA, MODEL_MLP = 1, 2
 
@@ -46,6 +48,26 @@ class IndexHost(ctypes.Structure):
                 ("seg_head", _vp), ("seg_median", _vp), ("lrc_info", _vp), ("seg_info", _vp),
                 ("seg_base", _vp), ("list_info", _vp), ("hash_offset", _vp), ("hash_info", _vp),
                 ("hash_words", _u64)]
+
+
+RATIO_CLASSES = 6
+
+
+class TuneTable(ctypes.Structure):
+    _fields_ = [("queries", _u32 * RATIO_CLASSES), ("coop_max", _u32 * RATIO_CLASSES),
+                ("search", _u32 * RATIO_CLASSES), ("ms", ctypes.c_float * RATIO_CLASSES),
+                ("ms_default", ctypes.c_float * RATIO_CLASSES), ("persistent", _u32),
+                ("batch_ms", ctypes.c_float * 2)]
+
+    def as_dict(self):
+        return {"queries": list(self.queries), "coop_max": list(self.coop_max), "search": list(self.search),
+                "ms": [float(x) for x in self.ms], "ms_default": [float(x) for x in self.ms_default],
+                "persistent": int(self.persistent), "batch_ms": [float(x) for x in self.batch_ms]}
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [("batches", _u64), ("queries", _u64), ("max_inflight_batches", _u64),
+                ("overlapped_launches", _u64)]
 
 
 class ModelStats(ctypes.Structure):
@@ -123,6 +145,15 @@ def lib():
                                              ctypes.POINTER(ModelStats)]
         L.exc_index_model_info.argtypes = [_vp, _vp, _u32, ctypes.POINTER(ModelHost)]
         L.exc_mlp_gradient.argtypes = [_vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float)]
+        L.exc_autotune.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(TuneTable)]
+        L.exc_ctx_set_tuning.argtypes = [_vp, ctypes.POINTER(TuneTable)]
+        L.exc_server_destroy.restype = None
+        L.exc_server_create.argtypes = [_vp, _vp, _u32, _u32, _u64, ctypes.POINTER(_vp)]
+        L.exc_server_submit.argtypes = [_vp, _vp, _u32, ctypes.POINTER(_u64)]
+        L.exc_server_wait.argtypes = [_vp, _u64, _vp, _u64, ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_double)]
+        L.exc_server_replay.argtypes = [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u64]
+        L.exc_server_stats.argtypes = [_vp, ctypes.POINTER(ServerInfo)]
+        L.exc_server_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -357,6 +388,28 @@ class Context:
                                    ctypes.byref(n)))
         return cum[:nq], res[:int(n.value)]
 
+    # -- autotuner
+    def autotune(self, index, queries, reps=3):
+        """Pick per-ratio-class knobs and the launch mode on this sample and
+        install them (exc_autotune); returns the table as a dict."""
+        qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
+        t = TuneTable()
+        _check(lib().exc_autotune(self._h, index.handle, _p(qterms), _p(qoff), len(qoff) - 1, reps,
+                                  ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, table):
+        """Install a tuning table (dict with coop_max, search lists and persistent), or None."""
+        if table is None:
+            _check(lib().exc_ctx_set_tuning(self._h, None))
+            return
+        t = TuneTable()
+        for k in range(RATIO_CLASSES):
+            t.coop_max[k] = int(table["coop_max"][k])
+            t.search[k] = int(table["search"][k])
+        t.persistent = int(table.get("persistent", 0))
+        _check(lib().exc_ctx_set_tuning(self._h, ctypes.byref(t)))
+
     # -- learned position models
     def train_models(self, index, kind=MODEL_PLA, min_len=65536, epochs=300, samples=4096, lr=3e-4, seed=4):
         st = ModelStats()
@@ -391,3 +444,60 @@ class Context:
         if self._h:
             lib().exc_ctx_destroy(self._h)
             self._h = None
+
+
+class Server:
+    """Continuous-batching request loop over a resident index (exc_server_*).
+
+    Queries are admitted into `nslots` batch slots, each on its own HIP stream,
+    while the device memory of the in-flight batches stays within `mem_budget`
+    bytes (0 = unlimited).  The slots take the search options of `ctx`."""
+
+    def __init__(self, ctx, index, nslots=2, max_batch=1024, mem_budget=0):
+        self.index = index
+        h = _vp()
+        _check(lib().exc_server_create(ctx.handle, index.handle, nslots, max_batch, mem_budget, ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, terms):
+        t = np.ascontiguousarray(terms, np.uint32)
+        tk = _u64()
+        _check(lib().exc_server_submit(self._h, _p(t), len(t), ctypes.byref(tk)))
+        return int(tk.value)
+
+    def wait(self, ticket, cap=None):
+        if cap is None:
+            cap = int(self.index.built.ucnt.max()) if self.index.built is not None else 1 << 26
+        out = np.zeros(max(cap, 1), np.uint32)
+        n, lat = _u64(), ctypes.c_double()
+        _check(lib().exc_server_wait(self._h, ticket, _p(out), cap, ctypes.byref(n), ctypes.byref(lat)))
+        return out[:int(n.value)], lat.value
+
+    def replay(self, queries, arrival_s, cap):
+        """Submit every query at its arrival time (seconds from now) and wait
+        for all; returns (latency_ms, counts, concatenated results)."""
+        qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
+        nq = len(qoff) - 1
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        assert len(arr) == nq
+        lat = np.zeros(max(nq, 1), np.float64)
+        cnt = np.zeros(max(nq, 1), np.uint32)
+        res = np.zeros(max(cap, 1), np.uint32)
+        _check(lib().exc_server_replay(self._h, _p(qterms), _p(qoff), _p(arr), nq, _p(lat), _p(cnt), _p(res), cap))
+        return lat[:nq], cnt[:nq], res[:int(cnt[:nq].sum())]
+
+    def stats(self):
+        st = ServerInfo()
+        _check(lib().exc_server_stats(self._h, ctypes.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in ServerInfo._fields_}
+
+    def close(self):
+        if self._h:
+            lib().exc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -81,11 +81,25 @@ struct QueryRec {
     uint32_t first_unit;  // first work unit of the query
     uint32_t nunits;      // work units (shortest-list chunks)
     uint64_t short_off;   // offset of the query's shortest list in the stage buffer
-    uint64_t pad;
+    uint32_t knobs;       // search knobs of this query, see make_knobs
+    uint32_t klass;       // list-length-ratio class (kRatioClasses)
 };
 static_assert(sizeof(QueryRec) == 32, "QueryRec is 32 bytes");
 
 enum { SEARCH_BINARY = 0, SEARCH_FIT = 1, SEARCH_LEARNED = 2 };
+
+// Per-query search knobs: bits 0-7 = longest run of a longer list's segments a
+// wave decodes cooperatively (0 = one lane per candidate, 64 = always
+// cooperative), bits 8-9 = search mode.
+constexpr uint32_t make_knobs(uint32_t coop, uint32_t search) { return (coop & 0xffu) | ((search & 3u) << 8); }
+__host__ __device__ __forceinline__ uint32_t knob_coop(uint32_t k) { return k & 0xffu; }
+__host__ __device__ __forceinline__ uint32_t knob_search(uint32_t k) { return (k >> 8) & 3u; }
+
+// List-length-ratio classes: class c < 4 holds queries whose second-shortest
+// list is 10^c .. 10^(c+1) times the shortest, class 4 the ratios >= 10^4,
+// class 5 single-term queries (nothing to probe).
+constexpr int kRatioClasses = 6;
+static_assert(kRatioClasses == EXC_RATIO_CLASSES, "ratio classes");
 
 struct ProbeArgs {
     DevIndex ix;
@@ -93,10 +107,7 @@ struct ProbeArgs {
     const uint32_t* qterms;
     const uint32_t* unit_q;
     uint32_t nunits;
-    int32_t search;
-    int32_t tpe;
     int32_t persistent;
-    int32_t coop_max;   // decode runs of up to this many segments cooperatively (tpe 0)
     uint32_t* stage;
     uint32_t* ucount;
     uint32_t* counter;

@@ -322,6 +322,28 @@ __device__ __forceinline__ void fit_window(const TermRec& R, uint32_t v, int64_t
     wb = phi <= 0.0f ? 1 : (int64_t)(phi / SEG) + 2;
 }
 
+// mlp_pred for vmin and vmax at once: lane j (mod 32) evaluates hidden unit j,
+// lanes 0-31 for vmin and 32-63 for vmax; the output sums then run in unit
+// order from the lanes' values, the same operations as mlp_pred, so the
+// result is bit-identical while each lane holds three parameters.
+__device__ __forceinline__ void mlp_pred_pair(const float* w, float x0, float xs, uint32_t n, uint32_t vmin,
+                                              uint32_t vmax, float& plo, float& phi)
+{
+    const uint32_t ln = lane_id(), j = ln & (kMlpH - 1);
+    const float x = ((float)(ln < 32 ? vmin : vmax) - x0) * xs;
+    const float h = fmaxf(fmaf(w[j], x, w[kMlpH + j]), 0.0f);
+    const float w2 = w[2 * kMlpH + j];
+    const int hb = __float_as_int(h), wb = __float_as_int(w2);
+    float a0 = w[3 * kMlpH], a1 = a0;
+    for (int k = 0; k < kMlpH; ++k) {
+        a0 = fmaf(__int_as_float(__builtin_amdgcn_readlane(wb, k)), __int_as_float(__builtin_amdgcn_readlane(hb, k)), a0);
+        a1 = fmaf(__int_as_float(__builtin_amdgcn_readlane(wb, 32 + k)),
+                  __int_as_float(__builtin_amdgcn_readlane(hb, 32 + k)), a1);
+    }
+    plo = a0 * (float)n;
+    phi = a1 * (float)n;
+}
+
 // Window [wa, wb] for "number of segment heads <= v" over [vmin, vmax] from
 // the list's learned model: prediction +- its recorded max error.
 template <int SEG>
@@ -335,8 +357,7 @@ __device__ __forceinline__ void model_window(const DevIndex& ix, uint32_t mid, u
         plo = pla_pred(prm, M.x0, M.xs, vmin);
         phi = pla_pred(prm, M.x0, M.xs, vmax);
     } else {
-        plo = mlp_pred(prm, M.x0, M.xs, M.n, vmin);
-        phi = mlp_pred(prm, M.x0, M.xs, M.n, vmax);
+        mlp_pred_pair(prm, M.x0, M.xs, M.n, vmin, vmax, plo, phi);
     }
     plo = fminf(fmaxf(plo - M.err - 2.0f, -1.0e9f), 4.0e9f);
     phi = fminf(fmaxf(phi + M.err + 2.0f, -1.0e9f), 4.0e9f);
@@ -351,8 +372,9 @@ struct LdsWave {
 };
 
 template <int CODEC, int SEG>
-__device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive)
+__device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
 {
+    const uint32_t search = knob_search(knobs);
     constexpr int V = kV;
     constexpr int NB = (int)(kBufWords / SEG);
     const uint32_t ln = lane_id();
@@ -365,9 +387,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     int64_t hb0 = 0;
     uint32_t H = kNone;
     bool small = false;
-    if (A.search != SEARCH_BINARY && nseg > 64) {
+    if (search != SEARCH_BINARY && nseg > 64) {
         int64_t wa, t0, t1, wb;
-        if (A.search == SEARCH_LEARNED && R.model != kNoModel) {
+        if (search == SEARCH_LEARNED && R.model != kNoModel) {
             model_window<SEG>(A.ix, R.model, vmin, vmax, wa, wb);
         } else {
             fit_window<CODEC, SEG>(R, vmin, wa, t0);
@@ -417,8 +439,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         found[i] = false;
     }
 
-    const uint32_t cmax = (A.tpe == 1 && CODEC != EXC_PARAPFD) ? 0u
-                          : (A.tpe == 64 || CODEC == EXC_PARAPFD) ? 64u : (uint32_t)A.coop_max;
+    const uint32_t cmax = knob_coop(knobs);
     if (r <= cmax) {
         // ---- the run of segments sf .. sf+r-1 decoded piece by piece into LDS;
         // one gather brings every segment record of the run, one load round
@@ -531,7 +552,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                     if (s[i + x] >= 0)
                         found[i + x] = seg_find(A.ix.words + R.woff + rec[x].off + 1,
                                                 seg_len(R, (uint32_t)s[i + x], SEG), rec[x], v[i + x],
-                                                A.search != SEARCH_BINARY);
+                                                search != SEARCH_BINARY);
             }
         } else {
             // LRC: binary search inside the segment
@@ -602,8 +623,9 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
 
 // HS_SegLRC probe: O(1) bucket, fit window, exact fallback
 // (HS_SegLRC_Intersection_kernel.cu:334-401 plus the fallback it lacks).
-__device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive)
+__device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
 {
+    const uint32_t search = knob_search(knobs);
     constexpr int V = kV;
     const uint32_t ln = lane_id();
     const uint32_t shift = (uint32_t)R.icpt, nb = R.nseg;
@@ -635,7 +657,7 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
         for (int x = 0; x < 2; ++x)
             if (ok[x] && len[x] > 0)
                 found[i + x] = seg_find(A.ix.words + R.woff + rec[x].off + 1, len[x], rec[x], v[i + x],
-                                        A.search != SEARCH_BINARY);
+                                        search != SEARCH_BINARY);
     }
     uint32_t cnt = 0, pos[V];
 #pragma unroll
@@ -679,9 +701,9 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     for (uint32_t j = 1; j < Q.tc && nlive; ++j) {
         const TermRec R = A.ix.terms[uni(A.qterms[Q.toff + j])];
         if constexpr (CODEC == EXC_HS_SEGLRC)
-            nlive = probe_hs(A, R, W, nlive);
+            nlive = probe_hs(A, R, W, nlive, Q.knobs);
         else
-            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive);
+            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, Q.knobs);
     }
     uint32_t* out = A.stage + Q.short_off + (uint64_t)k * kChunk;
     for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];

@@ -11,7 +11,14 @@
 // (index arrays + one 64-byte record per term); per-batch buffers grow
 // geometrically and are reused, so a steady stream of batches allocates nothing.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstdarg>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -80,7 +87,10 @@ struct exc_ctx {
     float probe_ms = 0, invoke_ms = 0;
     // graph of the current plan
     hipGraphExec_t gexec = nullptr;
-    int gtpe = -1;
+    // per-ratio-class knobs chosen by exc_autotune (used when tuned)
+    bool tuned = false;
+    exc_tune_table tune{};
+    uint64_t opt_gen = 0, plan_gen = 0;  // knob options changed since the plan was made
 };
 
 struct exc_index {
@@ -212,10 +222,12 @@ extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
     case EXC_OPT_SEARCH:
         if (v < 0 || v > 2) return set_error(EXC_ERR_ARG, "search mode must be 0, 1 or 2");
         c->search = (int)v;
+        ++c->opt_gen;
         break;
     case EXC_OPT_TPE:
         if (v != 0 && v != 1 && v != 64) return set_error(EXC_ERR_ARG, "tpe must be 0, 1 or 64");
         c->tpe = (int)v;
+        ++c->opt_gen;
         break;
     case EXC_OPT_PERSISTENT: c->persistent = v != 0; break;
     case EXC_OPT_GRAPH: c->graph = v != 0; break;
@@ -223,6 +235,7 @@ extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
     case EXC_OPT_COOP_MAX:
         if (v < 0 || v > 64) return set_error(EXC_ERR_ARG, "coop_max must be in [0, 64]");
         c->coop_max = (int)v;
+        ++c->opt_gen;
         break;
     default: return set_error(EXC_ERR_ARG, "unknown option %d", key);
     }
@@ -486,6 +499,27 @@ namespace {
 // Terms ordered by ascending list length with a stable insertion sort
 // (LRC_Intersection.cu:157-179), work units = shortest-list segments
 // (:182-213; one wave per segment here, one 256-thread block there).
+uint32_t ratio_class(const exc_index* ix, const uint32_t* sorted_terms, uint32_t tc)
+{
+    if (tc < 2) return kRatioClasses - 1;
+    const double n0 = std::max<uint32_t>(1, ix->n[sorted_terms[0]]), n1 = ix->n[sorted_terms[1]];
+    double r = n1 / n0;
+    uint32_t k = 0;
+    while (k < 4 && r >= 10.0) { r /= 10.0; ++k; }
+    return k;
+}
+
+// Knobs of a query of ratio class k: the tuned table, else the context options
+// (tpe 1 = one lane per candidate, tpe 64 = always cooperative, tpe 0 =
+// cooperative for runs up to coop_max segments; ParaPFD decodes cooperatively).
+uint32_t knobs_for(const exc_ctx* c, int codec, uint32_t k)
+{
+    if (c->tuned) return make_knobs(c->tune.coop_max[k], c->tune.search[k]);
+    uint32_t coop = c->tpe == 1 ? 0u : c->tpe == 64 ? 64u : (uint32_t)c->coop_max;
+    if (codec == EXC_PARAPFD) coop = 64;
+    return make_knobs(coop, (uint32_t)c->search);
+}
+
 int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq)
 {
     c->ix = ix;
@@ -517,6 +551,8 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         const uint32_t n0 = tc ? ix->n[tmp[0]] : 0;
         Q.nunits = (n0 + kChunk - 1) / kChunk;
         Q.short_off = c->short_sum;
+        Q.klass = ratio_class(ix, tmp.data(), tc);
+        Q.knobs = knobs_for(c, ix->codec, Q.klass);
         c->short_sum += n0;
         c->qterms.insert(c->qterms.end(), tmp.begin(), tmp.end());
         for (uint32_t k = 0; k < Q.nunits; ++k) c->unit_q.push_back(q);
@@ -524,6 +560,7 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         if (units > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 work units in one batch");
     }
     c->nunits = (uint32_t)units;
+    c->plan_gen = c->opt_gen;
     if (c->short_sum > 0xffffffffull)
         return set_error(EXC_ERR_CAPACITY, "summed shortest-list length %llu exceeds 2^32",
                          (unsigned long long)c->short_sum);
@@ -531,7 +568,7 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
     return EXC_OK;
 }
 
-int upload_plan(exc_ctx* c)
+int upload_plan(exc_ctx* c, bool sync = true)
 {
     if (c->state < 1) return set_error(EXC_ERR_STATE, "htodTransfer before CPUPreprocess");
     HIP_TRY(hipSetDevice(c->device));
@@ -552,16 +589,27 @@ int upload_plan(exc_ctx* c)
     if (c->nq) HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice, c->stream));
     if (!c->qterms.empty()) HIP_TRY(hipMemcpyAsync(c->d_qterms.p, c->qterms.data(), c->qterms.size() * 4, hipMemcpyHostToDevice, c->stream));
     if (c->nunits) HIP_TRY(hipMemcpyAsync(c->d_unit_q.p, c->unit_q.data(), (size_t)c->nunits * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (sync) HIP_TRY(hipStreamSynchronize(c->stream));
     c->state = 2;
     return EXC_OK;
 }
 
-// Threads per short-list element when autotuned: random access (1 lane per
-// candidate) for the random-access codecs, wave-cooperative decode for ParaPFD.
-int pick_tpe(exc_ctx* c) { return c->tpe; }
+// Options changed after planning: recompute every query's knobs and send the
+// query records again (the records live in device memory, so a captured graph
+// stays valid).
+int refresh_knobs(exc_ctx* c)
+{
+    if (c->plan_gen == c->opt_gen) return EXC_OK;
+    for (auto& Q : c->queries) Q.knobs = knobs_for(c, c->ix->codec, Q.klass);
+    if (c->nq)
+        HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice,
+                               c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->plan_gen = c->opt_gen;
+    return EXC_OK;
+}
 
-int enqueue_invoke(exc_ctx* c, int tpe, bool timed)
+int enqueue_invoke(exc_ctx* c, bool timed)
 {
     exc_index* ix = c->ix;
     ProbeArgs a;
@@ -571,10 +619,7 @@ int enqueue_invoke(exc_ctx* c, int tpe, bool timed)
     a.qterms = dptr<uint32_t>(c->d_qterms);
     a.unit_q = dptr<uint32_t>(c->d_unit_q);
     a.nunits = c->nunits;
-    a.search = c->search;
-    a.tpe = tpe;
     a.persistent = c->persistent;
-    a.coop_max = c->coop_max;
     a.stage = dptr<uint32_t>(c->d_stage);
     a.ucount = dptr<uint32_t>(c->d_ucount);
     a.counter = dptr<uint32_t>(c->d_counter);
@@ -610,28 +655,27 @@ int invoke(exc_ctx* c)
 {
     if (c->state < 2 || !c->ix) return set_error(EXC_ERR_STATE, "kernelInvoke before htodTransfer");
     HIP_TRY(hipSetDevice(c->device));
-    const int tpe = pick_tpe(c);
+    if (int rc = refresh_knobs(c)) return rc;
     const bool timed = c->timing && !c->graph;
     if (c->graph) {
-        if (!c->gexec || c->gtpe != tpe) {
+        if (!c->gexec) {
             drop_graph(c);
             hipGraph_t g;
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_invoke(c, tpe, false);
+            int rc = enqueue_invoke(c, false);
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc) return rc;
             if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
             e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
-            c->gtpe = tpe;
         }
         if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
         HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
         if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     } else {
         if (timed) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-        if (int rc = enqueue_invoke(c, tpe, timed)) return rc;
+        if (int rc = enqueue_invoke(c, timed)) return rc;
         if (timed) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1010,4 +1054,454 @@ extern "C" int exc_mlp_gradient(exc_ctx* c, const float* params, const uint32_t*
     std::memcpy(grad, g, kMlpParams * 4);
     if (loss) *loss = g[kMlpParams];
     return EXC_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+namespace {
+
+// Best probe time over `reps` runs of the planned batch (after one warm run).
+int time_probe(exc_ctx* c, uint32_t reps, float* best)
+{
+    int rc = upload_plan(c);
+    if (rc) return rc;
+    if ((rc = invoke(c))) return rc;
+    *best = 1e30f;
+    for (uint32_t r = 0; r < reps; ++r) {
+        if ((rc = invoke(c))) return rc;
+        *best = std::min(*best, c->probe_ms);
+    }
+    return EXC_OK;
+}
+
+}  // namespace
+
+extern "C" int exc_autotune(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq,
+                            uint32_t reps, exc_tune_table* out)
+{
+    if (!c || !ix || !qoff || (nq && !qterms) || !out) return set_error(EXC_ERR_ARG, "exc_autotune: bad argument");
+    reps = std::max<uint32_t>(1, std::min<uint32_t>(reps, 50));
+    // classify the sample
+    std::vector<std::vector<uint32_t>> cls(kRatioClasses);
+    std::vector<uint32_t> tmp;
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (qoff[q + 1] < qoff[q]) return set_error(EXC_ERR_ARG, "query %u: bad offsets", q);
+        tmp.assign(qterms + qoff[q], qterms + qoff[q + 1]);
+        for (uint32_t t : tmp)
+            if (t >= ix->nterms) return set_error(EXC_ERR_ARG, "query %u: term %u out of range", q, t);
+        std::stable_sort(tmp.begin(), tmp.end(), [&](uint32_t a, uint32_t b) { return ix->n[a] < ix->n[b]; });
+        cls[ratio_class(ix, tmp.data(), (uint32_t)tmp.size())].push_back(q);
+    }
+    const bool had_graph = c->graph, had_timing = c->timing;
+    c->graph = 0;
+    c->timing = 1;
+    drop_graph(c);
+    exc_tune_table T;
+    std::memset(&T, 0, sizeof T);
+    // candidates: cooperative-run limits per codec x search modes
+    std::vector<uint32_t> coops;
+    if (ix->codec == EXC_PARAPFD) coops = {16, 32, 64};
+    else if (ix->codec == EXC_HS_SEGLRC) coops = {0};
+    else coops = {0, 2, 4, 8, 16, 32, 64};
+    std::vector<uint32_t> searches = {SEARCH_BINARY, SEARCH_FIT};
+    if (!ix->models.empty()) searches.push_back(SEARCH_LEARNED);
+    int rc = EXC_OK;
+    for (int k = 0; k < kRatioClasses && !rc; ++k) {
+        c->tuned = false;
+        const uint32_t dk = knobs_for(c, ix->codec, (uint32_t)k);
+        T.coop_max[k] = knob_coop(dk);
+        T.search[k] = knob_search(dk);
+        T.queries[k] = (uint32_t)cls[k].size();
+        if (cls[k].empty() || k == kRatioClasses - 1) continue;  // single-term queries probe nothing
+        // sub-batch of the class, at most 4096 queries spread over the sample
+        std::vector<uint32_t> sq;
+        std::vector<uint64_t> so{0};
+        const size_t m = cls[k].size(), take = std::min<size_t>(m, 4096);
+        for (size_t i = 0; i < take; ++i) {
+            const uint32_t q = cls[k][i * m / take];
+            sq.insert(sq.end(), qterms + qoff[q], qterms + qoff[q + 1]);
+            so.push_back(sq.size());
+        }
+        std::vector<uint32_t> cands;
+        for (uint32_t s : searches)
+            for (uint32_t cp : coops) cands.push_back(make_knobs(cp, s));
+        if (std::find(cands.begin(), cands.end(), dk) == cands.end()) cands.push_back(dk);
+        float best = 1e30f;
+        for (uint32_t kn : cands) {
+            c->tuned = true;
+            for (int j = 0; j < kRatioClasses; ++j) {
+                c->tune.coop_max[j] = knob_coop(kn);
+                c->tune.search[j] = knob_search(kn);
+            }
+            float ms = 0;
+            if ((rc = plan_queries(c, ix, sq.data(), so.data(), (uint32_t)take)) ||
+                (rc = time_probe(c, reps, &ms)))
+                break;
+            if (kn == dk) T.ms_default[k] = ms;
+            if (ms < best) {
+                best = ms;
+                T.coop_max[k] = knob_coop(kn);
+                T.search[k] = knob_search(kn);
+            }
+        }
+        T.ms[k] = best;
+    }
+    // launch mode over the whole sample with the chosen knobs
+    const int had_persistent = c->persistent;
+    if (!rc) {
+        c->tuned = true;
+        c->tune = T;
+        for (int p = 0; p < 2 && !rc; ++p) {
+            c->persistent = p;
+            if (!(rc = plan_queries(c, ix, qterms, qoff, nq))) rc = time_probe(c, reps, &T.batch_ms[p]);
+        }
+        T.persistent = T.batch_ms[1] < T.batch_ms[0] ? 1u : 0u;
+    }
+    c->graph = had_graph;
+    c->timing = had_timing;
+    c->state = 0;  // the last plan was a tuning batch
+    if (rc) {
+        c->tuned = false;
+        c->persistent = had_persistent;
+        return rc;
+    }
+    c->persistent = (int)T.persistent;
+    c->tune = T;
+    c->tuned = true;
+    ++c->opt_gen;
+    *out = T;
+    return EXC_OK;
+}
+
+extern "C" int exc_ctx_set_tuning(exc_ctx* c, const exc_tune_table* t)
+{
+    if (!c) return set_error(EXC_ERR_ARG, "exc_ctx_set_tuning: null ctx");
+    if (t) {
+        for (int k = 0; k < kRatioClasses; ++k)
+            if (t->coop_max[k] > 64 || t->search[k] > 2)
+                return set_error(EXC_ERR_ARG, "tuning table class %d: coop_max <= 64 and search <= 2", k);
+        c->tune = *t;
+        c->persistent = t->persistent != 0;
+    }
+    c->tuned = t != nullptr;
+    ++c->opt_gen;
+    drop_graph(c);
+    return EXC_OK;
+}
+
+// ---------------------------------------------------------------- serving
+
+// Continuous-batching request loop.  The compressed index stays resident; the
+// loop owns `nslots` batch contexts (one HIP stream each).  Arriving queries
+// wait in a FIFO; whenever a slot is free the loop admits queries into a new
+// batch until the batch holds max_batch queries or the device memory that the
+// in-flight batches need (stage + result buffers, 8 B per shortest-list docID,
+// plus per-query and per-unit records) would exceed the budget.  Batches on
+// different slots overlap on the device; each finishing batch hands every
+// query its own results and latency.
+struct exc_server {
+    struct Req {
+        std::vector<uint32_t> terms;
+        std::chrono::steady_clock::time_point arrival;
+        std::vector<uint32_t> res;
+        double latency_ms = 0;
+        uint64_t need = 0;
+        bool done = false;
+        int err = 0;
+    };
+    struct Slot {
+        exc_ctx* ctx = nullptr;
+        std::vector<uint64_t> reqs;  // tickets in the batch
+        std::vector<uint32_t> qt;    // their terms, copied at admission
+        std::vector<uint64_t> qo{0};
+        uint64_t bytes = 0;
+        bool busy = false;
+        hipEvent_t ev = nullptr;
+        uint32_t* h_cum = nullptr;   // pinned
+        size_t h_cap = 0;
+    };
+    exc_index* ix = nullptr;
+    uint32_t max_batch = 1024;
+    uint64_t budget = 0, inflight = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_req, cv_done;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Req> reqs;  // open requests by ticket
+    uint64_t next_ticket = 0;
+    std::thread loop;
+    std::atomic<bool> stop{false};
+    // stats
+    uint64_t batches = 0, queries_done = 0, max_inflight_batches = 0, overlap_launches = 0;
+};
+
+namespace {
+
+uint64_t query_need(exc_index* ix, const std::vector<uint32_t>& terms)
+{
+    uint32_t mn = 0;
+    bool first = true;
+    for (uint32_t t : terms) {
+        if (t >= ix->nterms) return 0;
+        if (first || ix->n[t] < mn) mn = ix->n[t];
+        first = false;
+    }
+    return 8ull * mn + sizeof(QueryRec) + 4ull * terms.size() + 16ull * ((mn + kChunk - 1) / kChunk) + 64;
+}
+
+int slot_launch(exc_server* sv, exc_server::Slot& sl)
+{
+    exc_ctx* c = sl.ctx;
+    int rc;
+    if ((rc = plan_queries(c, sv->ix, sl.qt.data(), sl.qo.data(), (uint32_t)sl.reqs.size()))) return rc;
+    if ((rc = upload_plan(c, false))) return rc;
+    if ((rc = enqueue_invoke(c, false))) return rc;
+    const size_t nq = sl.reqs.size();
+    if (sl.h_cap < nq) {
+        if (sl.h_cum) (void)hipHostFree(sl.h_cum);
+        sl.h_cap = std::max<size_t>(nq, 2 * sl.h_cap);
+        HIP_TRY(hipHostMalloc((void**)&sl.h_cum, sl.h_cap * 4, hipHostMallocDefault));
+    }
+    HIP_TRY(hipMemcpyAsync(sl.h_cum, c->d_qcum.p, nq * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(sl.ev, c->stream));
+    c->state = 3;
+    return EXC_OK;
+}
+
+void slot_finish(exc_server* sv, exc_server::Slot& sl, int err)
+{
+    exc_ctx* c = sl.ctx;
+    const size_t nq = sl.reqs.size();
+    const uint64_t total = (!err && nq) ? sl.h_cum[nq - 1] : 0;
+    std::vector<uint32_t> all(total);
+    if (total && hipMemcpy(all.data(), c->d_result.p, total * 4, hipMemcpyDeviceToHost) != hipSuccess) err = EXC_ERR_DEVICE;
+    const auto now = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (size_t i = 0; i < nq; ++i) {
+        auto& r = sv->reqs.at(sl.reqs[i]);
+        r.err = err;
+        if (!err) {
+            const uint32_t b = i ? sl.h_cum[i - 1] : 0, e = sl.h_cum[i];
+            r.res.assign(all.begin() + b, all.begin() + e);
+        }
+        r.latency_ms = std::chrono::duration<double, std::milli>(now - r.arrival).count();
+        r.done = true;
+    }
+    sv->queries_done += nq;
+    sv->inflight -= sl.bytes;
+    sl.reqs.clear();
+    sl.qt.clear();
+    sl.qo.assign(1, 0);
+    sl.bytes = 0;
+    sl.busy = false;
+    sv->cv_done.notify_all();
+}
+
+void server_loop(exc_server* sv)
+{
+    (void)hipSetDevice(sv->slots[0].ctx->device);
+    for (;;) {
+        bool progressed = false;
+        // 1. retire finished batches
+        for (auto& sl : sv->slots) {
+            if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
+                slot_finish(sv, sl, 0);
+                progressed = true;
+            }
+        }
+        // 2. admit waiting queries into free slots
+        for (auto& sl : sv->slots) {
+            if (sl.busy) continue;
+            {
+                std::unique_lock<std::mutex> lk(sv->mu);
+                while (!sv->queue.empty() && sl.reqs.size() < sv->max_batch) {
+                    const uint64_t tk = sv->queue.front();
+                    const auto& r = sv->reqs.at(tk);
+                    if (sv->budget && sv->inflight + sl.bytes + r.need > sv->budget && (sl.bytes || sv->inflight)) break;
+                    sv->queue.pop_front();
+                    sl.reqs.push_back(tk);
+                    sl.qt.insert(sl.qt.end(), r.terms.begin(), r.terms.end());
+                    sl.qo.push_back(sl.qt.size());
+                    sl.bytes += r.need;
+                }
+                if (sl.reqs.empty()) break;
+                sv->inflight += sl.bytes;
+            }
+            uint32_t busy_now = 0;
+            for (auto& o : sv->slots) busy_now += o.busy;
+            if (busy_now) ++sv->overlap_launches;
+            sv->max_inflight_batches = std::max<uint64_t>(sv->max_inflight_batches, busy_now + 1);
+            const int rc = slot_launch(sv, sl);
+            ++sv->batches;
+            sl.busy = true;
+            if (rc) slot_finish(sv, sl, rc);
+            progressed = true;
+        }
+        if (sv->stop.load()) {
+            bool any = false;
+            for (auto& sl : sv->slots) any = any || sl.busy;
+            std::lock_guard<std::mutex> g(sv->mu);
+            if (!any && sv->queue.empty()) break;
+        }
+        if (!progressed) {
+            bool any = false;
+            for (auto& sl : sv->slots) any = any || sl.busy;
+            std::unique_lock<std::mutex> lk(sv->mu);
+            if (!any && sv->queue.empty() && !sv->stop.load())
+                sv->cv_req.wait_for(lk, std::chrono::milliseconds(5));
+            else if (any)
+                std::this_thread::yield();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int exc_server_create(exc_ctx* c, exc_index* ix, uint32_t nslots, uint32_t max_batch, uint64_t budget,
+                                 exc_server** out)
+{
+    if (!c || !ix || !out || nslots == 0 || nslots > 16 || max_batch == 0)
+        return set_error(EXC_ERR_ARG, "exc_server_create: bad argument");
+    auto* sv = new exc_server();
+    sv->ix = ix;
+    sv->max_batch = max_batch;
+    sv->budget = budget;
+    sv->slots.resize(nslots);
+    for (auto& sl : sv->slots) {
+        int rc = exc_ctx_create(c->device, 0, &sl.ctx);
+        if (rc) {
+            for (auto& o : sv->slots)
+                if (o.ctx) exc_ctx_destroy(o.ctx);
+            delete sv;
+            return rc;
+        }
+        sl.ctx->search = c->search;
+        sl.ctx->tpe = c->tpe;
+        sl.ctx->persistent = c->persistent;
+        sl.ctx->coop_max = c->coop_max;
+        sl.ctx->tuned = c->tuned;
+        sl.ctx->tune = c->tune;
+        sl.ctx->timing = 0;
+        (void)hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    }
+    sv->loop = std::thread(server_loop, sv);
+    *out = sv;
+    return EXC_OK;
+}
+
+extern "C" int exc_server_submit(exc_server* sv, const uint32_t* terms, uint32_t nterms, uint64_t* ticket)
+{
+    if (!sv || (nterms && !terms) || !ticket) return set_error(EXC_ERR_ARG, "exc_server_submit: bad argument");
+    exc_server::Req r;
+    r.terms.assign(terms, terms + nterms);
+    for (uint32_t t : r.terms)
+        if (t >= sv->ix->nterms) return set_error(EXC_ERR_ARG, "query term %u out of range", t);
+    r.need = query_need(sv->ix, r.terms);
+    if (sv->budget && r.need > sv->budget)
+        return set_error(EXC_ERR_BUDGET, "query needs %llu bytes, more than the whole budget",
+                         (unsigned long long)r.need);
+    r.arrival = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        *ticket = sv->next_ticket++;
+        sv->reqs.emplace(*ticket, std::move(r));
+        sv->queue.push_back(*ticket);
+    }
+    sv->cv_req.notify_one();
+    return EXC_OK;
+}
+
+extern "C" int exc_server_wait(exc_server* sv, uint64_t ticket, uint32_t* results, uint64_t cap, uint64_t* n,
+                               double* latency_ms)
+{
+    if (!sv) return set_error(EXC_ERR_ARG, "exc_server_wait: null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return set_error(EXC_ERR_ARG, "unknown or already collected ticket");
+    sv->cv_done.wait(lk, [&] { return it->second.done; });
+    auto& r = it->second;
+    if (n) *n = r.res.size();
+    if (latency_ms) *latency_ms = r.latency_ms;
+    if (r.err) {
+        const int e = r.err;
+        sv->reqs.erase(it);
+        return set_error(e, "query failed on the device");
+    }
+    // a too-small buffer keeps the request so the caller can retry with *n
+    if (r.res.size() > cap) return set_error(EXC_ERR_CAPACITY, "result buffer too small");
+    if (results && !r.res.empty()) std::memcpy(results, r.res.data(), r.res.size() * 4);
+    sv->reqs.erase(it);
+    return EXC_OK;
+}
+
+extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const uint64_t* qoff, const double* arrival_s,
+                                 uint32_t nq, double* lat_ms, uint32_t* counts, uint32_t* results, uint64_t res_cap)
+{
+    if (!sv || !qoff || !arrival_s) return set_error(EXC_ERR_ARG, "exc_server_replay: bad argument");
+    for (uint64_t k = 0; k < qoff[nq]; ++k)
+        if (qterms[k] >= sv->ix->nterms) return set_error(EXC_ERR_ARG, "query term %u out of range", qterms[k]);
+    std::vector<uint64_t> tickets(nq);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t q = 0; q < nq; ++q) {
+        const auto at = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                 std::chrono::duration<double>(arrival_s[q]));
+        while (std::chrono::steady_clock::now() < at) std::this_thread::yield();
+        exc_server::Req r;
+        r.terms.assign(qterms + qoff[q], qterms + qoff[q + 1]);
+        r.need = query_need(sv->ix, r.terms);
+        r.arrival = at;  // latency counts from the scheduled arrival
+        {
+            std::lock_guard<std::mutex> g(sv->mu);
+            tickets[q] = sv->next_ticket++;
+            sv->reqs.emplace(tickets[q], std::move(r));
+            sv->queue.push_back(tickets[q]);
+        }
+        sv->cv_req.notify_one();
+    }
+    uint64_t pos = 0;
+    int rc = EXC_OK;
+    for (uint32_t q = 0; q < nq; ++q) {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(tickets[q]);
+        sv->cv_done.wait(lk, [&] { return it->second.done; });
+        auto& r = it->second;
+        if (r.err && !rc) rc = set_error(r.err, "query %u failed on the device", q);
+        if (lat_ms) lat_ms[q] = r.latency_ms;
+        if (counts) counts[q] = (uint32_t)r.res.size();
+        if (results) {
+            if (pos + r.res.size() > res_cap) {
+                if (!rc) rc = set_error(EXC_ERR_CAPACITY, "replay result buffer too small");
+            } else if (!r.res.empty()) {
+                std::memcpy(results + pos, r.res.data(), r.res.size() * 4);
+            }
+        }
+        pos += r.res.size();
+        sv->reqs.erase(it);
+    }
+    return rc;
+}
+
+extern "C" int exc_server_stats(exc_server* sv, exc_server_info* st)
+{
+    if (!sv || !st) return set_error(EXC_ERR_ARG, "exc_server_stats: bad argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    st->batches = sv->batches;
+    st->queries = sv->queries_done;
+    st->max_inflight_batches = sv->max_inflight_batches;
+    st->overlapped_launches = sv->overlap_launches;
+    return EXC_OK;
+}
+
+extern "C" void exc_server_destroy(exc_server* sv)
+{
+    if (!sv) return;
+    sv->stop.store(true);
+    sv->cv_req.notify_all();
+    if (sv->loop.joinable()) sv->loop.join();
+    for (auto& sl : sv->slots) {
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.h_cum) (void)hipHostFree(sl.h_cum);
+        exc_ctx_destroy(sl.ctx);
+    }
+    delete sv;
 }

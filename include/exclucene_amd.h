@@ -144,6 +144,32 @@ int exc_intersect(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uin
 int exc_plan(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
              uint32_t nq, exc_batchInfo *bi);
 
+/* ---- autotuner.  No reference counterpart: the reference hard-codes one
+ *      launch shape per program (256 threads per shortest-list segment,
+ *      GC/LRC/LRC_Intersection/LRC_Intersection.cu:256-283).
+ * Queries fall into list-length-ratio classes by |second-shortest| /
+ * |shortest|: class c < 4 for ratios in [10^c, 10^(c+1)), class 4 for ratios
+ * >= 10^4, class 5 for single-term queries.  For every class present in the
+ * sample, exc_autotune times the class's queries under each candidate of
+ * (threads per short-list element as the cooperative-run limit, search mode)
+ * and keeps the fastest; then it times the whole sample with the persistent
+ * work queue on and off.  The result is installed in ctx (later plans use it,
+ * EXC_OPT_PERSISTENT is set) and returned. */
+#define EXC_RATIO_CLASSES 6
+typedef struct {
+    uint32_t queries[EXC_RATIO_CLASSES];  /* sample queries of the class                    */
+    uint32_t coop_max[EXC_RATIO_CLASSES]; /* 0 = one lane per element ... 64 = wave-cooperative */
+    uint32_t search[EXC_RATIO_CLASSES];   /* EXC_OPT_SEARCH value                            */
+    float ms[EXC_RATIO_CLASSES];          /* class probe time with the choice               */
+    float ms_default[EXC_RATIO_CLASSES];  /* ... with the context's options before tuning   */
+    uint32_t persistent;                  /* chosen launch mode                              */
+    float batch_ms[2];                    /* whole-sample probe time, non-persistent / persistent */
+} exc_tune_table;
+int exc_autotune(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff, uint32_t nq,
+                 uint32_t reps, exc_tune_table *out);
+/* Install a table (NULL: back to the plain options). */
+int exc_ctx_set_tuning(exc_ctx *ctx, const exc_tune_table *table);
+
 /* ---- decompression programs (GC/<codec>/<codec>_Decompression):
  * decode whole lists into out (concatenated in the given order). */
 int exc_decompress(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, uint32_t nterms,
@@ -175,6 +201,28 @@ int exc_index_model_info(exc_ctx *ctx, exc_index *ix, uint32_t term, exc_model_h
  * over `samples` strided pairs; loss gets the mean squared error. */
 int exc_mlp_gradient(exc_ctx *ctx, const float *params, const uint32_t *docs, uint32_t n, uint32_t samples,
                      float *grad, float *loss);
+
+/* ---- serving: continuous batching over a resident index.  No reference
+ *      counterpart (the reference batches a static query file by a
+ *      shortest-list threshold, GC/LRC/LRC_Intersection/LRC_Intersection.cu:449-540).
+ * Queries arrive at any time; a request loop admits them into free batch slots
+ * (one HIP stream each) while the slots' device memory stays within
+ * mem_budget, so batches overlap on the device; every query gets its own
+ * results and latency.  nslots <= 16. */
+typedef struct exc_server exc_server;
+typedef struct { uint64_t batches, queries, max_inflight_batches, overlapped_launches; } exc_server_info;
+int exc_server_create(exc_ctx *ctx, exc_index *ix, uint32_t nslots, uint32_t max_batch, uint64_t mem_budget,
+                      exc_server **out);
+int exc_server_submit(exc_server *sv, const uint32_t *terms, uint32_t nterms, uint64_t *ticket);
+int exc_server_wait(exc_server *sv, uint64_t ticket, uint32_t *results, uint64_t cap, uint64_t *n,
+                    double *latency_ms);
+/* Submit queries at arrival_s[q] seconds from now (the caller's thread paces
+ * them) and wait for all: per-query latency (ms from the scheduled arrival),
+ * result counts and concatenated results. */
+int exc_server_replay(exc_server *sv, const uint32_t *qterms, const uint64_t *qoff, const double *arrival_s,
+                      uint32_t nq, double *latency_ms, uint32_t *counts, uint32_t *results, uint64_t res_cap);
+int exc_server_stats(exc_server *sv, exc_server_info *st);
+void exc_server_destroy(exc_server *sv);
 
 /* ---- index builder (replaces the *_Generator and *_Compression programs,
  *      GC/LRC/LRC_Generator/LRC_Generator.cu:93-287, GC/LRC/LRC_Compression/main.cpp:86-238
