@@ -178,6 +178,28 @@ def serving_leg(ea, ctx, dix, queries, batch_res, batch_cum, qps=20000.0, tiles=
             "overlapped_launches": st["overlapped_launches"], "results_match_batch": same}
 
 
+def decode_leg(ea, ctx, dix, built, docs, reps=5):
+    """Decode every list of the index into HBM (k_decompress); algorithmic
+    bytes = compressed words + per-block side data read + 4 B per docID
+    written.  The output is compared with the generated lists."""
+    terms = np.arange(len(built.ind1), dtype=np.uint32)
+    total = int(built.ind1["m_urlcount"].sum())
+    out = ctx.device_alloc(total * 4)
+    ms = []
+    for _ in range(reps + 1):
+        ctx.decompress_device(dix, terms, out, total)
+        ms.append(ctx.last_times()[0])
+    ms = np.array(ms[1:])
+    nblocks = int(sum((int(n) + built.seg - 1) // built.seg for n in built.ind1["m_urlcount"]))
+    alg = len(built.ind2) * 4 + nblocks * META[built.codec] + total * 4
+    same = bool(np.array_equal(ctx.from_device(out, total), docs))
+    ctx.device_free(out)
+    return {"kernel": "k_decompress<SegLRC,128>", "docids": total, "kernel_ms": float(ms.mean()),
+            "algorithmic_bytes": int(alg), "achieved_GBps": alg / (ms.mean() * 1e-3) / 1e9,
+            "frac_of_peak": alg / (ms.mean() * 1e-3) / 1e9 / HBM_PEAK_GBS, "docids_per_s": total / (ms.mean() * 1e-3),
+            "matches_lists": same}
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings of the probe kernel."""
@@ -202,13 +224,18 @@ def main():
     ap.add_argument("--nterms", type=int, default=100_000)
     ap.add_argument("--nq", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick knobs per ratio class on a separate query sample (seed 23) first")
+    ap.add_argument("--tune-table", default=None,
+                    help='install this table instead of tuning: {"coop_max": [6], "search": [6], "persistent": p}')
     ap.add_argument("--search", type=int, default=1, help="0 binary, 1 fit window, 2 learned")
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--coop-max", type=int, default=16)
-    ap.add_argument("--traffic-kib", type=float, default=None,
-                    help="rocprofv3 FETCH_SIZE+WRITE_SIZE KiB per probe launch (scaled), if measured")
+    # k_probe<SegLRC,128> HBM traffic per launch on this workload with the tuned knobs: rocprofv3 FETCH_SIZE
+    # (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
+    ap.add_argument("--traffic-kib", type=float, default=1282431.0,
+                    help="scaled rocprofv3 FETCH_SIZE+WRITE_SIZE KiB per probe launch (0: report null)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -236,7 +263,9 @@ def main():
     dix = ctx.upload(seg)
 
     tune = None
-    if args.autotune:
+    if args.tune_table:
+        tune = json.loads(args.tune_table)
+    elif args.autotune:
         t0 = time.perf_counter()
         tune = ctx.autotune(dix, W.df_weighted_queries(df, 2000, 2, 5, 23), reps=3)
         log("autotune %.1f s: %s" % (time.perf_counter() - t0, tune))
@@ -303,6 +332,10 @@ def main():
     headline_opts()
     log("graph + persistent: %.0f q/s" % extra["graph_persistent_qps"])
 
+    if not args.no_decode:
+        extra["decode"] = decode_leg(ea, ctx, dix, seg, docs)
+        log("decode: %s" % extra["decode"])
+
     if not args.no_serve:
         extra["serving"] = serving_leg(ea, ctx, dix, queries, res, cum)
         log("serving: %s" % extra["serving"])
@@ -312,7 +345,7 @@ def main():
         pix = ctx.upload(pfd)
         ctx.set_tuning(None)
         ctx.set_option(ea.OPT_GRAPH, 0)
-        if args.autotune:
+        if args.autotune and not args.tune_table:
             extra["parapfd_tune"] = ctx.autotune(pix, W.df_weighted_queries(df, 2000, 2, 5, 23), reps=2)
         pdt, pprobe, _ = time_batch(ctx, pix, queries, max(3, args.steps // 2), 1)
         extra["parapfd_batched_qps"] = nq * max(3, args.steps // 2) / pdt
@@ -355,7 +388,9 @@ def main():
         "checksum": [int(x) for x in cs],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if args.traffic_kib is None else args.traffic_kib * 1024.0,
+                     "traffic": args.traffic_kib * 1024.0 if args.traffic_kib else None,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE then WRITE_SIZE on this workload, scaled by a "
+                                       "1 GiB copy kernel's readings (profiles/k_probe_seglrc128_c2.md)",
                      "kernel": "k_probe<SegLRC,128>", "probe_ms": float(probe.mean()),
                      "algorithmic_bytes": int(alg), "decode_everything_bytes": int(full)},
         "cpu_baseline": cpu,

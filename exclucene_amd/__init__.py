@@ -31,7 +31,8 @@ EXPORTS = [
     "exc_build_index", "exc_built_view", "exc_built_free", "exc_index_save_files",
     "exc_synth_lists", "exc_index_train_models", "exc_index_model_info", "exc_mlp_gradient",
     "exc_server_create", "exc_server_submit", "exc_server_wait", "exc_server_replay", "exc_server_stats",
-    "exc_server_destroy", "exc_autotune", "exc_ctx_set_tuning",
+    "exc_server_destroy", "exc_autotune", "exc_ctx_set_tuning", "exc_decompress_device",
+    "exc_device_alloc", "exc_device_free", "exc_memcpy",
 ]
 MODEL_PLA, MODEL_MLP = 1, 2
 
@@ -145,6 +146,10 @@ def lib():
                                              ctypes.POINTER(ModelStats)]
         L.exc_index_model_info.argtypes = [_vp, _vp, _u32, ctypes.POINTER(ModelHost)]
         L.exc_mlp_gradient.argtypes = [_vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float)]
+        L.exc_device_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
+        L.exc_device_free.argtypes = [_vp, _vp]
+        L.exc_memcpy.argtypes = [_vp, _vp, _vp, _u64, ctypes.c_int]
+        L.exc_decompress_device.argtypes = [_vp, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]
         L.exc_autotune.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(TuneTable)]
         L.exc_ctx_set_tuning.argtypes = [_vp, ctypes.POINTER(TuneTable)]
         L.exc_server_destroy.restype = None
@@ -439,6 +444,33 @@ class Context:
         n = _u64()
         _check(lib().exc_decompress(self._h, index.handle, _p(terms), len(terms), _p(out), cap, ctypes.byref(n)))
         return out[:int(n.value)]
+
+    # -- device buffers owned by the library
+    def device_alloc(self, nbytes):
+        p = _vp()
+        _check(lib().exc_device_alloc(self._h, nbytes, ctypes.byref(p)))
+        return int(p.value)
+
+    def device_free(self, d_ptr):
+        _check(lib().exc_device_free(self._h, ctypes.c_void_p(d_ptr)))
+
+    def to_device(self, d_ptr, arr):
+        a = np.ascontiguousarray(arr)
+        _check(lib().exc_memcpy(self._h, ctypes.c_void_p(d_ptr), _p(a), a.nbytes, 1))
+
+    def from_device(self, d_ptr, count, dtype=np.uint32):
+        out = np.empty(count, dtype)
+        _check(lib().exc_memcpy(self._h, _p(out), ctypes.c_void_p(d_ptr), out.nbytes, 0))
+        return out
+
+    def decompress_device(self, index, terms, d_out, cap):
+        """Decode whole lists into device memory at address d_out (capacity
+        `cap` docIDs); returns the docIDs written."""
+        terms = np.ascontiguousarray(terms, np.uint32)
+        n = _u64()
+        _check(lib().exc_decompress_device(self._h, index.handle, _p(terms), len(terms), ctypes.c_void_p(d_out), cap,
+                                           ctypes.byref(n)))
+        return int(n.value)
 
     def close(self):
         if self._h:

@@ -91,6 +91,8 @@ struct exc_ctx {
     bool tuned = false;
     exc_tune_table tune{};
     uint64_t opt_gen = 0, plan_gen = 0;  // knob options changed since the plan was made
+    // device buffers handed to the caller (exc_device_alloc)
+    std::unordered_map<void*, size_t> user_bufs;
 };
 
 struct exc_index {
@@ -209,10 +211,48 @@ extern "C" void exc_ctx_destroy(exc_ctx* c)
     for (DevBuf* b : {&c->d_queries, &c->d_qterms, &c->d_unit_q, &c->d_stage, &c->d_ucount, &c->d_uoff,
                       &c->d_bsum, &c->d_result, &c->d_qcum, &c->d_counter})
         dev_free(c, *b);
+    for (auto& kv : c->user_bufs) (void)hipFree(kv.first);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+extern "C" int exc_device_alloc(exc_ctx* c, uint64_t bytes, void** d_ptr)
+{
+    if (!c || !d_ptr || !bytes) return set_error(EXC_ERR_ARG, "exc_device_alloc: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->budget && c->used + bytes > c->budget)
+        return set_error(EXC_ERR_BUDGET, "device-memory budget of %llu bytes exceeded", (unsigned long long)c->budget);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, bytes));
+    c->user_bufs[p] = bytes;
+    c->used += bytes;
+    *d_ptr = p;
+    return EXC_OK;
+}
+
+extern "C" int exc_device_free(exc_ctx* c, void* d_ptr)
+{
+    if (!c) return set_error(EXC_ERR_ARG, "exc_device_free: null ctx");
+    auto it = c->user_bufs.find(d_ptr);
+    if (it == c->user_bufs.end()) return set_error(EXC_ERR_ARG, "exc_device_free: not a buffer of this context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_ptr);
+    c->used -= it->second;
+    c->user_bufs.erase(it);
+    return EXC_OK;
+}
+
+extern "C" int exc_memcpy(exc_ctx* c, void* dst, const void* src, uint64_t bytes, int to_device)
+{
+    if (!c || (bytes && (!dst || !src))) return set_error(EXC_ERR_ARG, "exc_memcpy: bad argument");
+    if (!bytes) return EXC_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+    return EXC_OK;
 }
 
 extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
@@ -223,11 +263,13 @@ extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
         if (v < 0 || v > 2) return set_error(EXC_ERR_ARG, "search mode must be 0, 1 or 2");
         c->search = (int)v;
         ++c->opt_gen;
+        c->tuned = false;  // an explicit knob replaces the tuned table
         break;
     case EXC_OPT_TPE:
         if (v != 0 && v != 1 && v != 64) return set_error(EXC_ERR_ARG, "tpe must be 0, 1 or 64");
         c->tpe = (int)v;
         ++c->opt_gen;
+        c->tuned = false;  // an explicit knob replaces the tuned table
         break;
     case EXC_OPT_PERSISTENT: c->persistent = v != 0; break;
     case EXC_OPT_GRAPH: c->graph = v != 0; break;
@@ -236,6 +278,7 @@ extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
         if (v < 0 || v > 64) return set_error(EXC_ERR_ARG, "coop_max must be in [0, 64]");
         c->coop_max = (int)v;
         ++c->opt_gen;
+        c->tuned = false;  // an explicit knob replaces the tuned table
         break;
     default: return set_error(EXC_ERR_ARG, "unknown option %d", key);
     }
@@ -816,8 +859,11 @@ namespace {
 
 // Decode whole lists of `terms` into dec (device), concatenated; off[i] =
 // offset of terms[i].  Temporary buffers are freed before returning.
+// Decode whole lists into device memory: into `dst` when given (capacity
+// checked by the caller), else into `dec`, allocated here.  With timing on,
+// the kernel's duration lands in probe_ms.
 int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& terms, DevBuf& dec,
-                     std::vector<uint64_t>& off)
+                     std::vector<uint64_t>& off, uint32_t* dst = nullptr)
 {
     const uint32_t nt = (uint32_t)terms.size();
     off.assign(nt + 1, 0);
@@ -827,10 +873,12 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
         off[i + 1] = off[i] + n;
         for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; ++k) { ut.push_back(i); uk.push_back(k); }
     }
+    c->probe_ms = 0;
     if (!off[nt]) return EXC_OK;
+    if (ut.size() > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 decode units");
     DevBuf dt, doff, dut, duk;
     int rc;
-    if ((rc = dev_alloc(c, dec, off[nt] * 4)) || (rc = dev_alloc(c, dt, nt * 4)) ||
+    if ((!dst && (rc = dev_alloc(c, dec, off[nt] * 4))) || (rc = dev_alloc(c, dt, nt * 4)) ||
         (rc = dev_alloc(c, doff, (nt + 1) * 8)) || (rc = dev_alloc(c, dut, ut.size() * 4)) ||
         (rc = dev_alloc(c, duk, uk.size() * 4))) {
         for (DevBuf* b : {&dt, &doff, &dut, &duk}) dev_free(c, *b);
@@ -840,10 +888,13 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
     if (e == hipSuccess) e = hipMemcpy(doff.p, off.data(), (nt + 1) * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dut.p, ut.data(), ut.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(duk.p, uk.data(), uk.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && c->timing) e = hipEventRecord(c->ev[1], c->stream);
     if (e == hipSuccess)
         e = launch_decompress(ix->dix, dptr<uint32_t>(dt), dptr<uint64_t>(doff), dptr<uint32_t>(dut),
-                              dptr<uint32_t>(duk), (uint32_t)ut.size(), dptr<uint32_t>(dec), c->stream);
+                              dptr<uint32_t>(duk), (uint32_t)ut.size(), dst ? dst : dptr<uint32_t>(dec), c->stream);
+    if (e == hipSuccess && c->timing) e = hipEventRecord(c->ev[2], c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && c->timing) (void)hipEventElapsedTime(&c->probe_ms, c->ev[1], c->ev[2]);
     for (DevBuf* b : {&dt, &doff, &dut, &duk}) dev_free(c, *b);
     if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "decode: %s", hipGetErrorString(e));
     return EXC_OK;
@@ -851,43 +902,53 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
 
 }  // namespace
 
+static int check_terms(exc_index* ix, const uint32_t* terms, uint32_t nt, uint64_t* total)
+{
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < nt; ++i) {
+        if (terms[i] >= ix->nterms) return set_error(EXC_ERR_ARG, "term %u out of range", terms[i]);
+        n += ix->n[terms[i]];
+    }
+    *total = n;
+    return EXC_OK;
+}
+
 extern "C" int exc_decompress(exc_ctx* c, exc_index* ix, const uint32_t* terms, uint32_t nt, uint32_t* out,
                               uint64_t cap, uint64_t* n_out)
 {
     if (!c || !ix || (nt && !terms)) return set_error(EXC_ERR_ARG, "exc_decompress: null argument");
-    std::vector<uint64_t> off(nt + 1, 0);
-    std::vector<uint32_t> ut, uk;
-    for (uint32_t i = 0; i < nt; ++i) {
-        if (terms[i] >= ix->nterms) return set_error(EXC_ERR_ARG, "term %u out of range", terms[i]);
-        const uint32_t n = ix->n[terms[i]];
-        off[i + 1] = off[i] + n;
-        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; ++k) { ut.push_back(i); uk.push_back(k); }
-    }
-    if (n_out) *n_out = off[nt];
-    if (off[nt] > cap) return set_error(EXC_ERR_CAPACITY, "output holds %llu, need %llu",
-                                        (unsigned long long)cap, (unsigned long long)off[nt]);
-    if (!off[nt]) return EXC_OK;
+    uint64_t total;
+    if (int rc = check_terms(ix, terms, nt, &total)) return rc;
+    if (n_out) *n_out = total;
+    if (total > cap) return set_error(EXC_ERR_CAPACITY, "output holds %llu, need %llu", (unsigned long long)cap,
+                                      (unsigned long long)total);
+    if (!total) return EXC_OK;
+    if (!out) return set_error(EXC_ERR_ARG, "exc_decompress: null output");
     HIP_TRY(hipSetDevice(c->device));
-    DevBuf dt, doff, dut, duk, dout;
-    int rc;
-    if ((rc = dev_alloc(c, dt, nt * 4)) || (rc = dev_alloc(c, doff, (nt + 1) * 8)) ||
-        (rc = dev_alloc(c, dut, ut.size() * 4)) || (rc = dev_alloc(c, duk, uk.size() * 4)) ||
-        (rc = dev_alloc(c, dout, off[nt] * 4))) {
-        for (DevBuf* b : {&dt, &doff, &dut, &duk, &dout}) dev_free(c, *b);
-        return rc;
-    }
-    hipError_t e = hipMemcpy(dt.p, terms, nt * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(doff.p, off.data(), (nt + 1) * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dut.p, ut.data(), ut.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(duk.p, uk.data(), uk.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = launch_decompress(ix->dix, dptr<uint32_t>(dt), dptr<uint64_t>(doff), dptr<uint32_t>(dut), dptr<uint32_t>(duk),
-                              (uint32_t)ut.size(), dptr<uint32_t>(dout), c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(out, dout.p, off[nt] * 4, hipMemcpyDeviceToHost);
-    for (DevBuf* b : {&dt, &doff, &dut, &duk, &dout}) dev_free(c, *b);
-    if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "exc_decompress: %s", hipGetErrorString(e));
-    return EXC_OK;
+    DevBuf dec;
+    std::vector<uint64_t> off;
+    int rc = decode_to_device(c, ix, std::vector<uint32_t>(terms, terms + nt), dec, off);
+    if (!rc && hipMemcpy(out, dec.p, total * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_error(EXC_ERR_DEVICE, "exc_decompress: copy to host failed");
+    dev_free(c, dec);
+    return rc;
+}
+
+extern "C" int exc_decompress_device(exc_ctx* c, exc_index* ix, const uint32_t* terms, uint32_t nt, uint32_t* d_out,
+                                     uint64_t cap, uint64_t* n_out)
+{
+    if (!c || !ix || (nt && !terms)) return set_error(EXC_ERR_ARG, "exc_decompress_device: null argument");
+    uint64_t total;
+    if (int rc = check_terms(ix, terms, nt, &total)) return rc;
+    if (n_out) *n_out = total;
+    if (total > cap) return set_error(EXC_ERR_CAPACITY, "output holds %llu, need %llu", (unsigned long long)cap,
+                                      (unsigned long long)total);
+    if (!total) return EXC_OK;
+    if (!d_out) return set_error(EXC_ERR_ARG, "exc_decompress_device: null output");
+    HIP_TRY(hipSetDevice(c->device));
+    DevBuf unused;
+    std::vector<uint64_t> off;
+    return decode_to_device(c, ix, std::vector<uint32_t>(terms, terms + nt), unused, off, d_out);
 }
 
 // ---------------------------------------------------------------- learned models

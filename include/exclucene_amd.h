@@ -96,10 +96,21 @@ int exc_ctx_create(int device, uint64_t mem_budget_bytes, exc_ctx **out);
 void exc_ctx_destroy(exc_ctx *ctx);
 const char *exc_last_error(void);
 
+/* Device buffers for callers without a HIP runtime of their own (no
+ * reference counterpart; the reference allocates everything itself in
+ * allocateResource).  They count against the context's budget and are freed
+ * with the context at the latest.  exc_memcpy waits for the context's stream;
+ * to_device = 1 copies host -> device, 0 device -> host. */
+int exc_device_alloc(exc_ctx *ctx, uint64_t bytes, void **d_ptr);
+int exc_device_free(exc_ctx *ctx, void *d_ptr);
+int exc_memcpy(exc_ctx *ctx, void *dst, const void *src, uint64_t bytes, int to_device);
+
 /* options (no reference counterpart: the reference fixes these with #define) */
 enum {
     EXC_OPT_SEARCH = 0,     /* 0 plain binary search, 1 fit window + fallback, 2 learned window + fallback */
-    EXC_OPT_TPE = 1,        /* threads per short-list element: 0 autotuned, 1 random access, 64 wave-cooperative decode */
+    EXC_OPT_TPE = 1,        /* threads per short-list element: 1 random access, 64 wave-cooperative decode,
+                               0 cooperative for runs up to EXC_OPT_COOP_MAX blocks.  Setting SEARCH, TPE
+                               or COOP_MAX drops a table installed by exc_autotune/exc_ctx_set_tuning. */
     EXC_OPT_PERSISTENT = 2, /* 1: persistent work-queue launch, 0: one wave per work unit */
     EXC_OPT_GRAPH = 3,      /* 1: replay kernelInvoke as a captured hipGraph */
     EXC_OPT_TIMING = 4,     /* 1: hipEvent timing around each kernel of kernelInvoke */
@@ -174,6 +185,11 @@ int exc_ctx_set_tuning(exc_ctx *ctx, const exc_tune_table *table);
  * decode whole lists into out (concatenated in the given order). */
 int exc_decompress(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, uint32_t nterms,
                    uint32_t *out, uint64_t out_cap, uint64_t *n_out);
+/* The same into device memory d_out (HIP device pointer of out_cap docIDs);
+ * with EXC_OPT_TIMING the decode kernel's time is reported by
+ * exc_ctx_last_times as probe_ms. */
+int exc_decompress_device(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, uint32_t nterms,
+                          uint32_t *d_out, uint64_t out_cap, uint64_t *n_out);
 
 /* ---- learned position models.  No reference counterpart: the reference's
  *      precursor is the single linear-regression window of

@@ -134,3 +134,30 @@ def test_budget_admission():
             assert e.value.code == -6
     finally:
         c.close()
+
+
+def test_decompress_into_device_memory(ctx):
+    lists = _corpus(26, 15)
+    docs, off = ea.lists_to_flat(lists)
+    for codec, frac in ((ea.SEGLRC, 0.0), (ea.PARAPFD, 0.1)):
+        built = ea.build_index(docs, off, codec, seg=128, frac=frac)
+        dix = ctx.upload(built)
+        try:
+            terms = [3, 0, 7, 7, 14]
+            want = np.concatenate([lists[t] for t in terms])
+            cap = len(want) + 10
+            out = ctx.device_alloc(cap * 4)
+            ctx.to_device(out, np.full(cap, 0xFFFFFFFF, np.uint32))
+            n = ctx.decompress_device(dix, terms, out, cap)
+            got = ctx.from_device(out, cap)
+            assert n == len(want) and np.array_equal(got[:n], want)
+            assert np.all(got[n:] == 0xFFFFFFFF)  # nothing written past the output
+            assert ctx.last_times()[0] > 0
+            with pytest.raises(ea.ExcError) as e:
+                ctx.decompress_device(dix, terms, out, len(want) - 1)
+            assert e.value.code == -3
+            ctx.device_free(out)
+            with pytest.raises(ea.ExcError):
+                ctx.device_free(out)
+        finally:
+            dix.close()
