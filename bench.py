@@ -200,6 +200,39 @@ def decode_leg(ea, ctx, dix, built, docs, reps=5):
             "matches_lists": same}
 
 
+def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
+    """C4: 10,000 queries of 2-4 terms (seed 44); the probe kernel with plain
+    binary search, the list-fit window and the learned window (PLA-256, then
+    MLP 1-32-1) for lists longer than 65,536; results must be identical."""
+    qs = W.df_weighted_queries(df, nq, 2, 4, 44)
+    coop = tune["coop_max"] if tune else [16] * 6
+    out, ref = {"queries": nq, "models_on_lists_longer_than": 65536}, None
+    for kind, name in ((ea.MODEL_PLA, "pla"), (ea.MODEL_MLP, "mlp")):
+        st = ctx.train_models(dix, kind, min_len=65536, epochs=300, samples=4096, lr=1e-3)
+        row = {"models": st["nmodels"], "train_ms": st["train_ms"], "mean_err": st["mean_err"], "max_err": st["max_err"]}
+        for search, sname in ((0, "binary"), (1, "fit"), (2, "learned")):
+            ctx.set_tuning({"coop_max": coop, "search": [search] * 6, "persistent": 0})
+            ctx.plan(dix, qs)
+            bi = ctx._bi
+            ctx.kernelInvoke(bi)
+            ms = []
+            for _ in range(5):
+                ctx.kernelInvoke(bi)
+                ms.append(ctx.last_times()[0])
+            cum, res = ctx.dtohTransfer(bi)
+            if ref is None:
+                ref = (cum, res)
+            row[sname + "_probe_ms"] = float(np.median(ms))
+            row[sname + "_qps"] = nq / (np.median(ms) * 1e-3)
+            row[sname + "_mismatches"] = int(0 if (np.array_equal(cum, ref[0]) and np.array_equal(res, ref[1]))
+                                             else max(1, int(np.sum(cum != ref[0]))))
+        row["learned_gain_vs_binary"] = row["binary_probe_ms"] / row["learned_probe_ms"]
+        out[name] = row
+    if tune:
+        ctx.set_tuning(tune)
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings of the probe kernel."""
@@ -225,6 +258,9 @@ def main():
     ap.add_argument("--nq", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
+    ap.add_argument("--no-c4", action="store_true", help="skip the learned-window leg")
+    ap.add_argument("--models", default="pla", choices=["pla", "mlp", "none"],
+                    help="learned position models for lists longer than 65,536 before tuning")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick knobs per ratio class on a separate query sample (seed 23) first")
     ap.add_argument("--tune-table", default=None,
@@ -261,6 +297,12 @@ def main():
 
     ctx = ea.Context(0)
     dix = ctx.upload(seg)
+
+    models = None
+    if args.models != "none":
+        models = ctx.train_models(dix, ea.MODEL_PLA if args.models == "pla" else ea.MODEL_MLP, min_len=65536,
+                                  epochs=300, samples=4096, lr=1e-3)
+        log("%s models: %s" % (args.models, models))
 
     tune = None
     if args.tune_table:
@@ -355,6 +397,10 @@ def main():
         log("ParaPFD batched: %.0f q/s" % extra["parapfd_batched_qps"])
         pix.close()
         del pfd
+
+    if not args.no_c4:
+        extra["learned"] = learned_leg(ea, W, ctx, dix, df, tune)
+        log("learned: %s" % extra["learned"])
 
     cpu = None
     if not args.no_cpu:

@@ -24,9 +24,11 @@ struct alignas(16) TermRec {
     float range_l;    // LRC fSearchRange | SegLRC list-level nRangeLeft
     float range_r;    // SegLRC list-level nRangeRight
     uint32_t model;   // learned model slot, or kNoModel
-    uint32_t pad0;
-    uint64_t pad1;
-    uint64_t pad2;
+    uint32_t term;    // the term's own index
+    float merr;       // learned model: recorded max error (positions)
+    float mx0;        //   input offset
+    float mxs;        //   input scale
+    uint32_t mkind;   //   kModelPla | kModelMlp
 };
 static_assert(sizeof(TermRec) == 64, "TermRec is one 64-byte record");
 
@@ -101,11 +103,22 @@ __host__ __device__ __forceinline__ uint32_t knob_search(uint32_t k) { return (k
 constexpr int kRatioClasses = 6;
 static_assert(kRatioClasses == EXC_RATIO_CLASSES, "ratio classes");
 
+// One record per work unit (query, kChunk-docID slice k of its shortest
+// list), carrying what a wave needs before its first index read.
+struct UnitRec {
+    uint32_t q;      // query
+    uint32_t k;      // slice of the shortest list
+    uint32_t toff;   // the query's first term record in the batch copy (sorted by length)
+    uint16_t tc;     // term count
+    uint16_t knobs;  // search knobs (make_knobs)
+};
+static_assert(sizeof(UnitRec) == 16, "UnitRec is 16 bytes");
+
 struct ProbeArgs {
     DevIndex ix;
     const QueryRec* queries;
-    const uint32_t* qterms;
-    const uint32_t* unit_q;
+    const TermRec* qrecs;  // per query term, sorted by length: copies of the index's TermRecs
+    const UnitRec* units;
     uint32_t nunits;
     int32_t persistent;
     uint32_t* stage;
@@ -115,7 +128,7 @@ struct ProbeArgs {
 
 struct ScanArgs {
     const QueryRec* queries;
-    const uint32_t* unit_q;
+    const UnitRec* units;
     const uint32_t* ucount;
     const uint32_t* stage;
     uint32_t nunits;
@@ -129,6 +142,7 @@ struct ScanArgs {
 
 constexpr uint32_t kScanBlockItems = 4096;  // units per scan block (256 threads x 16)
 constexpr uint32_t kChunk = 512;            // shortest-list docIDs per work unit
+constexpr uint32_t kDecodeUnit = 1024;      // docIDs per wave of whole-list decompression (HS: one chunk)
 
 // launchers (kernels.hip); all return hipError_t of the launch
 hipError_t launch_reset(uint32_t* counter, hipStream_t s);

@@ -19,6 +19,10 @@ constexpr uint32_t kModelMlp = 2;
 constexpr int kPlaKnots = 257;
 constexpr int kMlpH = 32;
 constexpr int kMlpParams = 3 * kMlpH + 1;  // w1, b1, w2, b2
+// floats per model slot in the parameter pool (padded)
+constexpr uint32_t kPlaStride = 260;
+constexpr uint32_t kMlpStride = 100;
+__host__ __device__ __forceinline__ uint32_t model_stride(uint32_t kind) { return kind == kModelPla ? kPlaStride : kMlpStride; }
 
 __host__ __device__ __forceinline__ float pla_pred(const float* y, float x0, float xs, uint32_t v)
 {

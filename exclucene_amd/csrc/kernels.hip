@@ -346,21 +346,22 @@ __device__ __forceinline__ void mlp_pred_pair(const float* w, float x0, float xs
 
 // Window [wa, wb] for "number of segment heads <= v" over [vmin, vmax] from
 // the list's learned model: prediction +- its recorded max error.
+// (The model's scale, offset and error travel in the list's TermRec, so only
+// the parameters are read here.)
 template <int SEG>
-__device__ __forceinline__ void model_window(const DevIndex& ix, uint32_t mid, uint32_t vmin, uint32_t vmax,
+__device__ __forceinline__ void model_window(const DevIndex& ix, const TermRec& R, uint32_t vmin, uint32_t vmax,
                                              int64_t& wa, int64_t& wb)
 {
-    const Model M = ix.models[mid];
-    const float* prm = ix.mparams + M.param_off;
+    const float* prm = ix.mparams + (uint64_t)R.model * model_stride(R.mkind);
     float plo, phi;
-    if (M.kind == kModelPla) {
-        plo = pla_pred(prm, M.x0, M.xs, vmin);
-        phi = pla_pred(prm, M.x0, M.xs, vmax);
+    if (R.mkind == kModelPla) {
+        plo = pla_pred(prm, R.mx0, R.mxs, vmin);
+        phi = pla_pred(prm, R.mx0, R.mxs, vmax);
     } else {
-        mlp_pred_pair(prm, M.x0, M.xs, M.n, vmin, vmax, plo, phi);
+        mlp_pred_pair(prm, R.mx0, R.mxs, R.n, vmin, vmax, plo, phi);
     }
-    plo = fminf(fmaxf(plo - M.err - 2.0f, -1.0e9f), 4.0e9f);
-    phi = fminf(fmaxf(phi + M.err + 2.0f, -1.0e9f), 4.0e9f);
+    plo = fminf(fmaxf(plo - R.merr - 2.0f, -1.0e9f), 4.0e9f);
+    phi = fminf(fmaxf(phi + R.merr + 2.0f, -1.0e9f), 4.0e9f);
     wa = plo <= 0.0f ? 0 : (int64_t)(plo / SEG);
     wb = phi <= 0.0f ? 1 : (int64_t)(phi / SEG) + 2;
 }
@@ -390,7 +391,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     if (search != SEARCH_BINARY && nseg > 64) {
         int64_t wa, t0, t1, wb;
         if (search == SEARCH_LEARNED && R.model != kNoModel) {
-            model_window<SEG>(A.ix, R.model, vmin, vmax, wa, wb);
+            model_window<SEG>(A.ix, R, vmin, vmax, wa, wb);
         } else {
             fit_window<CODEC, SEG>(R, vmin, wa, t0);
             fit_window<CODEC, SEG>(R, vmax, t1, wb);
@@ -683,10 +684,11 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     constexpr uint32_t U = kChunk / SEG;  // shortest-list segments per unit
     const LdsWave W{L, L + kChunk, L + kChunk + kBufWords};
     const uint32_t ln = lane_id();
-    const uint32_t q = uni(A.unit_q[u]);
-    const QueryRec Q = A.queries[q];
-    const uint32_t k = u - Q.first_unit;
-    const TermRec R0 = A.ix.terms[uni(A.qterms[Q.toff])];
+    const UnitRec ur = A.units[u];
+    const uint32_t k = uni(ur.k), tc = uni((uint32_t)ur.tc), knobs = uni((uint32_t)ur.knobs);
+    const TermRec* qr = A.qrecs + uni(ur.toff);
+    const TermRec R0 = qr[0];
+    const uint64_t short_off = A.queries[uni(ur.q)].short_off;  // used at the end only
     uint32_t nlive;
     if constexpr (CODEC == EXC_HS_SEGLRC) {
         nlive = decode_hs_chunk<kChunk>(A.ix, R0, k, W.cand);
@@ -698,14 +700,14 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     }
     nlive = uni(nlive);
     wave_sync();
-    for (uint32_t j = 1; j < Q.tc && nlive; ++j) {
-        const TermRec R = A.ix.terms[uni(A.qterms[Q.toff + j])];
+    for (uint32_t j = 1; j < tc && nlive; ++j) {
+        const TermRec R = qr[j];
         if constexpr (CODEC == EXC_HS_SEGLRC)
-            nlive = probe_hs(A, R, W, nlive, Q.knobs);
+            nlive = probe_hs(A, R, W, nlive, knobs);
         else
-            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, Q.knobs);
+            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs);
     }
-    uint32_t* out = A.stage + Q.short_off + (uint64_t)k * kChunk;
+    uint32_t* out = A.stage + short_off + (uint64_t)k * kChunk;
     for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];
     if (ln == 0) A.ucount[u] = nlive;
     wave_sync();
@@ -801,9 +803,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
     if (u < a.nunits) {
         const uint32_t cnt = a.ucount[u];
         if (cnt) {
-            const uint32_t q = a.unit_q[u];
-            const QueryRec Q = a.queries[q];
-            const uint32_t* src = a.stage + Q.short_off + (uint64_t)(u - Q.first_unit) * kChunk;
+            const UnitRec ur = a.units[u];
+            const uint32_t* src = a.stage + a.queries[ur.q].short_off + (uint64_t)ur.k * kChunk;
             uint32_t* dst = a.result + unit_excl(a, u);
             for (uint32_t e = ln; e < cnt; e += 64) dst[e] = src[e];
         }
@@ -816,33 +817,44 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
 }
 
 // ---- whole-list decompression (the *_Decompression programs)
+// One wave per unit of kDecodeUnit docIDs: a run of kDecodeUnit/SEG blocks of
+// one list (the HS codec: one chunk of SEG elements).  Every block record and
+// packed word of the run is requested before any value is formed; LRC and
+// SegLRC values go from registers straight to HBM in coalesced stores,
+// ParaPFD values pass through LDS for the exception patch and the scan.
 template <int CODEC, int SEG>
 __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const uint32_t* terms, const uint64_t* out_off,
                                                     const uint32_t* unit_term, const uint32_t* unit_k,
                                                     uint32_t nunits, uint32_t* out)
 {
-    constexpr int V = SEG / 64;
-    __shared__ uint32_t lds[kWaves * SEG];
+    constexpr bool HS = CODEC == EXC_HS_SEGLRC;
+    constexpr int U = HS ? 1 : (int)(kDecodeUnit / SEG);
+    constexpr bool VIA_LDS = HS || CODEC == EXC_PARAPFD;
+    __shared__ uint32_t lds[VIA_LDS ? kWaves * U * SEG : 1];
     const uint32_t wid = uni(threadIdx.x >> 6), ln = lane_id();
     const uint32_t u = blockIdx.x * kWaves + wid;
     if (u >= nunits) return;
-    uint32_t* L = lds + wid * SEG;
-    const uint32_t ti = uni(unit_term[u]), k = uni(unit_k[u]);
+    const uint32_t ti = uni(unit_term[u]), k = uni(unit_k[u]);  // k = first block (HS: chunk)
     const TermRec R = ix.terms[uni(terms[ti])];
-    uint32_t n;
-    if constexpr (CODEC == EXC_HS_SEGLRC) {
-        n = decode_hs_chunk<SEG>(ix, R, k, L);  // HS decompression works in chunks of SEG elements
-    } else {
-        const SegRec rec = ix.rec[R.segnum + k];
-        n = uni(seg_len(R, k, SEG));
-        const uint32_t* data = ix.words + R.woff + rec.off + 1;
-        uint32_t f[V];
-        load_fields<CODEC, SEG>(data, desc_lb(rec.desc), n, f);
-        finish_block<CODEC, SEG>(R, k, rec, n, data, f, L);
-    }
-    wave_sync();
     uint32_t* o = out + out_off[ti] + (uint64_t)k * SEG;
-    for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
+    if constexpr (HS) {
+        uint32_t* L = lds + wid * SEG;
+        const uint32_t n = decode_hs_chunk<SEG>(ix, R, k, L);
+        wave_sync();
+        for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
+    } else {
+        const uint32_t nb = uni(min((uint32_t)U, R.nseg - k));
+        const auto sid = [&](int g) { return k + (uint32_t)g; };
+        if constexpr (VIA_LDS) {
+            uint32_t* L = lds + wid * U * SEG;
+            decode_blocks<CODEC, SEG, U>(ix, R, nb, sid, L);
+            wave_sync();
+            const uint32_t n = (nb - 1) * SEG + seg_len(R, k + nb - 1, SEG);
+            for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
+        } else {
+            decode_blocks<CODEC, SEG, U>(ix, R, nb, sid, o);
+        }
+    }
 }
 
 template <int CODEC>

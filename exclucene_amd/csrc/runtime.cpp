@@ -76,12 +76,14 @@ struct exc_ctx {
     // batch plan (host)
     exc_index* ix = nullptr;
     std::vector<QueryRec> queries;
-    std::vector<uint32_t> qterms, unit_q;
+    std::vector<uint32_t> qterms;  // sorted terms of the batch
+    std::vector<TermRec> qrecs;    // their TermRec copies
+    std::vector<UnitRec> units;
     uint64_t short_sum = 0;
     uint32_t nq = 0, nunits = 0, seg = 0;
     int state = 0;  // 0 none, 1 planned, 2 on device, 3 invoked
     // device batch buffers
-    DevBuf d_queries, d_qterms, d_unit_q, d_stage, d_ucount, d_uoff, d_bsum, d_result, d_qcum, d_counter;
+    DevBuf d_queries, d_qrecs, d_units, d_stage, d_ucount, d_uoff, d_bsum, d_result, d_qcum, d_counter;
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float probe_ms = 0, invoke_ms = 0;
@@ -91,6 +93,7 @@ struct exc_ctx {
     bool tuned = false;
     exc_tune_table tune{};
     uint64_t opt_gen = 0, plan_gen = 0;  // knob options changed since the plan was made
+    uint64_t plan_model_gen = 0;         // model generation of the batch's TermRec copies
     // device buffers handed to the caller (exc_device_alloc)
     std::unordered_map<void*, size_t> user_bufs;
 };
@@ -106,6 +109,7 @@ struct exc_index {
     TermRec* d_terms = nullptr;
     std::vector<Model> models;     // learned models (host copy)
     DevBuf d_models, d_params;
+    uint64_t model_gen = 0;        // bumped whenever the learned models change
     std::vector<DevBuf> bufs;
     uint64_t bytes = 0;
     DevIndex dix{};
@@ -208,7 +212,7 @@ extern "C" void exc_ctx_destroy(exc_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (DevBuf* b : {&c->d_queries, &c->d_qterms, &c->d_unit_q, &c->d_stage, &c->d_ucount, &c->d_uoff,
+    for (DevBuf* b : {&c->d_queries, &c->d_qrecs, &c->d_units, &c->d_stage, &c->d_ucount, &c->d_uoff,
                       &c->d_bsum, &c->d_result, &c->d_qcum, &c->d_counter})
         dev_free(c, *b);
     for (auto& kv : c->user_bufs) (void)hipFree(kv.first);
@@ -334,6 +338,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
         R.n = n;
         R.segnum = h->seg_num[t];
         R.model = kNoModel;
+        R.term = t;
         uint32_t nseg;
         std::vector<uint32_t> blen;
         if (hs) {
@@ -569,7 +574,8 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
     c->seg = ix->seg;
     c->queries.resize(nq);
     c->qterms.clear();
-    c->unit_q.clear();
+    c->qrecs.clear();
+    c->units.clear();
     c->short_sum = 0;
     c->nq = nq;
     drop_graph(c);
@@ -598,12 +604,16 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         Q.knobs = knobs_for(c, ix->codec, Q.klass);
         c->short_sum += n0;
         c->qterms.insert(c->qterms.end(), tmp.begin(), tmp.end());
-        for (uint32_t k = 0; k < Q.nunits; ++k) c->unit_q.push_back(q);
+        for (uint32_t t : tmp) c->qrecs.push_back(ix->recs[t]);
+        if (tc > 0xffff) return set_error(EXC_ERR_ARG, "query %u: more than 65535 terms", q);
+        for (uint32_t k = 0; k < Q.nunits; ++k)
+            c->units.push_back(UnitRec{q, k, Q.toff, (uint16_t)tc, (uint16_t)Q.knobs});
         units += Q.nunits;
         if (units > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 work units in one batch");
     }
     c->nunits = (uint32_t)units;
     c->plan_gen = c->opt_gen;
+    c->plan_model_gen = ix->model_gen;
     if (c->short_sum > 0xffffffffull)
         return set_error(EXC_ERR_CAPACITY, "summed shortest-list length %llu exceeds 2^32",
                          (unsigned long long)c->short_sum);
@@ -618,8 +628,8 @@ int upload_plan(exc_ctx* c, bool sync = true)
     const uint32_t nblk = (c->nunits + kScanBlockItems - 1) / kScanBlockItems;
     int rc;
     if ((rc = dev_alloc(c, c->d_queries, std::max<size_t>(1, c->nq) * sizeof(QueryRec))) ||
-        (rc = dev_alloc(c, c->d_qterms, std::max<size_t>(1, c->qterms.size()) * 4)) ||
-        (rc = dev_alloc(c, c->d_unit_q, std::max<size_t>(1, c->nunits) * 4)) ||
+        (rc = dev_alloc(c, c->d_qrecs, std::max<size_t>(1, c->qrecs.size()) * sizeof(TermRec))) ||
+        (rc = dev_alloc(c, c->d_units, std::max<size_t>(1, c->nunits) * sizeof(UnitRec))) ||
         (rc = dev_alloc(c, c->d_stage, std::max<uint64_t>(1, c->short_sum) * 4)) ||
         (rc = dev_alloc(c, c->d_ucount, std::max<size_t>(1, c->nunits) * 4)) ||
         (rc = dev_alloc(c, c->d_uoff, std::max<size_t>(1, c->nunits) * 4)) ||
@@ -630,8 +640,12 @@ int upload_plan(exc_ctx* c, bool sync = true)
         return rc;
     drop_graph(c);
     if (c->nq) HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice, c->stream));
-    if (!c->qterms.empty()) HIP_TRY(hipMemcpyAsync(c->d_qterms.p, c->qterms.data(), c->qterms.size() * 4, hipMemcpyHostToDevice, c->stream));
-    if (c->nunits) HIP_TRY(hipMemcpyAsync(c->d_unit_q.p, c->unit_q.data(), (size_t)c->nunits * 4, hipMemcpyHostToDevice, c->stream));
+    if (!c->qrecs.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_qrecs.p, c->qrecs.data(), c->qrecs.size() * sizeof(TermRec), hipMemcpyHostToDevice,
+                               c->stream));
+    if (c->nunits)
+        HIP_TRY(hipMemcpyAsync(c->d_units.p, c->units.data(), (size_t)c->nunits * sizeof(UnitRec),
+                               hipMemcpyHostToDevice, c->stream));
     if (sync) HIP_TRY(hipStreamSynchronize(c->stream));
     c->state = 2;
     return EXC_OK;
@@ -642,11 +656,26 @@ int upload_plan(exc_ctx* c, bool sync = true)
 // stays valid).
 int refresh_knobs(exc_ctx* c)
 {
+    if (c->plan_model_gen != c->ix->model_gen) {
+        // the index's models changed after planning: fresh record copies, and
+        // no graph that captured the old model arrays
+        for (size_t i = 0; i < c->qterms.size(); ++i) c->qrecs[i] = c->ix->recs[c->qterms[i]];
+        if (!c->qrecs.empty())
+            HIP_TRY(hipMemcpyAsync(c->d_qrecs.p, c->qrecs.data(), c->qrecs.size() * sizeof(TermRec),
+                                   hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        drop_graph(c);
+        c->plan_model_gen = c->ix->model_gen;
+    }
     if (c->plan_gen == c->opt_gen) return EXC_OK;
     for (auto& Q : c->queries) Q.knobs = knobs_for(c, c->ix->codec, Q.klass);
+    for (auto& U : c->units) U.knobs = (uint16_t)c->queries[U.q].knobs;
     if (c->nq)
         HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice,
                                c->stream));
+    if (c->nunits)
+        HIP_TRY(hipMemcpyAsync(c->d_units.p, c->units.data(), (size_t)c->nunits * sizeof(UnitRec),
+                               hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->plan_gen = c->opt_gen;
     return EXC_OK;
@@ -659,8 +688,8 @@ int enqueue_invoke(exc_ctx* c, bool timed)
     std::memset(&a, 0, sizeof a);
     a.ix = ix->dix;
     a.queries = dptr<QueryRec>(c->d_queries);
-    a.qterms = dptr<uint32_t>(c->d_qterms);
-    a.unit_q = dptr<uint32_t>(c->d_unit_q);
+    a.qrecs = dptr<TermRec>(c->d_qrecs);
+    a.units = dptr<UnitRec>(c->d_units);
     a.nunits = c->nunits;
     a.persistent = c->persistent;
     a.stage = dptr<uint32_t>(c->d_stage);
@@ -681,7 +710,7 @@ int enqueue_invoke(exc_ctx* c, bool timed)
     ScanArgs s;
     std::memset(&s, 0, sizeof s);
     s.queries = a.queries;
-    s.unit_q = a.unit_q;
+    s.units = a.units;
     s.ucount = a.ucount;
     s.stage = a.stage;
     s.nunits = c->nunits;
@@ -785,7 +814,7 @@ extern "C" int exc_CPUPreprocess(exc_ctx* c, exc_index* ix, const uint8_t** ptr_
     if (int rc = plan_queries(c, ix, terms.data(), off.data(), nq)) return rc;
     *ptr_in = p;
     bi->blockNum = c->nunits;
-    bi->constantUsedInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, nq * sizeof(QueryRec) + c->qterms.size() * 4 + (uint64_t)c->nunits * 4);
+    bi->constantUsedInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, nq * sizeof(QueryRec) + c->qrecs.size() * sizeof(TermRec) + (uint64_t)c->nunits * sizeof(UnitRec));
     bi->ucntShortest_sum = (uint32_t)c->short_sum;
     bi->nTotalQueryNum = nq;
     bi->nLinearInfoInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, c->qterms.size() * sizeof(TermRec));
@@ -837,7 +866,7 @@ extern "C" int exc_plan(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const
     if (!c || !ix || !qoff || (nq && !qterms && qoff[nq]) || !bi) return set_error(EXC_ERR_ARG, "exc_plan: null argument");
     if (int rc = plan_queries(c, ix, qterms, qoff, nq)) return rc;
     bi->blockNum = c->nunits;
-    bi->constantUsedInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, nq * sizeof(QueryRec) + c->qterms.size() * 4 + (uint64_t)c->nunits * 4);
+    bi->constantUsedInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, nq * sizeof(QueryRec) + c->qrecs.size() * sizeof(TermRec) + (uint64_t)c->nunits * sizeof(UnitRec));
     bi->ucntShortest_sum = (uint32_t)c->short_sum;
     bi->nTotalQueryNum = nq;
     bi->nLinearInfoInByte = (uint32_t)std::min<uint64_t>(0xffffffffu, c->qterms.size() * sizeof(TermRec));
@@ -868,10 +897,12 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
     const uint32_t nt = (uint32_t)terms.size();
     off.assign(nt + 1, 0);
     std::vector<uint32_t> ut, uk;
+    // units: runs of kDecodeUnit/seg blocks (HS: chunks of seg elements), k = first block
+    const uint32_t step = ix->codec == EXC_HS_SEGLRC ? 1u : kDecodeUnit / ix->seg;
     for (uint32_t i = 0; i < nt; ++i) {
         const uint32_t n = ix->n[terms[i]];
         off[i + 1] = off[i] + n;
-        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; ++k) { ut.push_back(i); uk.push_back(k); }
+        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; k += step) { ut.push_back(i); uk.push_back(k); }
     }
     c->probe_ms = 0;
     if (!off[nt]) return EXC_OK;
@@ -965,7 +996,7 @@ extern "C" int exc_index_train_models(exc_ctx* c, exc_index* ix, int kind, uint3
     for (uint32_t t = 0; t < ix->nterms; ++t)
         if (ix->n[t] > min_len && ix->n[t] >= 2) terms.push_back(t);
     const uint32_t nm = (uint32_t)terms.size();
-    const uint32_t P = kind == EXC_MODEL_PLA ? 260u : 100u;  // floats per model (padded)
+    const uint32_t P = model_stride((uint32_t)kind);  // floats per model slot
     ix->models.assign(nm, Model{});
     for (uint32_t m = 0; m < nm; ++m) {
         Model& M = ix->models[m];
@@ -1042,8 +1073,21 @@ extern "C" int exc_index_train_models(exc_ctx* c, exc_index* ix, int kind, uint3
         sum += e;
         mx = std::max(mx, e);
     }
-    for (auto& R : ix->recs) R.model = kNoModel;
-    for (uint32_t m = 0; m < nm; ++m) ix->recs[terms[m]].model = m;
+    for (auto& R : ix->recs) {
+        R.model = kNoModel;
+        R.mkind = 0;
+        R.merr = R.mx0 = R.mxs = 0.0f;
+    }
+    for (uint32_t m = 0; m < nm; ++m) {
+        TermRec& R = ix->recs[terms[m]];
+        const Model& M = ix->models[m];
+        R.model = m;
+        R.mkind = M.kind;
+        R.merr = M.err;
+        R.mx0 = M.x0;
+        R.mxs = M.xs;
+    }
+    ++ix->model_gen;  // batch copies of the records are refreshed before their next launch
     if (nm) HIP_TRY(hipMemcpy(ix->d_models.p, ix->models.data(), nm * sizeof(Model), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ix->d_terms, ix->recs.data(), ix->recs.size() * sizeof(TermRec), hipMemcpyHostToDevice));
     ix->dix.models = dptr<Model>(ix->d_models);

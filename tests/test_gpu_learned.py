@@ -101,3 +101,27 @@ def test_learned_search_is_exact(ctx, indexed, kind):
     finally:
         ctx.set_option(ea.OPT_SEARCH, 1)
         ctx.set_option(ea.OPT_TPE, 0)
+
+
+def test_retrain_after_plan_refreshes_batch(ctx, indexed):
+    """A planned batch (and its captured graph) replayed after the models were
+    retrained uses the new models and stays exact."""
+    lists, built, dix = indexed
+    rng = np.random.default_rng(12)
+    qs = [[int(x) for x in rng.choice(len(lists), int(rng.integers(2, 4)), replace=False)] for _ in range(200)]
+    ocum, ores = oracle.eval_queries(oracle.build_index(lists, oracle.SEGLRC, seg=128), qs)
+    ctx.train_models(dix, ea.MODEL_PLA, min_len=65536)
+    try:
+        ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_LEARNED)
+        for graph in (0, 1):
+            ctx.set_option(ea.OPT_GRAPH, graph)
+            bi = ctx.plan(dix, qs)
+            ctx.kernelInvoke(bi)
+            assert all(np.array_equal(a, b) for a, b in zip(ctx.dtohTransfer(bi), (ocum, ores)))
+            ctx.train_models(dix, ea.MODEL_MLP, min_len=65536, epochs=50, samples=2048, lr=1e-3)
+            ctx.kernelInvoke(bi)
+            assert all(np.array_equal(a, b) for a, b in zip(ctx.dtohTransfer(bi), (ocum, ores))), graph
+            ctx.train_models(dix, ea.MODEL_PLA, min_len=65536)
+    finally:
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        ctx.set_option(ea.OPT_SEARCH, 1)
