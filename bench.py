@@ -25,6 +25,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# k_probe<SegLRC,128> HBM bytes per launch on the C2 batch with the tuned knobs, in KiB: rocprofv3
+# FETCH_SIZE (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
+TRAFFIC_KIB = 1573484.0
 META = {0: 8, 1: 28, 2: 32, 3: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD
 
 
@@ -268,9 +271,7 @@ def main():
     ap.add_argument("--search", type=int, default=1, help="0 binary, 1 fit window, 2 learned")
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--coop-max", type=int, default=16)
-    # k_probe<SegLRC,128> HBM traffic per launch on this workload with the tuned knobs: rocprofv3 FETCH_SIZE
-    # (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
-    ap.add_argument("--traffic-kib", type=float, default=1282431.0,
+    ap.add_argument("--traffic-kib", type=float, default=TRAFFIC_KIB,
                     help="scaled rocprofv3 FETCH_SIZE+WRITE_SIZE KiB per probe launch (0: report null)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (rocprofv3 counter scaling)")

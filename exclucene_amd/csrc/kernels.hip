@@ -38,7 +38,8 @@ namespace {
 constexpr int kWaves = 4;
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kClaim = 4;        // units per work-queue claim (persistent launch)
-constexpr uint32_t kBufWords = 1024;  // per-wave LDS decode buffer
+constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
+constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative LRC/SegLRC probe
 constexpr int kV = (int)(kChunk / 64);  // candidates per lane
 
 __device__ __forceinline__ void wave_sync()
@@ -284,6 +285,17 @@ __device__ __forceinline__ bool seg_find(const uint32_t* data, uint32_t len, con
     return p > 0 && seg_bsearch(data, lb, r, 0, p - 1, v);
 }
 
+// Value j of block s of a random-access list from its residual f
+// (the expressions of finish_block).
+template <int CODEC, int SEG>
+__device__ __forceinline__ uint32_t block_value(const TermRec& R, uint32_t s, const SegRec& rec, uint32_t j, uint32_t f)
+{
+    if constexpr (CODEC == EXC_LRC)
+        return f + (uint32_t)(lrc_y(s * SEG + j, R.slope) + R.icpt);
+    else
+        return f + (uint32_t)(seg_y(j, rec.slope, rec.icpt) + rec.base);
+}
+
 // Branchless search of v in the sorted LDS run buf[0..n) (n uniform).
 __device__ __forceinline__ bool lds_contains(const uint32_t* buf, uint32_t n, uint32_t v)
 {
@@ -441,7 +453,73 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     }
 
     const uint32_t cmax = knob_coop(knobs);
-    if (r <= cmax) {
+    constexpr uint32_t kWin = kBufWords * 32u;  // docIDs one LDS bitmap covers
+    if (CODEC != EXC_PARAPFD && r <= cmax && ((vmax - vmin) / kWin) < kMaxWindows) {
+        // ---- LRC/SegLRC: the run sf .. sf+r-1 decoded straight from
+        // registers into a bitmap of a docID window in LDS; every candidate of
+        // the window then tests its bit.  Windows of kWin docIDs cover
+        // [vmin, vmax]; each decodes only the blocks that overlap it (the run's
+        // heads tell which) and windows without candidates are skipped.
+        // Independent LDS operations only, no per-candidate search chains.
+        uint32_t* bm = W.buf;
+        const SegRec gr = A.ix.rec[R.segnum + sf + min(ln, r - 1)];
+        const uint32_t hd = ln < r ? gr.head : kNone;
+        for (uint32_t lo = vmin;;) {
+            const uint32_t hi = (vmax - lo < kWin) ? vmax : lo + (kWin - 1);
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < V; ++i) any = any || (ln + 64 * i < nlive && v[i] >= lo && v[i] <= hi);
+            if (__ballot(any)) {
+                const uint32_t c_lo = popc(__ballot(hd <= lo)), c_hi = popc(__ballot(hd <= hi));
+                const uint32_t b0 = c_lo ? c_lo - 1 : 0, b1 = c_hi ? c_hi - 1 : 0;
+                const uint32_t nw = ((hi - lo) >> 5) + 1;
+                for (uint32_t e = ln; e < nw; e += 64) bm[e] = 0;
+                wave_sync();
+                for (uint32_t p0 = b0; p0 <= b1; p0 += NB) {
+                    const uint32_t np = min((uint32_t)NB, b1 + 1 - p0);
+                    SegRec rec[NB];
+                    uint32_t len[NB];
+                    uint32_t f[NB][SEG / 64];
+#pragma unroll
+                    for (int g = 0; g < NB; ++g) {
+                        if ((uint32_t)g < np) {
+                            const int src = (int)(p0 + g);
+                            rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
+                            rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, src));
+                            rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), src)));
+                            rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
+                            rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
+                            len[g] = seg_len(R, sf + p0 + g, SEG);
+                            load_fields<CODEC, SEG>(A.ix.words + R.woff + rec[g].off + 1, desc_lb(rec[g].desc), len[g],
+                                                    f[g]);
+                        }
+                    }
+#pragma unroll
+                    for (int g = 0; g < NB; ++g) {
+                        if ((uint32_t)g < np) {
+#pragma unroll
+                            for (int t = 0; t < SEG / 64; ++t) {
+                                const uint32_t j = ln + 64 * t;
+                                const uint32_t x = block_value<CODEC, SEG>(R, sf + p0 + g, rec[g], j, f[g][t]) - lo;
+                                if (j < len[g] && x <= hi - lo)
+                                    __hip_atomic_fetch_or(bm + (x >> 5), 1u << (x & 31), __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    const uint32_t x = v[i] - lo;
+                    if (ln + 64 * i < nlive && v[i] >= lo && x <= hi - lo) found[i] = (bm[x >> 5] >> (x & 31)) & 1u;
+                }
+                wave_sync();
+            }
+            if (hi == vmax) break;
+            lo = hi + 1;
+        }
+    } else if (CODEC == EXC_PARAPFD && r <= cmax) {
         // ---- the run of segments sf .. sf+r-1 decoded piece by piece into LDS;
         // one gather brings every segment record of the run, one load round
         // per piece brings the packed words, sorted candidates resolve in the
@@ -714,7 +792,10 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
 }
 
 template <int CODEC, int SEG>
-__global__ __launch_bounds__(256) void k_probe(ProbeArgs A)
+// Eight waves per SIMD for the random-access codecs (64 VGPRs, 20 KB of LDS
+// per workgroup): the probe is bound by load latency, and the extra resident
+// waves hide more of it than the few spilled registers cost.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 8, 8))) void k_probe(ProbeArgs A)
 {
     __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);
