@@ -111,14 +111,22 @@ struct UnitRec {
     uint32_t toff;   // the query's first term record in the batch copy (sorted by length)
     uint16_t tc;     // term count
     uint16_t knobs;  // search knobs (make_knobs)
+    // Filled on the device before the probe (k_unit_prologue): the blocks of
+    // the second-shortest list that can hold a docID of this slice, found from
+    // the slice's bounding segment heads.  l1_count = kUnset: not computed,
+    // 0: no docID of the slice can be in that list.
+    uint32_t l1_first;
+    uint32_t l1_count;
+    uint32_t pad[2];
 };
-static_assert(sizeof(UnitRec) == 16, "UnitRec is 16 bytes");
+static_assert(sizeof(UnitRec) == 32, "UnitRec is 32 bytes");
+constexpr uint32_t kUnset = 0xffffffffu;
 
 struct ProbeArgs {
     DevIndex ix;
     const QueryRec* queries;
     const TermRec* qrecs;  // per query term, sorted by length: copies of the index's TermRecs
-    const UnitRec* units;
+    UnitRec* units;
     uint32_t nunits;
     int32_t persistent;
     uint32_t* stage;
@@ -147,6 +155,7 @@ constexpr uint32_t kDecodeUnit = 1024;      // docIDs per wave of whole-list dec
 // launchers (kernels.hip); all return hipError_t of the launch
 hipError_t launch_reset(uint32_t* counter, hipStream_t s);
 hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s);
+hipError_t launch_unit_prologue(const ProbeArgs& a, hipStream_t s);
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
 hipError_t launch_decompress(const DevIndex& ix, const uint32_t* terms, const uint64_t* out_off,
                              const uint32_t* unit_term, const uint32_t* unit_k, uint32_t nunits,

@@ -40,6 +40,7 @@ constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kClaim = 4;        // units per work-queue claim (persistent launch)
 constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
 constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative LRC/SegLRC probe
+constexpr uint32_t kBitmapPiece = 768; // docIDs decoded per load round of the bitmap path
 constexpr int kV = (int)(kChunk / 64);  // candidates per lane
 
 __device__ __forceinline__ void wave_sync()
@@ -385,7 +386,8 @@ struct LdsWave {
 };
 
 template <int CODEC, int SEG>
-__device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
+__device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs,
+                              uint32_t pre_first, uint32_t pre_count)
 {
     const uint32_t search = knob_search(knobs);
     constexpr int V = kV;
@@ -400,7 +402,16 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     int64_t hb0 = 0;
     uint32_t H = kNone;
     bool small = false;
-    if (search != SEARCH_BINARY && nseg > 64) {
+    int64_t rng = 0;
+    if (pre_count != kUnset) {
+        // blocks found by the prologue from the slice's bounding heads (a
+        // superset of those the live candidates need)
+        if (pre_count == 0) return 0;
+        hb0 = pre_first;
+        rng = pre_count;
+        small = rng <= 64;
+        H = (small && (int64_t)ln < rng) ? head[hb0 + ln] : kNone;
+    } else if (search != SEARCH_BINARY && nseg > 64) {
         int64_t wa, t0, t1, wb;
         if (search == SEARCH_LEARNED && R.model != kNoModel) {
             model_window<SEG>(A.ix, R, vmin, vmax, wa, wb);
@@ -419,8 +430,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             small = h0 <= vmin && hend > vmax;
         }
     }
-    int64_t rng = 0;
-    if (!small) {
+    if (!small && pre_count == kUnset) {
         const uint32_t ub_lo = uni(wave_upper_bound(head, (uint32_t)nseg, vmin));
         const uint32_t ub_hi = uni(wave_upper_bound(head, (uint32_t)nseg, vmax));
         if (ub_hi == 0) return 0;  // every candidate precedes the list
@@ -454,6 +464,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 
     const uint32_t cmax = knob_coop(knobs);
     constexpr uint32_t kWin = kBufWords * 32u;  // docIDs one LDS bitmap covers
+    constexpr int NBB = kBitmapPiece / SEG;       // blocks per load round of the bitmap path
     if (CODEC != EXC_PARAPFD && r <= cmax && ((vmax - vmin) / kWin) < kMaxWindows) {
         // ---- LRC/SegLRC: the run sf .. sf+r-1 decoded straight from
         // registers into a bitmap of a docID window in LDS; every candidate of
@@ -475,13 +486,13 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                 const uint32_t nw = ((hi - lo) >> 5) + 1;
                 for (uint32_t e = ln; e < nw; e += 64) bm[e] = 0;
                 wave_sync();
-                for (uint32_t p0 = b0; p0 <= b1; p0 += NB) {
-                    const uint32_t np = min((uint32_t)NB, b1 + 1 - p0);
-                    SegRec rec[NB];
-                    uint32_t len[NB];
-                    uint32_t f[NB][SEG / 64];
+                for (uint32_t p0 = b0; p0 <= b1; p0 += NBB) {
+                    const uint32_t np = min((uint32_t)NBB, b1 + 1 - p0);
+                    SegRec rec[NBB];
+                    uint32_t len[NBB];
+                    uint32_t f[NBB][SEG / 64];
 #pragma unroll
-                    for (int g = 0; g < NB; ++g) {
+                    for (int g = 0; g < NBB; ++g) {
                         if ((uint32_t)g < np) {
                             const int src = (int)(p0 + g);
                             rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
@@ -495,7 +506,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                         }
                     }
 #pragma unroll
-                    for (int g = 0; g < NB; ++g) {
+                    for (int g = 0; g < NBB; ++g) {
                         if ((uint32_t)g < np) {
 #pragma unroll
                             for (int t = 0; t < SEG / 64; ++t) {
@@ -764,6 +775,7 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     const uint32_t ln = lane_id();
     const UnitRec ur = A.units[u];
     const uint32_t k = uni(ur.k), tc = uni((uint32_t)ur.tc), knobs = uni((uint32_t)ur.knobs);
+    const uint32_t l1_first = uni(ur.l1_first), l1_count = uni(ur.l1_count);
     const TermRec* qr = A.qrecs + uni(ur.toff);
     const TermRec R0 = qr[0];
     const uint64_t short_off = A.queries[uni(ur.q)].short_off;  // used at the end only
@@ -783,7 +795,7 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
         if constexpr (CODEC == EXC_HS_SEGLRC)
             nlive = probe_hs(A, R, W, nlive, knobs);
         else
-            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs);
+            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs, j == 1 ? l1_first : 0u, j == 1 ? l1_count : kUnset);
     }
     uint32_t* out = A.stage + short_off + (uint64_t)k * kChunk;
     for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];
@@ -791,10 +803,10 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     wave_sync();
 }
 
-template <int CODEC, int SEG>
 // Eight waves per SIMD for the random-access codecs (64 VGPRs, 20 KB of LDS
 // per workgroup): the probe is bound by load latency, and the extra resident
 // waves hide more of it than the few spilled registers cost.
+template <int CODEC, int SEG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 8, 8))) void k_probe(ProbeArgs A)
 {
     __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
@@ -813,6 +825,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EX
         const uint32_t u = blockIdx.x * kWaves + wid;
         if (u < A.nunits) run_unit<CODEC, SEG>(A, u, L);
     }
+}
+
+// One thread per work unit: the blocks of the unit's second-shortest list
+// that can hold a docID of its slice.  The slice lies in [head of its first
+// block, head of the block after it - 1] of the shortest list, so two
+// branchless upper-bound searches over the second list's heads bound the run
+// of blocks the probe must look at, off the probe kernel's critical path.
+template <int SEG>
+__global__ __launch_bounds__(256) void k_unit_prologue(ProbeArgs A)
+{
+    const uint32_t u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= A.nunits) return;
+    const UnitRec ur = A.units[u];
+    if (ur.tc < 2) return;
+    constexpr uint32_t U = kChunk / SEG;
+    const TermRec R0 = A.qrecs[ur.toff], R1 = A.qrecs[ur.toff + 1];
+    const uint32_t b0 = ur.k * U, nb = min(U, R0.nseg - b0);
+    const uint32_t* h0 = A.ix.seg_head + R0.segnum;
+    const uint32_t vmin = h0[b0];
+    const uint32_t vmax = (b0 + nb < R0.nseg) ? h0[b0 + nb] - 1u : 0xffffffffu;
+    const uint32_t* h1 = A.ix.seg_head + R1.segnum;
+    const uint32_t n = R1.nseg;
+    uint32_t a = 0, b = 0;  // upper bounds of vmin and vmax
+    for (uint32_t step = n ? 1u << (31 - __builtin_clz(n)) : 0u; step; step >>= 1) {
+        if (a + step <= n && h1[a + step - 1] <= vmin) a += step;
+        if (b + step <= n && h1[b + step - 1] <= vmax) b += step;
+    }
+    const uint32_t first = a ? a - 1 : 0;
+    A.units[u].l1_first = first;
+    A.units[u].l1_count = b ? b - first : 0;
 }
 
 __global__ void k_reset(uint32_t* c) { if (threadIdx.x == 0) *c = 0; }
@@ -1000,6 +1042,17 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s)
     case EXC_PARAPFD: return probe_for_seg<EXC_PARAPFD>(a, grid, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_unit_prologue(const ProbeArgs& a, hipStream_t s)
+{
+    if (!a.nunits || a.ix.codec == EXC_HS_SEGLRC) return hipSuccess;  // HS probes find buckets directly
+    const uint32_t grid = (a.nunits + 255) / 256;
+    if (a.ix.seg == 256)
+        hipLaunchKernelGGL(k_unit_prologue<256>, dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_unit_prologue<128>, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s)

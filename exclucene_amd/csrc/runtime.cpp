@@ -607,7 +607,7 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         for (uint32_t t : tmp) c->qrecs.push_back(ix->recs[t]);
         if (tc > 0xffff) return set_error(EXC_ERR_ARG, "query %u: more than 65535 terms", q);
         for (uint32_t k = 0; k < Q.nunits; ++k)
-            c->units.push_back(UnitRec{q, k, Q.toff, (uint16_t)tc, (uint16_t)Q.knobs});
+            c->units.push_back(UnitRec{q, k, Q.toff, (uint16_t)tc, (uint16_t)Q.knobs, 0, kUnset, {0, 0}});
         units += Q.nunits;
         if (units > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 work units in one batch");
     }
@@ -705,6 +705,7 @@ int enqueue_invoke(exc_ctx* c, bool timed)
         grid = std::max<uint32_t>(1, (c->nunits + wpb - 1) / wpb);
     }
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    if (c->nunits) HIP_TRY(launch_unit_prologue(a, c->stream));
     if (c->nunits) HIP_TRY(launch_probe(a, grid, c->stream));
     if (timed) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     ScanArgs s;
