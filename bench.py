@@ -236,6 +236,71 @@ def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
     return out
 
 
+def ratio_class(df, q):
+    """List-length-ratio class of a query (as the library's autotuner forms them)."""
+    s = sorted(q, key=lambda t: df[t])
+    if len(s) < 2:
+        return 5
+    r, k = df[s[1]] / max(1, df[s[0]]), 0
+    while k < 4 and r >= 10:
+        r /= 10
+        k += 1
+    return k
+
+
+def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
+    """C3: the C2 lists in hash buckets on the top docID bits (shift 13 = the top
+    12 bits of 2^25; also 12, 14 and the adaptive ~256-per-bucket width), 10,000
+    2-term queries with long:short ratios 10^U(1,4).  Per bucket width: search
+    knobs autotuned per ratio class, the whole batch as one persistent launch
+    replayed from a hipGraph, and every class's sub-batch timed; the bucket
+    width is then chosen per ratio class and the batch run routed that way."""
+    qs = W.ratio_queries(df, 10_000, 3)
+    sample = W.ratio_queries(df, 4000, 33)
+    rcum, rres = ctx.intersect(ref_dix, qs)  # SegLRC result of the same queries
+    cls = np.array([ratio_class(df, q) for q in qs])
+    classes = [int(k) for k in np.unique(cls)]
+    out = {"queries": len(qs), "classes": {str(k): int((cls == k).sum()) for k in classes}, "per_width": {}}
+    keep = {}
+    for shift in shifts:
+        hs = ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=shift)
+        dix = ctx.upload(hs)
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        tune = ctx.autotune(dix, sample, reps=2)
+        row = {"device_bytes": dix.device_bytes(), "search": tune["search"]}
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        ctx.set_option(ea.OPT_GRAPH, 1)
+        dt, _, inv = time_batch(ctx, dix, qs, steps, 2)
+        cum, res = ctx.dtohTransfer(ctx._bi)
+        row["graph_persistent_qps"] = len(qs) * steps / dt
+        row["graph_step_ms"] = float(np.median(inv))
+        row["matches_seglrc"] = bool(np.array_equal(cum, rcum) and np.array_equal(res, rres))
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        ctx.set_option(ea.OPT_PERSISTENT, 0)
+        row["class_probe_ms"] = {}
+        for k in classes:
+            sub = [q for q, c in zip(qs, cls) if c == k]
+            _, pr, _ = time_batch(ctx, dix, sub, 5, 1)
+            row["class_probe_ms"][str(k)] = float(np.median(pr))
+        out["per_width"]["adaptive" if shift == 0 else "shift%d" % shift] = row
+        keep[shift] = (dix, hs, row)
+        log("C3 shift %d: %s" % (shift, row))
+    # bucket width per ratio class, then the batch routed class by class
+    choice, routed_ms = {}, 0.0
+    for k in classes:
+        best = min(shifts, key=lambda sh: keep[sh][2]["class_probe_ms"][str(k)])
+        choice[str(k)] = "adaptive" if best == 0 else "shift%d" % best
+        routed_ms += keep[best][2]["class_probe_ms"][str(k)]
+    out["width_per_class"] = choice
+    out["routed_probe_ms"] = routed_ms
+    out["routed_qps"] = len(qs) / (routed_ms * 1e-3)
+    out["fixed_shift13_qps"] = keep[13][2]["graph_persistent_qps"]
+    for dix, hs, _ in keep.values():
+        dix.close()
+    ctx.set_tuning(None)
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings of the probe kernel."""
@@ -261,6 +326,7 @@ def main():
     ap.add_argument("--nq", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the hash-bucket leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-window leg")
     ap.add_argument("--models", default="pla", choices=["pla", "mlp", "none"],
                     help="learned position models for lists longer than 65,536 before tuning")
@@ -398,6 +464,11 @@ def main():
         log("ParaPFD batched: %.0f q/s" % extra["parapfd_batched_qps"])
         pix.close()
         del pfd
+
+    if not args.no_c3:
+        extra["hash_buckets"] = c3_leg(ea, W, ctx, docs, off, df, dix, max(5, args.steps // 2))
+        log("C3: %s" % extra["hash_buckets"])
+        headline_opts()
 
     if not args.no_c4:
         extra["learned"] = learned_leg(ea, W, ctx, dix, df, tune)
