@@ -33,6 +33,7 @@ EXPORTS = [
     "exc_server_create", "exc_server_submit", "exc_server_wait", "exc_server_replay", "exc_server_stats",
     "exc_server_destroy", "exc_autotune", "exc_ctx_set_tuning", "exc_decompress_device",
     "exc_device_alloc", "exc_device_free", "exc_memcpy",
+    "exc_synth_lists_device", "exc_build_index_device", "exc_synth_index_device", "exc_index_export",
 ]
 MODEL_PLA, MODEL_MLP = 1, 2
 
@@ -146,6 +147,11 @@ def lib():
                                              ctypes.POINTER(ModelStats)]
         L.exc_index_model_info.argtypes = [_vp, _vp, _u32, ctypes.POINTER(ModelHost)]
         L.exc_mlp_gradient.argtypes = [_vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float)]
+        L.exc_synth_lists_device.argtypes = [_vp, _u64, _u32, _vp, _u32, _vp, _vp, _u64]
+        L.exc_build_index_device.argtypes = [_vp, ctypes.c_int, _u32, _vp, _vp, _u32, ctypes.POINTER(_vp)]
+        L.exc_synth_index_device.argtypes = [_vp, ctypes.c_int, _u32, _u64, _u32, _vp, _u32, _u64, _vp,
+                                             ctypes.POINTER(_vp)]
+        L.exc_index_export.argtypes = [_vp, _vp, ctypes.POINTER(_vp)]
         L.exc_device_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
         L.exc_device_free.argtypes = [_vp, _vp]
         L.exc_memcpy.argtypes = [_vp, _vp, _vp, _u64, ctypes.c_int]
@@ -281,8 +287,16 @@ def queries_to_stream(queries):
 # --------------------------------------------------------------- device side
 
 class DeviceIndex:
-    def __init__(self, ctx, handle, built=None):
+    def __init__(self, ctx, handle, built=None, lengths=None):
         self.ctx, self._h, self.built = ctx, handle, built
+        self._lengths = lengths
+
+    @property
+    def ucnt(self):
+        """List lengths, or None when unknown (an index loaded from files)."""
+        if self.built is not None:
+            return self.built.ucnt
+        return self._lengths
 
     @property
     def handle(self):
@@ -376,7 +390,7 @@ class Context:
         """Exact results of every query: (cum_counts, results)."""
         qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
         nq = len(qoff) - 1
-        ucnt = index.built.ucnt if index.built is not None else None
+        ucnt = index.ucnt
         cap = 0
         if ucnt is not None:
             for q in range(nq):
@@ -438,12 +452,45 @@ class Context:
 
     def decompress(self, index, terms):
         terms = np.ascontiguousarray(terms, np.uint32)
-        ucnt = index.built.ucnt
+        ucnt = index.ucnt
         cap = int(sum(int(ucnt[t]) for t in terms))
         out = np.zeros(max(cap, 1), np.uint32)
         n = _u64()
         _check(lib().exc_decompress(self._h, index.handle, _p(terms), len(terms), _p(out), cap, ctypes.byref(n)))
         return out[:int(n.value)]
+
+    # -- index construction on the GPU
+    def synth_lists_device(self, seed, universe, df, d_out=None, cap=0):
+        """Seeded Bernoulli(df/universe) lists on the device; returns the offsets
+        (and writes the docIDs to device address d_out when given)."""
+        df = np.ascontiguousarray(df, np.uint32)
+        off = np.zeros(len(df) + 1, np.uint64)
+        _check(lib().exc_synth_lists_device(self._h, seed, universe, _p(df), len(df), _p(off),
+                                            ctypes.c_void_p(d_out) if d_out else None, cap))
+        return off
+
+    def build_index_device(self, d_docs, list_off, seg=128):
+        off = np.ascontiguousarray(list_off, np.uint64)
+        h = _vp()
+        _check(lib().exc_build_index_device(self._h, SEGLRC, seg, ctypes.c_void_p(d_docs), _p(off), len(off) - 1,
+                                            ctypes.byref(h)))
+        return DeviceIndex(self, h, lengths=np.diff(off).astype(np.uint32))
+
+    def synth_index_device(self, seed, universe, df, seg=128, batch_postings=1 << 30):
+        """Synthesize and encode a SegLRC index on the device in batches;
+        returns (DeviceIndex, list offsets)."""
+        df = np.ascontiguousarray(df, np.uint32)
+        off = np.zeros(len(df) + 1, np.uint64)
+        h = _vp()
+        _check(lib().exc_synth_index_device(self._h, SEGLRC, seg, seed, universe, _p(df), len(df), batch_postings,
+                                            _p(off), ctypes.byref(h)))
+        return DeviceIndex(self, h, lengths=np.diff(off).astype(np.uint32)), off
+
+    def export(self, index):
+        """The index's arrays in the reference layout on the host (SegLRC)."""
+        h = _vp()
+        _check(lib().exc_index_export(self._h, index.handle, ctypes.byref(h)))
+        return BuiltIndex(h)
 
     # -- device buffers owned by the library
     def device_alloc(self, nbytes):
@@ -499,7 +546,7 @@ class Server:
 
     def wait(self, ticket, cap=None):
         if cap is None:
-            cap = int(self.index.built.ucnt.max()) if self.index.built is not None else 1 << 26
+            cap = int(self.index.ucnt.max()) if self.index.ucnt is not None else 1 << 26
         out = np.zeros(max(cap, 1), np.uint32)
         n, lat = _u64(), ctypes.c_double()
         _check(lib().exc_server_wait(self._h, ticket, _p(out), cap, ctypes.byref(n), ctypes.byref(lat)))

@@ -28,6 +28,7 @@
 #include "../../include/exclucene_amd.h"
 #include "exc_format.h"
 #include "exc_internal.h"
+#include "exc_encode.h"
 
 using namespace exc;
 
@@ -133,39 +134,12 @@ exc_lrc_info lrc_fit(const uint32_t* L, uint32_t n)
     return r;
 }
 
-exc_seg_info seg_fit(const uint32_t* L, uint32_t n)
-{
-    exc_seg_info r{1.0f, 0.0f, 0, 0};
-    if (n == 0) return r;
-    if (n == 1) { r.fIntercept = (float)L[0]; return r; }
-    double xa = 0, ya = 0;
-    for (uint32_t i = 0; i < n; ++i) { xa += i; ya += L[i]; }
-    xa /= n; ya /= n;
-    double sx = 0, sxy = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        double dx = i - xa;
-        sx += dx * dx;
-        sxy += dx * ((double)L[i] - ya);
-    }
-    r.fSlope = (float)(sxy / sx);
-    r.fIntercept = (float)(ya - (double)r.fSlope * xa);
-    float rl = 0, rr = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        float d = ((float)L[i] - r.fIntercept) / r.fSlope - (float)i;
-        if (d > rl) rl = d;
-        else if (d < rr) rr = d;
-    }
-    r.nRangeLeft = (uint32_t)rl + 1;
-    r.nRangeRight = (uint32_t)(0 - rr) + 1;
-    return r;
-}
-
 // residuals against a segment fit, shifted by their minimum; returns the base
 int32_t seg_residuals(const uint32_t* L, uint32_t n, const exc_seg_info& f, uint32_t* vd)
 {
     int32_t mn = 0;
     for (uint32_t j = 0; j < n; ++j) {
-        int32_t d = (int32_t)(L[j] - (uint32_t)seg_y(j, f.fSlope, f.fIntercept));
+        int32_t d = seg_residual(L, j, f);
         vd[j] = (uint32_t)d;
         if (j == 0 || d < mn) mn = d;
     }
@@ -275,6 +249,42 @@ void parallel_for(uint32_t n, int nthreads, F f)
 }
 
 }  // namespace
+
+// A SegLRC index in the reference's arrays from device-resident records
+// (exc_index_export of an index built on the GPU).
+int exc::built_from_seglrc(uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
+                      const uint32_t* head, const uint32_t* words, uint64_t nwords, exc_built** out)
+{
+    auto* b = new exc_built();
+    b->codec = EXC_SEGLRC;
+    b->seg = seg;
+    b->ind1.resize(nterms);
+    b->seg_num.resize(nterms);
+    b->list_info.resize(nterms);
+    for (uint32_t t = 0; t < nterms; ++t) {
+        const TermRec& R = terms[t];
+        b->ind1[t].m_urlcount = R.n;
+        b->ind1[t].m_off = R.woff * 4;
+        b->seg_num[t] = R.segnum;
+        exc_seg_info& li = b->list_info[t];
+        li.fSlope = R.slope;
+        std::memcpy(&li.fIntercept, &R.icpt, 4);
+        li.nRangeLeft = (uint32_t)R.range_l;
+        li.nRangeRight = (uint32_t)R.range_r;
+    }
+    b->ind2.assign(words, words + nwords);
+    b->seg_offset.resize(nseg);
+    b->seg_head.assign(head, head + nseg);
+    b->seg_info.resize(nseg);
+    b->base.resize(nseg);
+    for (uint64_t g = 0; g < nseg; ++g) {
+        b->seg_offset[g] = rec[g].off;
+        b->seg_info[g] = exc_seg_info{rec[g].slope, rec[g].icpt, rec[g].rl, rec[g].rr};
+        b->base[g] = rec[g].base;
+    }
+    *out = b;
+    return EXC_OK;
+}
 
 extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
                                const uint32_t* docs, const uint64_t* list_off, uint32_t nterms,

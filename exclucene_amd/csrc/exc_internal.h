@@ -156,6 +156,51 @@ constexpr uint32_t kDecodeUnit = 1024;      // docIDs per wave of whole-list dec
 hipError_t launch_reset(uint32_t* counter, hipStream_t s);
 hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s);
 hipError_t launch_unit_prologue(const ProbeArgs& a, hipStream_t s);
+
+// ---- device index builder (gpubuild.hip)
+// Synthetic lists: item i = chunk item_chunk[i] of term item_term[i]'s docID
+// range (nchunks[t] equal spans of [0, universe)); docIDs of item i start at
+// item_off[i] of the virtual concatenation of all lists, docs[] holds the
+// part from doc_base on.
+struct SynthArgs {
+    uint64_t seed;
+    uint32_t universe;
+    uint32_t pad_;
+    const uint32_t* df;
+    const uint32_t* nchunks;
+    const uint32_t* item_term;
+    const uint32_t* item_chunk;
+    const uint64_t* item_off;
+    uint32_t* counts;
+    uint32_t* docs;
+    uint64_t doc_base;
+    uint64_t item0, item1;
+};
+// SegLRC encoding of segments [g0, g1) and lists [t0, t1) whose docIDs are in
+// docs[] from doc_base on (list t at list_off[t] of the concatenation).
+struct EncodeArgs {
+    const uint32_t* docs;
+    const uint64_t* list_off;  // nterms + 1
+    uint64_t doc_base;
+    const uint32_t* seg_num;   // nterms + 1
+    uint32_t nterms;
+    uint32_t seg;
+    uint64_t g0, g1;
+    uint32_t t0, t1;
+    SegRec* rec;
+    uint32_t* head;
+    uint32_t* nwords;          // per segment: descriptor + packed words
+    const uint64_t* goff;      // per segment (+1): global word offset
+    uint32_t* words;
+    TermRec* terms;
+};
+hipError_t launch_synth(const SynthArgs& a, bool write, hipStream_t s);
+hipError_t launch_seg_fit(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_seg_pack(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
+hipError_t scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
+int built_from_seglrc(uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
+                      const uint32_t* head, const uint32_t* words, uint64_t nwords, exc_built** out);
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
 hipError_t launch_decompress(const DevIndex& ix, const uint32_t* terms, const uint64_t* out_off,
                              const uint32_t* unit_term, const uint32_t* unit_k, uint32_t nunits,

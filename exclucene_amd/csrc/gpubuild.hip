@@ -1,0 +1,378 @@
+// gpubuild.hip -- index construction on the GPU: seeded synthetic posting
+// lists and the SegLRC encoder, so that an index of tens of GB is generated
+// and encoded where it is served instead of crossing PCIe.
+//
+// The encoder follows the host builder block for block: the per-segment fit,
+// base and bit width come from exc_encode.h (the same code, the same IEEE
+// operation order: both sides build with -ffp-contract=off), the ParaPFor
+// block is a descriptor word (lb << 26, no exceptions: SegLRC uses FRAC 0) and
+// the residuals packed LSB first (GI/ParaPFor.cpp:56-126).  Segment-level bytes
+// are therefore identical to the host builder's; only the list-level fit of
+// .regression_info_list sums in a different order (a block reduction).
+#include "exc_encode.h"
+#include "exc_format.h"
+#include "exc_internal.h"
+
+namespace exc {
+namespace {
+
+// ---------------------------------------------------------------- synthesis
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// One lane's share [lo, hi) of an item's docID span.  Dense lists (p >= 1/32)
+// keep each docID whose hash falls under p * 2^32; sparse lists draw geometric
+// gaps (a Bernoulli(p) process, so disjoint spans are independent).
+template <bool WRITE>
+__device__ uint32_t synth_lane(const SynthArgs& a, uint32_t t, uint32_t c, uint32_t ln, uint64_t lo, uint64_t hi,
+                               uint32_t* out)
+{
+    const double p = (double)a.df[t] / (double)a.universe;
+    uint32_t n = 0;
+    if (p >= 1.0 / 32.0) {
+        const uint64_t th = p >= 1.0 ? 0x100000000ull : (uint64_t)(p * 4294967296.0);
+        const uint64_t key = a.seed ^ ((uint64_t)t * 0x9E3779B97F4A7C15ull);
+        for (uint64_t d = lo; d < hi; ++d) {
+            if ((mix64(key + d * 0xD1B54A32D192ED03ull) >> 32) < th) {
+                if (WRITE) out[n] = (uint32_t)d;
+                ++n;
+            }
+        }
+    } else if (p > 0.0) {
+        uint64_t s = mix64(a.seed ^ ((uint64_t)t * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)c * 0xC2B2AE3D27D4EB4Full) ^
+                           ((uint64_t)ln * 0x165667B19E3779F9ull));
+        const double lq = log1p(-p);
+        uint64_t d = lo;
+        for (;;) {
+            s += 0x9E3779B97F4A7C15ull;
+            const double u = (double)((mix64(s) >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+            const double g = floor(log(u) / lq);
+            if (g >= (double)(hi - d)) break;
+            d += (uint64_t)g;
+            if (WRITE) out[n] = (uint32_t)d;
+            ++n;
+            ++d;
+            if (d >= hi) break;
+        }
+    }
+    return n;
+}
+
+// One wave per item (term t, chunk c of its docID range).  Counting pass:
+// counts[i] = docIDs of the item.  Writing pass: each lane counts its share,
+// a wave scan places it, and the lane writes its docIDs in order.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_synth(SynthArgs a)
+{
+    const uint32_t wid = threadIdx.x >> 6, ln = threadIdx.x & 63u;
+    const uint64_t i = (uint64_t)blockIdx.x * 4 + wid + a.item0;
+    if (i >= a.item1) return;
+    const uint32_t t = a.item_term[i], c = a.item_chunk[i], nc = a.nchunks[t];
+    const uint64_t lo = (uint64_t)c * a.universe / nc, hi = (uint64_t)(c + 1) * a.universe / nc;
+    const uint64_t span = hi - lo;
+    const uint64_t llo = lo + span * ln / 64, lhi = lo + span * (ln + 1) / 64;
+    const uint32_t cnt = synth_lane<false>(a, t, c, ln, llo, lhi, nullptr);
+    // inclusive wave scan of the lane counts
+    uint32_t incl = cnt;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d);
+        if (ln >= d) incl += y;
+    }
+    if (!WRITE) {
+        if (ln == 63) a.counts[i] = incl;
+        return;
+    }
+    uint32_t* out = a.docs + (a.item_off[i] - a.doc_base) + (incl - cnt);
+    synth_lane<true>(a, t, c, ln, llo, lhi, out);
+}
+
+// ---------------------------------------------------------------- encoder
+
+__device__ __forceinline__ uint32_t term_of_segment(const uint32_t* seg_num, uint32_t nterms, uint32_t g)
+{
+    // last t with seg_num[t] <= g (terms without segments share the next one's start)
+    uint32_t lo = 0, hi = nterms;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (seg_num[m] <= g) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+// Thread per segment: fit, base, bit width, head; the block's word count.
+__global__ __launch_bounds__(256) void k_seg_fit(EncodeArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x + a.g0;
+    if (g64 >= a.g1) return;
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    const uint32_t k = g - a.seg_num[t];
+    const uint64_t nt = a.list_off[t + 1] - a.list_off[t];
+    const uint32_t st = k * a.seg, n = (uint32_t)min((uint64_t)a.seg, nt - st);
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base) + st;
+    const exc_seg_info f = seg_fit(L, n);
+    int32_t base;
+    uint32_t lb;
+    seg_base_bits(L, n, f, base, lb);
+    SegRec q;
+    q.off = 0;
+    q.desc = lb << 26;
+    q.slope = f.fSlope;
+    q.icpt = f.fIntercept;
+    q.rl = f.nRangeLeft;
+    q.rr = f.nRangeRight;
+    q.base = base;
+    q.head = L[0];
+    if (!(q.slope > 0.0f)) {
+        q.slope = 1.0f;
+        q.rl = q.rr = 0x7fffffffu;
+    }
+    a.rec[g] = q;
+    a.head[g] = L[0];
+    a.nwords[g] = 1 + words_for(n, lb);
+}
+
+// Thread per segment: descriptor and packed residuals at the block's global
+// word offset; the record's offset relative to its list.
+__global__ __launch_bounds__(256) void k_seg_pack(EncodeArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x + a.g0;
+    if (g64 >= a.g1) return;
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    const uint32_t k = g - a.seg_num[t];
+    const uint64_t nt = a.list_off[t + 1] - a.list_off[t];
+    const uint32_t st = k * a.seg, n = (uint32_t)min((uint64_t)a.seg, nt - st);
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base) + st;
+    SegRec q = a.rec[g];
+    const uint64_t go = a.goff[g];
+    q.off = (uint32_t)(go - a.goff[a.seg_num[t]]);
+    a.rec[g].off = q.off;
+    const uint32_t lb = desc_lb(q.desc);
+    exc_seg_info f{q.slope, q.icpt, 0, 0};
+    uint32_t* o = a.words + go;
+    o[0] = q.desc;
+    ++o;
+    uint64_t acc = 0;
+    uint32_t nb = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t v = (uint32_t)(seg_residual(L, j, f) - q.base);
+        acc |= (uint64_t)v << nb;
+        nb += lb;
+        if (nb >= 32) {
+            *o++ = (uint32_t)acc;
+            acc >>= 32;
+            nb -= 32;
+        }
+    }
+    if (nb) *o = (uint32_t)acc;
+}
+
+// Block per list: the list-level fit of .regression_info_list (double sums as
+// a block reduction, float slope and intercept, ranges = max deviations + 1).
+__global__ __launch_bounds__(256) void k_list_fit(EncodeArgs a)
+{
+    __shared__ double r0[256], r1[256];
+    __shared__ float m0[256], m1[256];
+    const uint32_t t = a.t0 + blockIdx.x;
+    if (t >= a.t1) return;
+    TermRec& R = a.terms[t];
+    const uint64_t n = a.list_off[t + 1] - a.list_off[t];
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base);
+    const uint32_t tid = threadIdx.x;
+    if (n < 2) {
+        if (tid == 0) {
+            R.slope = 1.0f;
+            R.range_l = R.range_r = 4.0e9f;
+            const float ic = n ? (float)L[0] : 0.0f;
+            R.icpt = __float_as_int(ic);
+        }
+        return;
+    }
+    double s = 0;
+    for (uint64_t i = tid; i < n; i += 256) s += (double)L[i];
+    r0[tid] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w; w >>= 1) {
+        if (tid < w) r0[tid] += r0[tid + w];
+        __syncthreads();
+    }
+    const double ya = r0[0] / (double)n, xa = (double)(n - 1) * 0.5;
+    __syncthreads();
+    double sx = 0, sxy = 0;
+    for (uint64_t i = tid; i < n; i += 256) {
+        const double dx = (double)i - xa;
+        sx += dx * dx;
+        sxy += dx * ((double)L[i] - ya);
+    }
+    r0[tid] = sx;
+    r1[tid] = sxy;
+    __syncthreads();
+    for (uint32_t w = 128; w; w >>= 1) {
+        if (tid < w) {
+            r0[tid] += r0[tid + w];
+            r1[tid] += r1[tid + w];
+        }
+        __syncthreads();
+    }
+    const float slope = (float)(r1[0] / r0[0]);
+    const float icpt = (float)(ya - (double)slope * xa);
+    float rl = 0, rr = 0;
+    for (uint64_t i = tid; i < n; i += 256) {
+        const float d = ((float)L[i] - icpt) / slope - (float)i;
+        rl = fmaxf(rl, d);
+        rr = fminf(rr, d);
+    }
+    m0[tid] = rl;
+    m1[tid] = rr;
+    __syncthreads();
+    for (uint32_t w = 128; w; w >>= 1) {
+        if (tid < w) {
+            m0[tid] = fmaxf(m0[tid], m0[tid + w]);
+            m1[tid] = fminf(m1[tid], m1[tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (slope > 0.0f) {
+            R.slope = slope;
+            R.icpt = __float_as_int(icpt);
+            R.range_l = (float)((uint32_t)m0[0] + 1u);
+            R.range_r = (float)((uint32_t)(0.0f - m1[0]) + 1u);
+        } else {
+            R.slope = 1.0f;
+            R.range_l = R.range_r = 4.0e9f;
+        }
+    }
+}
+
+// Thread per term: the record fields that do not depend on the docIDs.
+__global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.nterms) return;
+    TermRec R;
+    R.woff = a.goff[a.seg_num[t]];
+    R.n = (uint32_t)(a.list_off[t + 1] - a.list_off[t]);
+    R.segnum = a.seg_num[t];
+    R.nseg = a.seg_num[t + 1] - a.seg_num[t];
+    R.aux = 0;
+    R.slope = 1.0f;
+    R.icpt = 0;
+    R.range_l = R.range_r = 4.0e9f;
+    R.model = kNoModel;
+    R.term = t;
+    R.merr = R.mx0 = R.mxs = 0.0f;
+    R.mkind = 0;
+    a.terms[t] = R;
+}
+
+// ---------------------------------------------------------------- scan (u64)
+
+constexpr uint32_t kScanItems = 4096;  // 256 threads x 16
+
+template <class T>
+__global__ __launch_bounds__(256) void k_block_scan(const T* in, uint64_t* out, uint64_t* bsum, uint64_t n)
+{
+    __shared__ uint64_t ws[4];
+    const uint32_t tid = threadIdx.x, ln = tid & 63u, wid = tid >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanItems + (uint64_t)tid * 16;
+    uint64_t x[16], tot = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        x[k] = base + k < n ? (uint64_t)in[base + k] : 0ull;
+        tot += x[k];
+    }
+    uint64_t incl = tot;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(incl, d);
+        if (ln >= d) incl += y;
+    }
+    if (ln == 63) ws[wid] = incl;
+    __syncthreads();
+    uint64_t woff = 0;
+    for (uint32_t w = 0; w < wid; ++w) woff += ws[w];
+    uint64_t run = woff + incl - tot;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += x[k];
+    }
+    if (tid == 255) bsum[blockIdx.x] = woff + incl;
+}
+
+__global__ __launch_bounds__(256) void k_add_offsets(uint64_t* out, const uint64_t* boff, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] += boff[i / kScanItems];
+}
+
+template <class T>
+hipError_t scan_rec(const T* in, uint64_t* out, uint64_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    const uint64_t nb = (n + kScanItems - 1) / kScanItems;
+    uint64_t* bsum = nullptr;
+    uint64_t* boff = nullptr;
+    hipError_t e = hipMallocAsync((void**)&bsum, nb * 8, s);
+    if (e == hipSuccess) e = hipMallocAsync((void**)&boff, nb * 8, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_block_scan<T>, dim3((uint32_t)nb), dim3(256), 0, s, in, out, bsum, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && nb > 1) e = scan_rec<uint64_t>(bsum, boff, nb, s);
+    if (e == hipSuccess && nb > 1) {
+        hipLaunchKernelGGL(k_add_offsets, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, out, boff, n);
+        e = hipGetLastError();
+    }
+    if (bsum) (void)hipFreeAsync(bsum, s);
+    if (boff) (void)hipFreeAsync(boff, s);
+    return e;
+}
+
+}  // namespace
+
+hipError_t launch_synth(const SynthArgs& a, bool write, hipStream_t s)
+{
+    if (a.item1 <= a.item0) return hipSuccess;
+    const uint64_t grid = (a.item1 - a.item0 + 3) / 4;
+    if (write)
+        hipLaunchKernelGGL(k_synth<true>, dim3((uint32_t)grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_synth<false>, dim3((uint32_t)grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_fit(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.g1 <= a.g0) return hipSuccess;
+    hipLaunchKernelGGL(k_seg_fit, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_pack(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.g1 > a.g0)
+        hipLaunchKernelGGL(k_seg_pack, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
+    if (a.t1 > a.t0) hipLaunchKernelGGL(k_list_fit, dim3(a.t1 - a.t0), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s)
+{
+    if (!a.nterms) return hipSuccess;
+    hipLaunchKernelGGL(k_term_recs, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s)
+{
+    return scan_rec<uint32_t>(in, out, n, s);
+}
+
+}  // namespace exc

@@ -104,6 +104,7 @@ struct exc_index {
     uint32_t seg = 256;
     uint32_t nterms = 0;
     uint64_t nseg = 0;
+    uint64_t nwords = 0;           // compressed words (.ind2)
     std::vector<uint32_t> n;       // list lengths (host copy for planning)
     std::vector<TermRec> recs;     // host copy of the device term records
     TermRec* d_terms = nullptr;
@@ -423,6 +424,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->seg = hs ? 128 : seg;
     ix->nterms = nt;
     ix->nseg = h->nseg;
+    ix->nwords = W;
     ix->n.resize(nt);
     for (uint32_t t = 0; t < nt; ++t) ix->n[t] = recs[t].n;
     HIP_TRY(hipSetDevice(c->device));
@@ -1160,6 +1162,313 @@ extern "C" int exc_mlp_gradient(exc_ctx* c, const float* params, const uint32_t*
     std::memcpy(grad, g, kMlpParams * 4);
     if (loss) *loss = g[kMlpParams];
     return EXC_OK;
+}
+
+// ---------------------------------------------------------------- device index builder
+
+namespace {
+
+// Device buffers that live only during a build.
+struct TempBufs {
+    exc_ctx* c;
+    std::vector<DevBuf> bufs;
+    explicit TempBufs(exc_ctx* ctx) : c(ctx) {}
+    ~TempBufs()
+    {
+        (void)hipStreamSynchronize(c->stream);
+        for (auto& b : bufs) dev_free(c, b);
+    }
+    template <class T>
+    int get(size_t count, T** p)
+    {
+        DevBuf b;
+        if (int rc = dev_alloc(c, b, std::max<size_t>(1, count) * sizeof(T))) return rc;
+        bufs.push_back(b);
+        *p = static_cast<T*>(b.p);
+        return EXC_OK;
+    }
+};
+
+// An index array of `bytes` followed by kPadWords zero words; nothing copied in.
+int alloc_padded(exc_index* ix, size_t bytes, void** out)
+{
+    DevBuf b;
+    if (int rc = dev_alloc(ix->ctx, b, bytes + kPadWords * 4)) return rc;
+    ix->bufs.push_back(b);
+    ix->bytes += b.bytes;
+    HIP_TRY(hipMemsetAsync(static_cast<char*>(b.p) + bytes, 0, kPadWords * 4, ix->ctx->stream));
+    *out = b.p;
+    return EXC_OK;
+}
+
+// Synthetic lists: items of ~4,096 expected docIDs per term, counted on the
+// device; fills the host item table and the lists' offsets.
+struct SynthPlan {
+    std::vector<uint32_t> nchunks, item_term, item_chunk;
+    std::vector<uint64_t> item_off, list_off;
+    uint32_t *d_df = nullptr, *d_nch = nullptr, *d_it = nullptr, *d_ic = nullptr;
+    uint64_t* d_ioff = nullptr;
+};
+
+int synth_plan(exc_ctx* c, TempBufs& tmp, uint64_t seed, uint32_t U, const uint32_t* df, uint32_t nterms,
+               SynthPlan& P, SynthArgs& a)
+{
+    if (U == 0 || U > 0x80000000u) return set_error(EXC_ERR_ARG, "universe must be in [1, 2^31]");
+    P.nchunks.resize(nterms);
+    for (uint32_t t = 0; t < nterms; ++t) {
+        if (df[t] > U) return set_error(EXC_ERR_ARG, "term %u: df %u exceeds the universe", t, df[t]);
+        const uint32_t nc = std::max<uint32_t>(1, (df[t] + 4095) / 4096);
+        P.nchunks[t] = nc;
+        for (uint32_t k = 0; k < nc; ++k) {
+            P.item_term.push_back(t);
+            P.item_chunk.push_back(k);
+        }
+    }
+    const uint64_t ni = P.item_term.size();
+    uint32_t* d_cnt;
+    int rc;
+    if ((rc = tmp.get(nterms, &P.d_df)) || (rc = tmp.get(nterms, &P.d_nch)) || (rc = tmp.get(ni, &P.d_it)) ||
+        (rc = tmp.get(ni, &P.d_ic)) || (rc = tmp.get(ni + 1, &P.d_ioff)) || (rc = tmp.get(ni, &d_cnt)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(P.d_df, df, nterms * 4ull, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(P.d_nch, P.nchunks.data(), nterms * 4ull, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(P.d_it, P.item_term.data(), ni * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(P.d_ic, P.item_chunk.data(), ni * 4, hipMemcpyHostToDevice, c->stream));
+    std::memset(&a, 0, sizeof a);
+    a.seed = seed;
+    a.universe = U;
+    a.df = P.d_df;
+    a.nchunks = P.d_nch;
+    a.item_term = P.d_it;
+    a.item_chunk = P.d_ic;
+    a.item_off = P.d_ioff;
+    a.counts = d_cnt;
+    a.item0 = 0;
+    a.item1 = ni;
+    HIP_TRY(launch_synth(a, false, c->stream));
+    std::vector<uint32_t> cnt(ni);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt, ni * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    P.item_off.assign(ni + 1, 0);
+    for (uint64_t i = 0; i < ni; ++i) P.item_off[i + 1] = P.item_off[i] + cnt[i];
+    P.list_off.assign(nterms + 1, 0);
+    uint64_t i = 0;
+    for (uint32_t t = 0; t < nterms; ++t) {
+        P.list_off[t] = P.item_off[i];
+        i += P.nchunks[t];
+        if (P.item_off[i] - P.list_off[t] > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "term %u: list too long", t);
+    }
+    P.list_off[nterms] = P.item_off[ni];
+    HIP_TRY(hipMemcpyAsync(P.d_ioff, P.item_off.data(), (ni + 1) * 8, hipMemcpyHostToDevice, c->stream));
+    return EXC_OK;
+}
+
+// Write the docIDs of terms [t0, t1) into docs (docs[0] = list t0's first).
+int synth_write(exc_ctx* c, const SynthPlan& P, SynthArgs a, uint32_t t0, uint32_t t1, uint32_t* docs)
+{
+    uint64_t i0 = 0;
+    for (uint32_t t = 0; t < t0; ++t) i0 += P.nchunks[t];
+    uint64_t i1 = i0;
+    for (uint32_t t = t0; t < t1; ++t) i1 += P.nchunks[t];
+    a.docs = docs;
+    a.doc_base = P.list_off[t0];
+    a.item0 = i0;
+    a.item1 = i1;
+    HIP_TRY(launch_synth(a, true, c->stream));
+    return EXC_OK;
+}
+
+// SegLRC encoding on the device.  docs_for(t0, t1, &ptr) makes the docIDs of
+// lists [t0, t1) available in device memory (ptr[0] = list t0's first); it is
+// called twice per batch (fit, then pack).
+template <class DocsFor>
+int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::vector<uint64_t>& list_off,
+                        const std::vector<std::pair<uint32_t, uint32_t>>& batches, DocsFor docs_for, exc_index** out)
+{
+    if (seg != 128 && seg != 256) return set_error(EXC_ERR_ARG, "segment size must be 128 or 256");
+    std::vector<uint32_t> seg_num(nterms + 1, 0);
+    uint64_t S = 0;
+    for (uint32_t t = 0; t < nterms; ++t) {
+        seg_num[t] = (uint32_t)S;
+        S += (list_off[t + 1] - list_off[t] + seg - 1) / seg;
+        if (S > 0xfffffffeull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 segments");
+    }
+    seg_num[nterms] = (uint32_t)S;
+    TempBufs tmp(c);
+    auto* ix = new exc_index();
+    ix->ctx = c;
+    ix->codec = EXC_SEGLRC;
+    ix->seg = seg;
+    ix->nterms = nterms;
+    ix->nseg = S;
+    int rc = EXC_OK;
+    auto fail = [&](int r) {
+        exc_index_free(ix);
+        return r;
+    };
+    uint64_t* d_loff;
+    uint32_t *d_snum, *d_nw;
+    uint64_t* d_goff;
+    void *p_rec, *p_head, *p_terms;
+    if ((rc = tmp.get(nterms + 1, &d_loff)) || (rc = tmp.get(nterms + 1, &d_snum)) || (rc = tmp.get(S + 1, &d_nw)) ||
+        (rc = tmp.get(S + 1, &d_goff)) || (rc = alloc_padded(ix, S * sizeof(SegRec), &p_rec)) ||
+        (rc = alloc_padded(ix, S * 4, &p_head)) || (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)))
+        return fail(rc);
+    hipError_t e = hipMemcpyAsync(d_loff, list_off.data(), (nterms + 1ull) * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_snum, seg_num.data(), (nterms + 1ull) * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_nw + S, 0, 4, c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    EncodeArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.list_off = d_loff;
+    a.seg_num = d_snum;
+    a.nterms = nterms;
+    a.seg = seg;
+    a.rec = static_cast<SegRec*>(p_rec);
+    a.head = static_cast<uint32_t*>(p_head);
+    a.nwords = d_nw;
+    a.goff = d_goff;
+    a.terms = static_cast<TermRec*>(p_terms);
+    // pass 1: fits and block sizes
+    for (auto [t0, t1] : batches) {
+        const uint32_t* docs;
+        if ((rc = docs_for(t0, t1, &docs))) return fail(rc);
+        a.docs = docs;
+        a.doc_base = list_off[t0];
+        a.g0 = seg_num[t0];
+        a.g1 = seg_num[t1];
+        if ((e = launch_seg_fit(a, c->stream)) != hipSuccess) break;
+    }
+    // block offsets, words, term records
+    uint64_t W = 0;
+    if (e == hipSuccess) e = scan_u32_to_u64(d_nw, d_goff, S + 1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&W, d_goff + S, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    void* p_words;
+    if ((rc = alloc_padded(ix, W * 4, &p_words))) return fail(rc);
+    a.words = static_cast<uint32_t*>(p_words);
+    if ((e = launch_term_recs(a, c->stream)) != hipSuccess)
+        return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    // pass 2: packed blocks, list-level fits
+    for (auto [t0, t1] : batches) {
+        const uint32_t* docs;
+        if ((rc = docs_for(t0, t1, &docs))) return fail(rc);
+        a.docs = docs;
+        a.doc_base = list_off[t0];
+        a.g0 = seg_num[t0];
+        a.g1 = seg_num[t1];
+        a.t0 = t0;
+        a.t1 = t1;
+        if ((e = launch_seg_pack(a, c->stream)) != hipSuccess) break;
+    }
+    ix->recs.resize(nterms);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ix->recs.data(), p_terms, (size_t)nterms * sizeof(TermRec), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    ix->nwords = W;
+    ix->n.resize(nterms);
+    for (uint32_t t = 0; t < nterms; ++t) ix->n[t] = ix->recs[t].n;
+    ix->dix.codec = EXC_SEGLRC;
+    ix->dix.seg = seg;
+    ix->dix.words = a.words;
+    ix->dix.terms = a.terms;
+    ix->d_terms = a.terms;
+    ix->dix.rec = a.rec;
+    ix->dix.seg_head = a.head;
+    *out = ix;
+    return EXC_OK;
+}
+
+}  // namespace
+
+extern "C" int exc_synth_lists_device(exc_ctx* c, uint64_t seed, uint32_t universe, const uint32_t* df, uint32_t nterms,
+                                      uint64_t* list_off, uint32_t* d_docs, uint64_t cap)
+{
+    if (!c || (nterms && !df) || !list_off) return set_error(EXC_ERR_ARG, "exc_synth_lists_device: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    TempBufs tmp(c);
+    SynthPlan P;
+    SynthArgs a;
+    if (int rc = synth_plan(c, tmp, seed, universe, df, nterms, P, a)) return rc;
+    std::memcpy(list_off, P.list_off.data(), (nterms + 1ull) * 8);
+    if (!d_docs) return EXC_OK;
+    if (P.list_off[nterms] > cap)
+        return set_error(EXC_ERR_CAPACITY, "output holds %llu docIDs, need %llu", (unsigned long long)cap,
+                         (unsigned long long)P.list_off[nterms]);
+    if (int rc = synth_write(c, P, a, 0, nterms, d_docs)) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return EXC_OK;
+}
+
+extern "C" int exc_build_index_device(exc_ctx* c, int codec, uint32_t seg, const uint32_t* d_docs,
+                                      const uint64_t* list_off, uint32_t nterms, exc_index** out)
+{
+    if (!c || !list_off || !out || (list_off[nterms] && !d_docs))
+        return set_error(EXC_ERR_ARG, "exc_build_index_device: bad argument");
+    if (codec != EXC_SEGLRC) return set_error(EXC_ERR_ARG, "exc_build_index_device: SegLRC only");
+    for (uint32_t t = 0; t < nterms; ++t)
+        if (list_off[t + 1] < list_off[t] || list_off[t + 1] - list_off[t] > 0xffffffffull)
+            return set_error(EXC_ERR_ARG, "exc_build_index_device: bad list offsets");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<uint64_t> lo(list_off, list_off + nterms + 1);
+    std::vector<std::pair<uint32_t, uint32_t>> batches{{0u, nterms}};
+    return build_seglrc_device(c, seg, nterms, lo, batches,
+                               [&](uint32_t t0, uint32_t, const uint32_t** p) {
+                                   *p = d_docs + lo[t0];
+                                   return EXC_OK;
+                               },
+                               out);
+}
+
+extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, uint64_t seed, uint32_t universe,
+                                      const uint32_t* df, uint32_t nterms, uint64_t batch_postings,
+                                      uint64_t* list_off_out, exc_index** out)
+{
+    if (!c || (nterms && !df) || !out) return set_error(EXC_ERR_ARG, "exc_synth_index_device: bad argument");
+    if (codec != EXC_SEGLRC) return set_error(EXC_ERR_ARG, "exc_synth_index_device: SegLRC only");
+    HIP_TRY(hipSetDevice(c->device));
+    TempBufs tmp(c);
+    SynthPlan P;
+    SynthArgs a;
+    if (int rc = synth_plan(c, tmp, seed, universe, df, nterms, P, a)) return rc;
+    if (list_off_out) std::memcpy(list_off_out, P.list_off.data(), (nterms + 1ull) * 8);
+    if (batch_postings == 0) batch_postings = 1ull << 30;
+    std::vector<std::pair<uint32_t, uint32_t>> batches;
+    uint64_t maxb = 1;
+    for (uint32_t t0 = 0; t0 < nterms;) {
+        uint32_t t1 = t0 + 1;
+        while (t1 < nterms && P.list_off[t1 + 1] - P.list_off[t0] <= batch_postings) ++t1;
+        batches.emplace_back(t0, t1);
+        maxb = std::max<uint64_t>(maxb, P.list_off[t1] - P.list_off[t0]);
+        t0 = t1;
+    }
+    uint32_t* d_docs;
+    if (int rc = tmp.get(maxb, &d_docs)) return rc;
+    return build_seglrc_device(c, seg, nterms, P.list_off, batches,
+                               [&](uint32_t t0, uint32_t t1, const uint32_t** p) {
+                                   *p = d_docs;
+                                   return synth_write(c, P, a, t0, t1, d_docs);
+                               },
+                               out);
+}
+
+extern "C" int exc_index_export(exc_ctx* c, exc_index* ix, exc_built** out)
+{
+    if (!c || !ix || !out) return set_error(EXC_ERR_ARG, "exc_index_export: bad argument");
+    if (ix->codec != EXC_SEGLRC) return set_error(EXC_ERR_ARG, "exc_index_export: SegLRC only");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<SegRec> rec(ix->nseg);
+    std::vector<uint32_t> head(ix->nseg), words(ix->nwords);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (ix->nseg) {
+        HIP_TRY(hipMemcpy(rec.data(), ix->dix.rec, ix->nseg * sizeof(SegRec), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(head.data(), ix->dix.seg_head, ix->nseg * 4, hipMemcpyDeviceToHost));
+    }
+    if (ix->nwords) HIP_TRY(hipMemcpy(words.data(), ix->dix.words, ix->nwords * 4, hipMemcpyDeviceToHost));
+    return built_from_seglrc(ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(), head.data(), words.data(),
+                             ix->nwords, out);
 }
 
 // ---------------------------------------------------------------- autotuner
