@@ -301,6 +301,54 @@ def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
     return out
 
 
+def c5_leg(ea, W, ctx, qps=20000.0, seconds=3.0, check=200):
+    """C5: a SegLRC index of 1M terms over 2^26 docIDs with df_k = min(2^25,
+    floor(2^33 / k^1.1)), synthesized and encoded on the GPU and kept resident;
+    2-6-term queries with terms drawn proportionally to df arrive as a Poisson
+    process at `qps` and go through the continuous-batching server (4 slots,
+    16 GiB admission budget).  A sample of queries is checked alone against the
+    mix."""
+    U = 1 << 26
+    df = W.zipf_df(1_000_000, universe=U, cap=1 << 25, s=1.1, scale=float(1 << 33))
+    t0 = time.perf_counter()
+    dix, off = ctx.synth_index_device(5, U, df, seg=128, batch_postings=1 << 30)
+    build_s = time.perf_counter() - t0
+    lengths = np.diff(off)
+    log("C5 index: %d postings, %.1f GB resident, built on the GPU in %.1f s"
+        % (int(off[-1]), dix.device_bytes() / 1e9, build_s))
+    nq = int(qps * seconds)
+    queries = W.df_weighted_queries_cdf(lengths, nq, 2, 6, 55)
+    rng = np.random.default_rng(55)
+    arrival = np.cumsum(rng.exponential(1.0 / qps, nq))
+    tune = ctx.autotune(dix, W.df_weighted_queries_cdf(lengths, 2000, 2, 6, 56), reps=2)
+    out = {"postings": int(off[-1]), "terms": len(df), "universe": U, "resident_bytes": dix.device_bytes(),
+           "build_s_on_gpu": build_s, "offered_qps": qps, "queries": nq, "tune": tune}
+    pick = np.random.default_rng(57).choice(nq, check, replace=False)
+    for mode, counts_only in (("counts", True), ("docids_to_host", False)):
+        srv = ea.Server(ctx, dix, nslots=4, max_batch=512, mem_budget=16 << 30, counts_only=counts_only)
+        try:
+            t0 = time.perf_counter()
+            lat, cnt, _ = srv.replay(queries, arrival + 0.05, 0)
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        finally:
+            srv.close()
+        row = {"achieved_qps": nq / wall, "p50_ms": float(np.percentile(lat, 50)),
+               "p99_ms": float(np.percentile(lat, 99)), "mean_results": float(cnt.mean()),
+               "batches": st["batches"], "overlapped_launches": st["overlapped_launches"]}
+        if counts_only:
+            same = True
+            for i in pick:
+                c, _ = ctx.intersect(dix, [queries[i]])
+                same = same and int(c[0]) == int(cnt[i])
+            row["alone_vs_mix_checked"] = int(check)
+            row["alone_vs_mix_equal"] = bool(same)
+        out[mode] = row
+        log("C5 %s: %s" % (mode, row))
+    dix.close()
+    return out
+
+
 def copy_calibration(mib):
     """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings of the probe kernel."""
@@ -327,6 +375,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the hash-bucket leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 1M-term, 35G-posting serving configuration")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-window leg")
     ap.add_argument("--models", default="pla", choices=["pla", "mlp", "none"],
                     help="learned position models for lists longer than 65,536 before tuning")
@@ -473,6 +522,10 @@ def main():
     if not args.no_c4:
         extra["learned"] = learned_leg(ea, W, ctx, dix, df, tune)
         log("learned: %s" % extra["learned"])
+
+    if not args.no_c5:
+        extra["serving_c5"] = c5_leg(ea, W, ctx)
+        log("C5: %s" % extra["serving_c5"])
 
     cpu = None
     if not args.no_cpu:

@@ -36,6 +36,7 @@ EXPORTS = [
     "exc_synth_lists_device", "exc_build_index_device", "exc_synth_index_device", "exc_index_export",
 ]
 MODEL_PLA, MODEL_MLP = 1, 2
+SERVER_COUNTS = 1
 
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -159,7 +160,7 @@ def lib():
         L.exc_autotune.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(TuneTable)]
         L.exc_ctx_set_tuning.argtypes = [_vp, ctypes.POINTER(TuneTable)]
         L.exc_server_destroy.restype = None
-        L.exc_server_create.argtypes = [_vp, _vp, _u32, _u32, _u64, ctypes.POINTER(_vp)]
+        L.exc_server_create.argtypes = [_vp, _vp, _u32, _u32, _u64, _u32, ctypes.POINTER(_vp)]
         L.exc_server_submit.argtypes = [_vp, _vp, _u32, ctypes.POINTER(_u64)]
         L.exc_server_wait.argtypes = [_vp, _u64, _vp, _u64, ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_double)]
         L.exc_server_replay.argtypes = [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u64]
@@ -532,10 +533,12 @@ class Server:
     while the device memory of the in-flight batches stays within `mem_budget`
     bytes (0 = unlimited).  The slots take the search options of `ctx`."""
 
-    def __init__(self, ctx, index, nslots=2, max_batch=1024, mem_budget=0):
+    def __init__(self, ctx, index, nslots=2, max_batch=1024, mem_budget=0, counts_only=False):
         self.index = index
+        self.counts_only = counts_only
         h = _vp()
-        _check(lib().exc_server_create(ctx.handle, index.handle, nslots, max_batch, mem_budget, ctypes.byref(h)))
+        _check(lib().exc_server_create(ctx.handle, index.handle, nslots, max_batch, mem_budget,
+                                       SERVER_COUNTS if counts_only else 0, ctypes.byref(h)))
         self._h = h
 
     def submit(self, terms):
@@ -550,7 +553,8 @@ class Server:
         out = np.zeros(max(cap, 1), np.uint32)
         n, lat = _u64(), ctypes.c_double()
         _check(lib().exc_server_wait(self._h, ticket, _p(out), cap, ctypes.byref(n), ctypes.byref(lat)))
-        return out[:int(n.value)], lat.value
+        count = int(n.value)
+        return (out[:0] if self.counts_only else out[:count]), lat.value, count
 
     def replay(self, queries, arrival_s, cap):
         """Submit every query at its arrival time (seconds from now) and wait
@@ -561,9 +565,10 @@ class Server:
         assert len(arr) == nq
         lat = np.zeros(max(nq, 1), np.float64)
         cnt = np.zeros(max(nq, 1), np.uint32)
-        res = np.zeros(max(cap, 1), np.uint32)
-        _check(lib().exc_server_replay(self._h, _p(qterms), _p(qoff), _p(arr), nq, _p(lat), _p(cnt), _p(res), cap))
-        return lat[:nq], cnt[:nq], res[:int(cnt[:nq].sum())]
+        res = np.zeros(max(cap, 1), np.uint32) if cap else None  # cap 0: counts and latencies only
+        _check(lib().exc_server_replay(self._h, _p(qterms), _p(qoff), _p(arr), nq, _p(lat), _p(cnt),
+                                       _p(res) if cap else None, cap))
+        return lat[:nq], cnt[:nq], (res[:int(cnt[:nq].sum())] if cap else np.zeros(0, np.uint32))
 
     def stats(self):
         st = ServerInfo()

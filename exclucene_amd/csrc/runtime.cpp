@@ -16,6 +16,7 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -1615,10 +1616,28 @@ extern "C" int exc_ctx_set_tuning(exc_ctx* c, const exc_tune_table* t)
 // different slots overlap on the device; each finishing batch hands every
 // query its own results and latency.
 struct exc_server {
+    // Pinned host buffers for batches' results, pooled (pinning is slow): a
+    // batch's results are copied once, by DMA, into one buffer that its
+    // requests share until the last of them is collected.
+    struct Pinned {
+        uint32_t* p = nullptr;
+        size_t cap = 0;
+    };
+    struct Pool {
+        std::mutex mu;
+        std::vector<Pinned> free;
+        ~Pool()
+        {
+            for (auto& b : free) (void)hipHostFree(b.p);
+        }
+    };
+    std::shared_ptr<Pool> pool = std::make_shared<Pool>();
     struct Req {
         std::vector<uint32_t> terms;
         std::chrono::steady_clock::time_point arrival;
-        std::vector<uint32_t> res;
+        std::shared_ptr<Pinned> buf;  // the batch's results
+        uint64_t off = 0;             // this request's first result in buf
+        uint64_t count = 0;
         double latency_ms = 0;
         uint64_t need = 0;
         bool done = false;
@@ -1631,12 +1650,15 @@ struct exc_server {
         std::vector<uint64_t> qo{0};
         uint64_t bytes = 0;
         bool busy = false;
+        int phase = 0;               // 1: counts on their way, 2: results on their way
         hipEvent_t ev = nullptr;
         uint32_t* h_cum = nullptr;   // pinned
         size_t h_cap = 0;
+        std::shared_ptr<Pinned> res;  // results of the batch in flight
     };
     exc_index* ix = nullptr;
     uint32_t max_batch = 1024;
+    uint32_t flags = 0;
     uint64_t budget = 0, inflight = 0;
     std::vector<Slot> slots;
     std::mutex mu;
@@ -1679,17 +1701,56 @@ int slot_launch(exc_server* sv, exc_server::Slot& sl)
     }
     HIP_TRY(hipMemcpyAsync(sl.h_cum, c->d_qcum.p, nq * 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipEventRecord(sl.ev, c->stream));
+    sl.phase = 1;
     c->state = 3;
+    return EXC_OK;
+}
+
+// Counts are on the host: start the results' copy into the slot's pinned
+// buffer on its own stream (phase 2).  Returns false when there is nothing to
+// copy (counts-only server or empty results).
+int slot_copy_results(exc_server* sv, exc_server::Slot& sl, bool& started)
+{
+    started = false;
+    const size_t nq = sl.reqs.size();
+    const uint64_t total = nq ? sl.h_cum[nq - 1] : 0;
+    if ((sv->flags & EXC_SERVER_COUNTS) || total == 0) return EXC_OK;
+    exc_server::Pinned b;
+    {
+        std::lock_guard<std::mutex> g(sv->pool->mu);
+        auto& fl = sv->pool->free;
+        for (size_t i = 0; i < fl.size(); ++i)
+            if (fl[i].cap >= total) {
+                b = fl[i];
+                fl.erase(fl.begin() + (long)i);
+                break;
+            }
+    }
+    if (!b.p) {
+        b.cap = std::max<size_t>(total + total / 2, 1u << 20);
+        HIP_TRY(hipHostMalloc((void**)&b.p, b.cap * 4, hipHostMallocDefault));
+    }
+    std::weak_ptr<exc_server::Pool> wp = sv->pool;
+    sl.res = std::shared_ptr<exc_server::Pinned>(new exc_server::Pinned(b), [wp](exc_server::Pinned* x) {
+        if (auto pl = wp.lock()) {
+            std::lock_guard<std::mutex> g(pl->mu);
+            pl->free.push_back(*x);
+        } else {
+            (void)hipHostFree(x->p);
+        }
+        delete x;
+    });
+    HIP_TRY(hipMemcpyAsync(sl.res->p, sl.ctx->d_result.p, total * 4, hipMemcpyDeviceToHost, sl.ctx->stream));
+    HIP_TRY(hipEventRecord(sl.ev, sl.ctx->stream));
+    sl.phase = 2;
+    started = true;
     return EXC_OK;
 }
 
 void slot_finish(exc_server* sv, exc_server::Slot& sl, int err)
 {
-    exc_ctx* c = sl.ctx;
     const size_t nq = sl.reqs.size();
-    const uint64_t total = (!err && nq) ? sl.h_cum[nq - 1] : 0;
-    std::vector<uint32_t> all(total);
-    if (total && hipMemcpy(all.data(), c->d_result.p, total * 4, hipMemcpyDeviceToHost) != hipSuccess) err = EXC_ERR_DEVICE;
+    const bool with_res = !err && !(sv->flags & EXC_SERVER_COUNTS) && sl.phase == 2;
     const auto now = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> g(sv->mu);
     for (size_t i = 0; i < nq; ++i) {
@@ -1697,7 +1758,11 @@ void slot_finish(exc_server* sv, exc_server::Slot& sl, int err)
         r.err = err;
         if (!err) {
             const uint32_t b = i ? sl.h_cum[i - 1] : 0, e = sl.h_cum[i];
-            r.res.assign(all.begin() + b, all.begin() + e);
+            r.count = e - b;
+            if (with_res) {
+                r.buf = sl.res;
+                r.off = b;
+            }
         }
         r.latency_ms = std::chrono::duration<double, std::milli>(now - r.arrival).count();
         r.done = true;
@@ -1709,6 +1774,8 @@ void slot_finish(exc_server* sv, exc_server::Slot& sl, int err)
     sl.qo.assign(1, 0);
     sl.bytes = 0;
     sl.busy = false;
+    sl.phase = 0;
+    sl.res.reset();
     sv->cv_done.notify_all();
 }
 
@@ -1717,10 +1784,12 @@ void server_loop(exc_server* sv)
     (void)hipSetDevice(sv->slots[0].ctx->device);
     for (;;) {
         bool progressed = false;
-        // 1. retire finished batches
+        // 1. move batches along: counts in -> results copy, results in -> retire
         for (auto& sl : sv->slots) {
             if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
-                slot_finish(sv, sl, 0);
+                bool started = false;
+                int rc = sl.phase == 1 ? slot_copy_results(sv, sl, started) : EXC_OK;
+                if (rc || !started) slot_finish(sv, sl, rc);
                 progressed = true;
             }
         }
@@ -1773,7 +1842,7 @@ void server_loop(exc_server* sv)
 }  // namespace
 
 extern "C" int exc_server_create(exc_ctx* c, exc_index* ix, uint32_t nslots, uint32_t max_batch, uint64_t budget,
-                                 exc_server** out)
+                                 uint32_t flags, exc_server** out)
 {
     if (!c || !ix || !out || nslots == 0 || nslots > 16 || max_batch == 0)
         return set_error(EXC_ERR_ARG, "exc_server_create: bad argument");
@@ -1781,6 +1850,7 @@ extern "C" int exc_server_create(exc_ctx* c, exc_index* ix, uint32_t nslots, uin
     sv->ix = ix;
     sv->max_batch = max_batch;
     sv->budget = budget;
+    sv->flags = flags;
     sv->slots.resize(nslots);
     for (auto& sl : sv->slots) {
         int rc = exc_ctx_create(c->device, 0, &sl.ctx);
@@ -1835,7 +1905,7 @@ extern "C" int exc_server_wait(exc_server* sv, uint64_t ticket, uint32_t* result
     if (it == sv->reqs.end()) return set_error(EXC_ERR_ARG, "unknown or already collected ticket");
     sv->cv_done.wait(lk, [&] { return it->second.done; });
     auto& r = it->second;
-    if (n) *n = r.res.size();
+    if (n) *n = r.count;
     if (latency_ms) *latency_ms = r.latency_ms;
     if (r.err) {
         const int e = r.err;
@@ -1843,8 +1913,9 @@ extern "C" int exc_server_wait(exc_server* sv, uint64_t ticket, uint32_t* result
         return set_error(e, "query failed on the device");
     }
     // a too-small buffer keeps the request so the caller can retry with *n
-    if (r.res.size() > cap) return set_error(EXC_ERR_CAPACITY, "result buffer too small");
-    if (results && !r.res.empty()) std::memcpy(results, r.res.data(), r.res.size() * 4);
+    if (!(sv->flags & EXC_SERVER_COUNTS) && r.count > cap)
+        return set_error(EXC_ERR_CAPACITY, "result buffer too small");
+    if (results && r.buf && r.count) std::memcpy(results, r.buf->p + r.off, r.count * 4);
     sv->reqs.erase(it);
     return EXC_OK;
 }
@@ -1856,6 +1927,38 @@ extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const u
     for (uint64_t k = 0; k < qoff[nq]; ++k)
         if (qterms[k] >= sv->ix->nterms) return set_error(EXC_ERR_ARG, "query term %u out of range", qterms[k]);
     std::vector<uint64_t> tickets(nq);
+    std::atomic<uint32_t> submitted{0};
+    // Completed queries are collected (results copied out, buffers released)
+    // while later ones are still being submitted.
+    int crc = EXC_OK, cq = 0;
+    std::thread collector([&] {
+        uint64_t pos = 0;
+        for (uint32_t q = 0; q < nq; ++q) {
+            while (submitted.load(std::memory_order_acquire) <= q) std::this_thread::yield();
+            std::unique_lock<std::mutex> lk(sv->mu);
+            auto it = sv->reqs.find(tickets[q]);
+            sv->cv_done.wait(lk, [&] { return it->second.done; });
+            auto& r = it->second;
+            if (r.err && !crc) {
+                crc = r.err;
+                cq = (int)q;
+            }
+            if (lat_ms) lat_ms[q] = r.latency_ms;
+            if (counts) counts[q] = (uint32_t)r.count;
+            if (results && r.buf && r.count) {
+                if (pos + r.count > res_cap) {
+                    if (!crc) {
+                        crc = EXC_ERR_CAPACITY;
+                        cq = -1;
+                    }
+                } else {
+                    std::memcpy(results + pos, r.buf->p + r.off, r.count * 4);
+                }
+            }
+            pos += r.buf ? r.count : 0;
+            sv->reqs.erase(it);
+        }
+    });
     const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t q = 0; q < nq; ++q) {
         const auto at = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
@@ -1871,29 +1974,13 @@ extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const u
             sv->reqs.emplace(tickets[q], std::move(r));
             sv->queue.push_back(tickets[q]);
         }
+        submitted.store(q + 1, std::memory_order_release);
         sv->cv_req.notify_one();
     }
-    uint64_t pos = 0;
-    int rc = EXC_OK;
-    for (uint32_t q = 0; q < nq; ++q) {
-        std::unique_lock<std::mutex> lk(sv->mu);
-        auto it = sv->reqs.find(tickets[q]);
-        sv->cv_done.wait(lk, [&] { return it->second.done; });
-        auto& r = it->second;
-        if (r.err && !rc) rc = set_error(r.err, "query %u failed on the device", q);
-        if (lat_ms) lat_ms[q] = r.latency_ms;
-        if (counts) counts[q] = (uint32_t)r.res.size();
-        if (results) {
-            if (pos + r.res.size() > res_cap) {
-                if (!rc) rc = set_error(EXC_ERR_CAPACITY, "replay result buffer too small");
-            } else if (!r.res.empty()) {
-                std::memcpy(results + pos, r.res.data(), r.res.size() * 4);
-            }
-        }
-        pos += r.res.size();
-        sv->reqs.erase(it);
-    }
-    return rc;
+    collector.join();
+    if (crc == EXC_ERR_CAPACITY && cq < 0) return set_error(crc, "replay result buffer too small");
+    if (crc) return set_error(crc, "query %d failed on the device", cq);
+    return EXC_OK;
 }
 
 extern "C" int exc_server_stats(exc_server* sv, exc_server_info* st)

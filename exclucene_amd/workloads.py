@@ -10,6 +10,9 @@ C3  the C2 lists split into hash buckets on the top 12 docID bits (shift 13);
     10,000 2-term queries with short:long ratios 10^U(1,4).
 C4  the C2 lists with a learned position model on every list longer than
     65,536; 10,000 queries of 2-4 terms.
+C5  universe 2^26, 1,000,000 terms, df_k = min(2^25, floor(2^33 / k^1.1)),
+    lists synthesized and SegLRC-encoded on the GPU (about 40 GB resident);
+    2-6-term queries drawn proportionally to df arriving at 20k per second.
 Lists come from exc_synth_lists (splitmix64 per list, Floyd sampling for dense
 lists, sort-unique-refill for sparse ones); query streams from numpy PCG64.
 """
@@ -52,6 +55,25 @@ def df_weighted_queries(df, nq, tmin, tmax, seed):
     for _ in range(nq):
         tc = int(rng.integers(tmin, tmax + 1))
         out.append([int(t) for t in rng.choice(len(df), tc, replace=False, p=p)])
+    return out
+
+
+def df_weighted_queries_cdf(df, nq, tmin, tmax, seed):
+    """As df_weighted_queries for large vocabularies: terms drawn by inverse
+    CDF (with replacement, repeats within a query redrawn)."""
+    rng = np.random.default_rng(seed)
+    cdf = np.cumsum(df.astype(np.float64))
+    cdf /= cdf[-1]
+    out = []
+    for _ in range(nq):
+        tc = int(rng.integers(tmin, tmax + 1))
+        q = []
+        while len(q) < tc:
+            t = int(np.searchsorted(cdf, rng.random(), side="right"))
+            t = min(t, len(df) - 1)
+            if t not in q:
+                q.append(t)
+        out.append(q)
     return out
 
 

@@ -227,8 +227,11 @@ int exc_mlp_gradient(exc_ctx *ctx, const float *params, const uint32_t *docs, ui
  * results and latency.  nslots <= 16. */
 typedef struct exc_server exc_server;
 typedef struct { uint64_t batches, queries, max_inflight_batches, overlapped_launches; } exc_server_info;
+/* flags: EXC_SERVER_COUNTS -- return per-query counts and latencies only (the
+ * docIDs stay in device memory; for callers that rank or page on the device). */
+#define EXC_SERVER_COUNTS 1u
 int exc_server_create(exc_ctx *ctx, exc_index *ix, uint32_t nslots, uint32_t max_batch, uint64_t mem_budget,
-                      exc_server **out);
+                      uint32_t flags, exc_server **out);
 int exc_server_submit(exc_server *sv, const uint32_t *terms, uint32_t nterms, uint64_t *ticket);
 int exc_server_wait(exc_server *sv, uint64_t ticket, uint32_t *results, uint64_t cap, uint64_t *n,
                     double *latency_ms);

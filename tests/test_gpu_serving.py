@@ -84,3 +84,17 @@ def test_submit_wait_from_threads(ctx, served):
         srv.close()
     for i in range(len(qs)):
         assert np.array_equal(out[i], alone[i]), i
+
+
+def test_counts_only_server(ctx, served):
+    lists, built, dix = served
+    qs = _queries(len(lists), 300, 77)
+    alone = [len(ctx.intersect(dix, [q])[1]) for q in qs]
+    srv = ea.Server(ctx, dix, nslots=3, max_batch=64, counts_only=True)
+    try:
+        lat, cnt, res = srv.replay(qs, np.zeros(len(qs)), 0)
+        tk = srv.submit(qs[0])
+        got, _, n = srv.wait(tk, cap=0)
+    finally:
+        srv.close()
+    assert np.array_equal(cnt, alone) and len(res) == 0 and len(got) == 0 and n == alone[0]
