@@ -10,8 +10,9 @@ C3  the C2 lists split into hash buckets on the top 12 docID bits (shift 13);
     10,000 2-term queries with short:long ratios 10^U(1,4).
 C4  the C2 lists with a learned position model on every list longer than
     65,536; 10,000 queries of 2-4 terms.
-C5  universe 2^26, 1,000,000 terms, df_k = min(2^25, floor(2^33 / k^1.1)),
-    lists synthesized and SegLRC-encoded on the GPU (about 40 GB resident);
+C5  universe 2^26, 1,000,000 terms, df_k = min(2^25, floor(C / k^1.1)) with
+    C = 5.8e9, lists synthesized and SegLRC-encoded on the GPU (about 40 GB
+    resident);
     2-6-term queries drawn proportionally to df arriving at 20k per second.
 Lists come from exc_synth_lists (splitmix64 per list, Floyd sampling for dense
 lists, sort-unique-refill for sparse ones); query streams from numpy PCG64.
