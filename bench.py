@@ -237,6 +237,34 @@ def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
     return out
 
 
+def c1_leg(ea, W, ctx, reps=2000):
+    """C1, the plain method: one 2-term query (1,000 and 1,000,000 docIDs from
+    [0, 2^24), LRC with 256-docID blocks) run `reps` times through the plain
+    single-query path (binary search, one lane per element, one launch, host
+    planning and PCIe included)."""
+    a, b = W.c1_lists(1)
+    docs, off = ea.lists_to_flat([a, b])
+    built = ea.build_index(docs, off, ea.LRC, seg=256)
+    dix = ctx.upload(built)
+    want = np.intersect1d(a, b)
+    ctx.set_tuning(None)
+    ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_BINARY)
+    ctx.set_option(ea.OPT_TPE, 1)
+    ctx.set_option(ea.OPT_PERSISTENT, 0)
+    ctx.set_option(ea.OPT_TIMING, 0)
+    lat, ok = [], True
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        cum, res = ctx.intersect(dix, [[0, 1]])
+        lat.append(time.perf_counter() - t0)
+        ok = ok and np.array_equal(res, want)
+    ctx.set_option(ea.OPT_TIMING, 1)
+    dix.close()
+    lat = np.array(lat) * 1e3
+    return {"queries_per_s": reps / (lat.sum() * 1e-3), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "results": int(len(want)), "exact": bool(ok), "reps": reps}
+
+
 def ratio_class(df, q):
     """List-length-ratio class of a query (as the library's autotuner forms them)."""
     s = sorted(q, key=lambda t: df[t])
@@ -376,6 +404,7 @@ def main():
     ap.add_argument("--nq", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the plain-method single-query leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the hash-bucket leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 1M-term, 35G-posting serving configuration")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-window leg")
@@ -515,6 +544,11 @@ def main():
         log("ParaPFD batched: %.0f q/s" % extra["parapfd_batched_qps"])
         pix.close()
         del pfd
+
+    if not args.no_c1:
+        extra["plain_c1"] = c1_leg(ea, W, ctx)
+        log("C1: %s" % extra["plain_c1"])
+        headline_opts()
 
     if not args.no_c3:
         extra["hash_buckets"] = c3_leg(ea, W, ctx, docs, off, df, dix, max(5, args.steps // 2))

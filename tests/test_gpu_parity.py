@@ -78,3 +78,26 @@ def test_decompress_roundtrip(ctx, codec, seg, frac, shift):
         assert np.array_equal(out, np.concatenate([lists[t] for t in terms]))
     finally:
         dix.close()
+
+
+def test_c1_plain_query_on_device(ctx):
+    """C1 on the device: LRC-256, the plain method (binary search, one lane per
+    element, one launch) and the tuned batch path give the oracle's answer."""
+    from exclucene_amd import workloads as W
+    a, b = W.c1_lists(1)
+    docs, off = ea.lists_to_flat([a, b])
+    built = ea.build_index(docs, off, ea.LRC, seg=256)
+    dix = ctx.upload(built)
+    want = np.intersect1d(a, b)
+    try:
+        for search, tpe, pers in ((0, 1, 0), (1, 0, 1), (1, 64, 0)):
+            ctx.set_option(ea.OPT_SEARCH, search)
+            ctx.set_option(ea.OPT_TPE, tpe)
+            ctx.set_option(ea.OPT_PERSISTENT, pers)
+            cum, res = ctx.intersect(dix, [[0, 1]])
+            assert cum[0] == len(want) and np.array_equal(res, want), (search, tpe, pers)
+    finally:
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        ctx.set_option(ea.OPT_TPE, 0)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        dix.close()

@@ -234,3 +234,17 @@ def test_planted_mistake_is_caught():
     ix = oracle.build_index([L], oracle.SEGLRC, seg=128)
     ix.words[ix.seg_off[3] + 2] ^= 1 << 7
     assert not np.array_equal(ix.decode_term(0), L)
+
+
+def test_c1_plain_query():
+    """The plain-method configuration: one 2-term query, 1,000 and 1,000,000
+    docIDs from [0, 2^24) (seed 1), LRC with 256-docID blocks; the oracle's
+    decode-and-intersect equals numpy's set intersection."""
+    from exclucene_amd import workloads as W
+    a, b = W.c1_lists(1)
+    ix = oracle.build_index([a, b], oracle.LRC, seg=256)
+    cum, res = oracle.eval_queries(ix, [[0, 1], [1, 0]])
+    want = np.intersect1d(a, b)
+    assert np.array_equal(res[:cum[0]], want) and np.array_equal(res[cum[0]:], want)
+    # expected size n_a * n_b / 2^24 = 59.6; a seeded draw lands near it
+    assert 30 <= len(want) <= 95
