@@ -135,7 +135,10 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
 #pragma unroll
     for (int t = 0; t < V; ++t) {
         const uint32_t j = (CODEC == EXC_PARAPFD) ? ln * V + t : ln + 64 * t;
-        f[t] = (j < len) ? bits_at(data, lb, j) : 0u;
+        // every lane loads (from a clamped position inside the block) so that
+        // all loads of a group are in flight before the first wait
+        const uint32_t x = bits_at(data, lb, len ? min(j, len - 1u) : 0u);
+        f[t] = (j < len) ? x : 0u;
     }
 }
 
@@ -217,17 +220,14 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
         const uint32_t* data[G];
         uint32_t f[G][V];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if (g0 + g < r) {
-                sg[g] = sid((int)(g0 + g));
-                rec[g] = ix.rec[R.segnum + sg[g]];
-                len[g] = seg_len(R, sg[g], SEG);
-                data[g] = ix.words + R.woff + rec[g].off + 1;
-            }
+        for (int g = 0; g < G; ++g) {  // past r: a copy of the last block, loaded but not used
+            sg[g] = sid((int)min(g0 + g, r - 1));
+            rec[g] = ix.rec[R.segnum + sg[g]];
+            len[g] = seg_len(R, sg[g], SEG);
+            data[g] = ix.words + R.woff + rec[g].off + 1;
         }
 #pragma unroll
-        for (int g = 0; g < G; ++g)
-            if (g0 + g < r) load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
+        for (int g = 0; g < G; ++g) load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
 #pragma unroll
         for (int g = 0; g < G; ++g)
             if (g0 + g < r)
@@ -493,14 +493,14 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                     uint32_t f[NBB][SEG / 64];
 #pragma unroll
                     for (int g = 0; g < NBB; ++g) {
-                        if ((uint32_t)g < np) {
-                            const int src = (int)(p0 + g);
+                        {  // past np: a copy of the piece's last block, loaded but not used
+                            const int src = (int)min(p0 + g, b1);
                             rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
                             rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, src));
                             rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), src)));
                             rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
                             rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
-                            len[g] = seg_len(R, sf + p0 + g, SEG);
+                            len[g] = seg_len(R, sf + (uint32_t)src, SEG);
                             load_fields<CODEC, SEG>(A.ix.words + R.woff + rec[g].off + 1, desc_lb(rec[g].desc), len[g],
                                                     f[g]);
                         }
@@ -549,15 +549,15 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             uint32_t f[P][SEG / 64];
 #pragma unroll
             for (int g = 0; g < P; ++g) {
-                if ((uint32_t)g < np) {
-                    const int src = (int)(p0 + g);
+                {  // past np: a copy of the piece's last block, loaded but not used
+                    const int src = (int)min(p0 + g, r - 1);
                     rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
                     rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, src));
                     rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), src)));
                     rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
                     rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
                     rec[g].head = uni(__builtin_amdgcn_readlane(gr.head, src));
-                    len[g] = seg_len(R, sf + p0 + g, SEG);
+                    len[g] = seg_len(R, sf + (uint32_t)src, SEG);
                     data[g] = A.ix.words + R.woff + rec[g].off + 1;
                     load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
                 }
