@@ -202,9 +202,21 @@ hipError_t scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStr
 int built_from_seglrc(uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
                       const uint32_t* head, const uint32_t* words, uint64_t nwords, exc_built** out);
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
-hipError_t launch_decompress(const DevIndex& ix, const uint32_t* terms, const uint64_t* out_off,
-                             const uint32_t* unit_term, const uint32_t* unit_k, uint32_t nunits,
-                             uint32_t* out, hipStream_t s);
+// One whole-list decode work unit (kDecodeUnit docIDs of one list), with the
+// list fields the decoder needs so that its first load brings all of them.
+struct alignas(16) DecUnit {
+    uint64_t out;     // output offset of the unit's first docID
+    uint64_t woff;    // the list's word offset
+    uint32_t segnum;  // the list's first segment
+    uint32_t n;       // list length
+    uint32_t k;       // first block of the unit (HS: chunk)
+    uint32_t term;    // term index (the HS decoder reads its bucket table)
+    float slope;      // LRC list-level fit
+    int32_t icpt;
+    uint32_t pad[2];
+};
+static_assert(sizeof(DecUnit) == 48, "DecUnit is 48 bytes");
+hipError_t launch_decompress(const DevIndex& ix, const DecUnit* units, uint32_t n, uint32_t* out, hipStream_t s);
 uint32_t probe_waves_per_block();
 
 }  // namespace exc

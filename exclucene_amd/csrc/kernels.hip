@@ -205,11 +205,15 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
     }
 }
 
-// Decode segments listed by sid(g), g < r <= NB, of list R into dst + g*SEG.
+// Decode segments listed by sid(g), g < r <= NB, of list R into dst + g*SEG
+// (sid is called with a lane-dependent g).  LANE_RECS: a group's segment
+// records come in one lane-parallel load and are spread by readlane (fewer
+// registers: the whole-list decoder); otherwise one uniform load per block
+// (what the probe kernel's register budget prefers).
 // Blocks go in groups (all of them for the random-access codecs, pairs for
 // ParaPFD whose scans hold more registers); every load of a group is issued
 // before any of its values is consumed.
-template <int CODEC, int SEG, int NB, class SegId>
+template <int CODEC, int SEG, int NB, bool LANE_RECS = false, class SegId>
 __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, SegId sid, uint32_t* dst)
 {
     constexpr int V = SEG / 64;
@@ -219,10 +223,30 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
         uint32_t len[G], sg[G];
         const uint32_t* data[G];
         uint32_t f[G][V];
+        // past r: a copy of the last block, loaded but not used
+        if constexpr (LANE_RECS) {
+            const uint32_t sl = sid((int)min(g0 + min(lane_id(), (uint32_t)(G - 1)), r - 1));
+            const SegRec gr = ix.rec[R.segnum + sl];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {  // past r: a copy of the last block, loaded but not used
-            sg[g] = sid((int)min(g0 + g, r - 1));
-            rec[g] = ix.rec[R.segnum + sg[g]];
+            for (int g = 0; g < G; ++g) {
+                sg[g] = uni(__builtin_amdgcn_readlane(sl, g));
+                rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, g));
+                rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, g));
+                rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), g)));
+                rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), g)));
+                rec[g].rl = rec[g].rr = 0;
+                rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, g));
+                rec[g].head = uni(__builtin_amdgcn_readlane(gr.head, g));
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                sg[g] = uni(sid((int)min(g0 + g, r - 1)));
+                rec[g] = ix.rec[R.segnum + sg[g]];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
             len[g] = seg_len(R, sg[g], SEG);
             data[g] = ix.words + R.woff + rec[g].off + 1;
         }
@@ -620,7 +644,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             wave_sync();
             for (uint32_t r0 = 0; r0 < nd; r0 += NB) {
                 const uint32_t gn = min((uint32_t)NB, nd - r0);
-                decode_blocks<CODEC, SEG, NB>(A.ix, R, gn, [&](int g) { return uni(W.dlist[r0 + g]); }, W.buf);
+                decode_blocks<CODEC, SEG, NB, true>(A.ix, R, gn, [&](int g) { return W.dlist[r0 + g]; }, W.buf);
                 wave_sync();
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
@@ -785,7 +809,8 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     } else {
         const uint32_t s0 = k * U;
         const uint32_t nu = uni(min(U, R0.nseg - s0));
-        decode_blocks<CODEC, SEG, U>(A.ix, R0, nu, [&](int g) { return s0 + (uint32_t)g; }, W.cand);
+        decode_blocks<CODEC, SEG, U, CODEC == EXC_PARAPFD>(A.ix, R0, nu, [&](int g) { return s0 + (uint32_t)g; },
+                                                           W.cand);
         nlive = (nu - 1) * SEG + seg_len(R0, s0 + nu - 1, SEG);
     }
     nlive = uni(nlive);
@@ -946,9 +971,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
 // SegLRC values go from registers straight to HBM in coalesced stores,
 // ParaPFD values pass through LDS for the exception patch and the scan.
 template <int CODEC, int SEG>
-__global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const uint32_t* terms, const uint64_t* out_off,
-                                                    const uint32_t* unit_term, const uint32_t* unit_k,
-                                                    uint32_t nunits, uint32_t* out)
+__global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
 {
     constexpr bool HS = CODEC == EXC_HS_SEGLRC;
     constexpr int U = HS ? 1 : (int)(kDecodeUnit / SEG);
@@ -957,25 +980,33 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const uint32_t*
     const uint32_t wid = uni(threadIdx.x >> 6), ln = lane_id();
     const uint32_t u = blockIdx.x * kWaves + wid;
     if (u >= nunits) return;
-    const uint32_t ti = uni(unit_term[u]), k = uni(unit_k[u]);  // k = first block (HS: chunk)
-    const TermRec R = ix.terms[uni(terms[ti])];
-    uint32_t* o = out + out_off[ti] + (uint64_t)k * SEG;
+    const DecUnit D = units[u];
+    const uint32_t k = uni(D.k);
+    uint32_t* o = out + D.out;
     if constexpr (HS) {
+        const TermRec R = ix.terms[uni(D.term)];
         uint32_t* L = lds + wid * SEG;
         const uint32_t n = decode_hs_chunk<SEG>(ix, R, k, L);
         wave_sync();
         for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
     } else {
+        TermRec R;  // the fields decode_blocks reads, from the unit record
+        R.woff = D.woff;
+        R.n = D.n;
+        R.segnum = D.segnum;
+        R.nseg = (D.n + SEG - 1) / SEG;
+        R.slope = D.slope;
+        R.icpt = D.icpt;
         const uint32_t nb = uni(min((uint32_t)U, R.nseg - k));
         const auto sid = [&](int g) { return k + (uint32_t)g; };
         if constexpr (VIA_LDS) {
             uint32_t* L = lds + wid * U * SEG;
-            decode_blocks<CODEC, SEG, U>(ix, R, nb, sid, L);
+            decode_blocks<CODEC, SEG, U, true>(ix, R, nb, sid, L);
             wave_sync();
             const uint32_t n = (nb - 1) * SEG + seg_len(R, k + nb - 1, SEG);
             for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
         } else {
-            decode_blocks<CODEC, SEG, U>(ix, R, nb, sid, o);
+            decode_blocks<CODEC, SEG, U, true>(ix, R, nb, sid, o);
         }
     }
 }
@@ -991,14 +1022,13 @@ hipError_t probe_for_seg(const ProbeArgs& a, uint32_t grid, hipStream_t s)
 }
 
 template <int CODEC>
-hipError_t decomp_for_seg(const DevIndex& ix, const uint32_t* terms, const uint64_t* out_off,
-                          const uint32_t* ut, const uint32_t* uk, uint32_t n, uint32_t* out, hipStream_t s)
+hipError_t decomp_for_seg(const DevIndex& ix, const DecUnit* units, uint32_t n, uint32_t* out, hipStream_t s)
 {
     const uint32_t grid = (n + kWaves - 1) / kWaves;
     if (ix.seg == 256)
-        hipLaunchKernelGGL((k_decompress<CODEC, 256>), dim3(grid), dim3(256), 0, s, ix, terms, out_off, ut, uk, n, out);
+        hipLaunchKernelGGL((k_decompress<CODEC, 256>), dim3(grid), dim3(256), 0, s, ix, units, n, out);
     else
-        hipLaunchKernelGGL((k_decompress<CODEC, 128>), dim3(grid), dim3(256), 0, s, ix, terms, out_off, ut, uk, n, out);
+        hipLaunchKernelGGL((k_decompress<CODEC, 128>), dim3(grid), dim3(256), 0, s, ix, units, n, out);
     return hipGetLastError();
 }
 
@@ -1066,15 +1096,14 @@ hipError_t launch_scan(const ScanArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_decompress(const DevIndex& ix, const uint32_t* terms, const uint64_t* out_off,
-                             const uint32_t* ut, const uint32_t* uk, uint32_t n, uint32_t* out, hipStream_t s)
+hipError_t launch_decompress(const DevIndex& ix, const DecUnit* units, uint32_t n, uint32_t* out, hipStream_t s)
 {
     if (!n) return hipSuccess;
     switch (ix.codec) {
-    case EXC_LRC: return decomp_for_seg<EXC_LRC>(ix, terms, out_off, ut, uk, n, out, s);
-    case EXC_SEGLRC: return decomp_for_seg<EXC_SEGLRC>(ix, terms, out_off, ut, uk, n, out, s);
-    case EXC_HS_SEGLRC: return decomp_for_seg<EXC_HS_SEGLRC>(ix, terms, out_off, ut, uk, n, out, s);
-    case EXC_PARAPFD: return decomp_for_seg<EXC_PARAPFD>(ix, terms, out_off, ut, uk, n, out, s);
+    case EXC_LRC: return decomp_for_seg<EXC_LRC>(ix, units, n, out, s);
+    case EXC_SEGLRC: return decomp_for_seg<EXC_SEGLRC>(ix, units, n, out, s);
+    case EXC_HS_SEGLRC: return decomp_for_seg<EXC_HS_SEGLRC>(ix, units, n, out, s);
+    case EXC_PARAPFD: return decomp_for_seg<EXC_PARAPFD>(ix, units, n, out, s);
     }
     return hipErrorInvalidValue;
 }

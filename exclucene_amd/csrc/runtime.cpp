@@ -900,37 +900,43 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
 {
     const uint32_t nt = (uint32_t)terms.size();
     off.assign(nt + 1, 0);
-    std::vector<uint32_t> ut, uk;
+    std::vector<DecUnit> units;
     // units: runs of kDecodeUnit/seg blocks (HS: chunks of seg elements), k = first block
     const uint32_t step = ix->codec == EXC_HS_SEGLRC ? 1u : kDecodeUnit / ix->seg;
     for (uint32_t i = 0; i < nt; ++i) {
+        const TermRec& R = ix->recs[terms[i]];
         const uint32_t n = ix->n[terms[i]];
         off[i + 1] = off[i] + n;
-        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; k += step) { ut.push_back(i); uk.push_back(k); }
+        for (uint32_t k = 0; k < (n + ix->seg - 1) / ix->seg; k += step) {
+            DecUnit d;
+            std::memset(&d, 0, sizeof d);
+            d.out = off[i] + (uint64_t)k * ix->seg;
+            d.woff = R.woff;
+            d.segnum = R.segnum;
+            d.n = n;
+            d.k = k;
+            d.term = terms[i];
+            d.slope = R.slope;
+            d.icpt = R.icpt;
+            units.push_back(d);
+        }
     }
     c->probe_ms = 0;
     if (!off[nt]) return EXC_OK;
-    if (ut.size() > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 decode units");
-    DevBuf dt, doff, dut, duk;
+    if (units.size() > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 decode units");
+    DevBuf du;
     int rc;
-    if ((!dst && (rc = dev_alloc(c, dec, off[nt] * 4))) || (rc = dev_alloc(c, dt, nt * 4)) ||
-        (rc = dev_alloc(c, doff, (nt + 1) * 8)) || (rc = dev_alloc(c, dut, ut.size() * 4)) ||
-        (rc = dev_alloc(c, duk, uk.size() * 4))) {
-        for (DevBuf* b : {&dt, &doff, &dut, &duk}) dev_free(c, *b);
+    if ((!dst && (rc = dev_alloc(c, dec, off[nt] * 4))) || (rc = dev_alloc(c, du, units.size() * sizeof(DecUnit))))
         return rc;
-    }
-    hipError_t e = hipMemcpy(dt.p, terms.data(), nt * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(doff.p, off.data(), (nt + 1) * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dut.p, ut.data(), ut.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(duk.p, uk.data(), uk.size() * 4, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(du.p, units.data(), units.size() * sizeof(DecUnit), hipMemcpyHostToDevice);
     if (e == hipSuccess && c->timing) e = hipEventRecord(c->ev[1], c->stream);
     if (e == hipSuccess)
-        e = launch_decompress(ix->dix, dptr<uint32_t>(dt), dptr<uint64_t>(doff), dptr<uint32_t>(dut),
-                              dptr<uint32_t>(duk), (uint32_t)ut.size(), dst ? dst : dptr<uint32_t>(dec), c->stream);
+        e = launch_decompress(ix->dix, dptr<DecUnit>(du), (uint32_t)units.size(), dst ? dst : dptr<uint32_t>(dec),
+                              c->stream);
     if (e == hipSuccess && c->timing) e = hipEventRecord(c->ev[2], c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && c->timing) (void)hipEventElapsedTime(&c->probe_ms, c->ev[1], c->ev[2]);
-    for (DevBuf* b : {&dt, &doff, &dut, &duk}) dev_free(c, *b);
+    dev_free(c, du);
     if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "decode: %s", hipGetErrorString(e));
     return EXC_OK;
 }
