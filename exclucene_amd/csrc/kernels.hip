@@ -828,11 +828,12 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     wave_sync();
 }
 
-// Eight waves per SIMD for the random-access codecs (64 VGPRs, 20 KB of LDS
-// per workgroup): the probe is bound by load latency, and the extra resident
-// waves hide more of it than the few spilled registers cost.
+// Seven waves per SIMD for the random-access codecs (72 VGPRs, 20 KB of LDS
+// per workgroup): the probe is bound by load latency, so resident waves pay,
+// up to the point where spilled registers cost more (measured on the C2
+// batch: 5 waves 1.00 ms, 6 0.93, 7 0.89, 8 0.96).
 template <int CODEC, int SEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 8, 8))) void k_probe(ProbeArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 7, 7))) void k_probe(ProbeArgs A)
 {
     __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);
