@@ -477,14 +477,15 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     const uint32_t r = uni((uint32_t)(s_last - s_first + 1));
     const uint32_t sf = uni((uint32_t)s_first);
 
-    uint32_t v[V];
+    // candidates are re-read from LDS where needed instead of being held in
+    // registers through the bitmap path (its register peak)
+    const auto cand_at = [&](int i) {
+        const uint32_t e = ln + 64 * i;
+        return e < nlive ? cand[e] : kNone;
+    };
     bool found[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-        const uint32_t e = ln + 64 * i;
-        v[i] = e < nlive ? cand[e] : kNone;
-        found[i] = false;
-    }
+    for (int i = 0; i < V; ++i) found[i] = false;
 
     const uint32_t cmax = knob_coop(knobs);
     constexpr uint32_t kWin = kBufWords * 32u;  // docIDs one LDS bitmap covers
@@ -503,7 +504,10 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             const uint32_t hi = (vmax - lo < kWin) ? vmax : lo + (kWin - 1);
             bool any = false;
 #pragma unroll
-            for (int i = 0; i < V; ++i) any = any || (ln + 64 * i < nlive && v[i] >= lo && v[i] <= hi);
+            for (int i = 0; i < V; ++i) {
+                const uint32_t vi = cand_at(i);
+                any = any || (ln + 64 * i < nlive && vi >= lo && vi <= hi);
+            }
             if (__ballot(any)) {
                 const uint32_t c_lo = popc(__ballot(hd <= lo)), c_hi = popc(__ballot(hd <= hi));
                 const uint32_t b0 = c_lo ? c_lo - 1 : 0, b1 = c_hi ? c_hi - 1 : 0;
@@ -546,8 +550,8 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                 wave_sync();
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
-                    const uint32_t x = v[i] - lo;
-                    if (ln + 64 * i < nlive && v[i] >= lo && x <= hi - lo) found[i] = (bm[x >> 5] >> (x & 31)) & 1u;
+                    const uint32_t vi = cand_at(i), x = vi - lo;
+                    if (ln + 64 * i < nlive && vi >= lo && x <= hi - lo) found[i] = (bm[x >> 5] >> (x & 31)) & 1u;
                 }
                 wave_sync();
             }
@@ -560,6 +564,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         // per piece brings the packed words, sorted candidates resolve in the
         // piece that reaches their value
         constexpr int P = (CODEC == EXC_PARAPFD) ? (int)(512 / SEG) : NB;
+        uint32_t v[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = cand_at(i);
         const uint32_t gsel = min(ln, r - 1);
         const SegRec gr = A.ix.rec[R.segnum + sf + gsel];
         bool done[V];
@@ -607,6 +614,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         }
     } else {
         // ---- segment of every candidate (-1: precedes the list, -2: no candidate)
+        uint32_t v[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = cand_at(i);
         const uint32_t h63 = uni(__builtin_amdgcn_readlane(H, 63));
         int32_t s[V];
 #pragma unroll
@@ -681,18 +691,18 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         }
     }
 
-    // ---- order-preserving compaction of survivors
-    uint32_t cnt = 0, pos[V];
+    // ---- order-preserving compaction of survivors, row by row: a row is read
+    // before it is written and its survivors move only to lower positions,
+    // so later rows are intact when read
+    uint32_t cnt = 0;
+    wave_sync();
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         const uint64_t m = __ballot(found[i]);
-        pos[i] = cnt + below_me(m);
+        const uint32_t vi = cand_at(i);
+        if (found[i]) cand[cnt + below_me(m)] = vi;
         cnt += popc(m);
     }
-    wave_sync();
-#pragma unroll
-    for (int i = 0; i < V; ++i)
-        if (found[i]) cand[pos[i]] = v[i];
     wave_sync();
     return uni(cnt);
 }
