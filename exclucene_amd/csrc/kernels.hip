@@ -838,12 +838,12 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     wave_sync();
 }
 
-// Seven waves per SIMD for the random-access codecs (72 VGPRs, 20 KB of LDS
-// per workgroup): the probe is bound by load latency, so resident waves pay,
-// up to the point where spilled registers cost more (measured on the C2
-// batch: 5 waves 1.00 ms, 6 0.93, 7 0.89, 8 0.96).
+// Eight waves per SIMD for the random-access codecs (64 VGPRs, 20 KB of LDS
+// per workgroup): the probe is bound by load latency, so resident waves pay
+// as long as the spilled registers stay few (C2 batch: 7 waves 0.88 ms,
+// 8 waves 0.85 ms with 32 spilled bytes per lane).
 template <int CODEC, int SEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 7, 7))) void k_probe(ProbeArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 8, 8))) void k_probe(ProbeArgs A)
 {
     __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);
