@@ -41,6 +41,8 @@ constexpr uint32_t kClaim = 4;        // units per work-queue claim (persistent 
 constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
 constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative LRC/SegLRC probe
 constexpr uint32_t kBitmapPiece = 768; // docIDs decoded per load round of the bitmap path
+constexpr uint32_t kWinWords = 12;     // words of a candidate's fit window staged in LDS (64 x 12 <= kBufWords)
+static_assert(64 * kWinWords <= kBufWords, "window staging fits the decode buffer");
 constexpr int kV = (int)(kChunk / 64);  // candidates per lane
 
 __device__ __forceinline__ void wave_sync()
@@ -319,6 +321,16 @@ __device__ __forceinline__ uint32_t block_value(const TermRec& R, uint32_t s, co
         return f + (uint32_t)(lrc_y(s * SEG + j, R.slope) + R.icpt);
     else
         return f + (uint32_t)(seg_y(j, rec.slope, rec.icpt) + rec.base);
+}
+
+// Value j of a SegLRC block whose packed words from w0 on are staged in LDS,
+// word k at col[64 k] (one column per lane).
+__device__ __forceinline__ uint32_t win_value(const uint32_t* col, uint32_t lb, uint32_t w0, uint32_t j,
+                                              const SegRec& r)
+{
+    const uint32_t bit = j * lb - w0 * 32u, k = bit >> 5;
+    const uint64_t cw = (uint64_t)col[64u * k] | ((uint64_t)col[64u * (k + 1)] << 32);
+    return ((uint32_t)(cw >> (bit & 31)) & ((1u << lb) - 1u)) + (uint32_t)(seg_y(j, r.slope, r.icpt) + r.base);
 }
 
 // Branchless search of v in the sorted LDS run buf[0..n) (n uniform).
@@ -613,29 +625,85 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             if (!__ballot(any)) break;
         }
     } else {
-        // ---- segment of every candidate (-1: precedes the list, -2: no candidate)
-        uint32_t v[V];
-#pragma unroll
-        for (int i = 0; i < V; ++i) v[i] = cand_at(i);
         const uint32_t h63 = uni(__builtin_amdgcn_readlane(H, 63));
-        int32_t s[V];
-#pragma unroll
-        for (int i = 0; i < V; ++i) {
-            const bool live = ln + 64 * i < nlive;
+        // segment of a candidate (-1: precedes the list, -2: no candidate)
+        const auto seg_of = [&](uint32_t vi, bool live) -> int32_t {
             int64_t cnt;
             if (small) {
-                cnt = lane_count_le(H, h63, v[i]);
+                cnt = lane_count_le(H, h63, vi);
             } else {
                 uint32_t lo = 0, hi = (uint32_t)rng;
                 if (live)
                     while (lo < hi) {
                         const uint32_t m = (lo + hi) >> 1;
-                        if (head[hb0 + m] <= v[i]) lo = m + 1; else hi = m;
+                        if (head[hb0 + m] <= vi) lo = m + 1; else hi = m;
                     }
                 cnt = lo;
             }
-            s[i] = live ? (int32_t)(hb0 + cnt - 1) : -2;
-        }
+            return live ? (int32_t)(hb0 + cnt - 1) : -2;
+        };
+        if constexpr (CODEC == EXC_SEGLRC) {
+            // ---- one lane per candidate, row by row (a loop, so its state stays
+            // small): the segment record, then the words of the positions the
+            // block fit allows ([p - rl - 1, p + rr + 1], up to kWinWords
+            // words) in one load round, staged in LDS (word k of lane l at
+            // buf[64k + l]) and binary-searched there.  A candidate outside the
+            // window's values, or a window too wide, falls back to the exact
+            // search in HBM (plain binary search when the search mode is binary).
+            uint32_t fmask = 0;
+            for (int i = 0; i < V; ++i) {
+                const uint32_t e = ln + 64u * (uint32_t)i;
+                if (!__ballot(e < nlive)) break;  // rows fill in order
+                const uint32_t vi = e < nlive ? cand[e] : kNone;
+                const int32_t sc = seg_of(vi, e < nlive);
+                if (!__ballot(sc >= 0)) continue;
+                const uint32_t si = (uint32_t)(sc >= 0 ? sc : (int32_t)sf);
+                const SegRec rc = A.ix.rec[R.segnum + si];
+                const uint32_t len = seg_len(R, si, SEG), lb = desc_lb(rc.desc);
+                const uint32_t* data = A.ix.words + R.woff + rc.off + 1;
+                const float fp = ((float)vi - rc.icpt) * __builtin_amdgcn_rcpf(rc.slope) + 0.5f;
+                const int last = (int)len - 1;
+                const int pp = (int)fminf(fmaxf(fp, 0.0f), (float)last);
+                const int wl = max(0, pp - (int)min(rc.rl, 64u) - 1), wh = min(last, pp + (int)min(rc.rr, 64u) + 1);
+                const uint32_t w0 = ((uint32_t)wl * lb) >> 5;
+                const bool ok = search != SEARCH_BINARY && lb > 0 && ((((uint32_t)wh * lb) >> 5) - w0) + 2 <= kWinWords;
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < (int)kWinWords; ++k) W.buf[64 * k + ln] = data[w0 + k];
+                wave_sync();
+                bool f = false;
+                if (sc >= 0) {
+                    if (!ok) {
+                        f = seg_find(data, len, rc, vi, search != SEARCH_BINARY);
+                    } else {
+                        const uint32_t vlo = win_value(W.buf + ln, lb, w0, (uint32_t)wl, rc);
+                        const uint32_t vhi = win_value(W.buf + ln, lb, w0, (uint32_t)wh, rc);
+                        if (vi < vlo) {
+                            f = wl > 0 && seg_bsearch(data, lb, rc, 0, wl - 1, vi);
+                        } else if (vi > vhi) {
+                            f = wh < last && seg_bsearch(data, lb, rc, wh + 1, last, vi);
+                        } else {
+                            int a0 = wl, a1 = wh;
+                            while (a0 <= a1) {
+                                const int m = (a0 + a1) >> 1;
+                                const uint32_t q = win_value(W.buf + ln, lb, w0, (uint32_t)m, rc);
+                                if (q == vi) { f = true; break; }
+                                if (q < vi) a0 = m + 1; else a1 = m - 1;
+                            }
+                        }
+                    }
+                }
+                if (f) fmask |= 1u << i;
+            }
+#pragma unroll
+            for (int i = 0; i < V; ++i) found[i] = (fmask >> i) & 1u;
+        } else {
+        uint32_t v[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) v[i] = cand_at(i);
+        int32_t s[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) s[i] = seg_of(v[i], ln + 64 * i < nlive);
         if constexpr (CODEC == EXC_PARAPFD) {
             // ---- decode each distinct needed segment once, NB per round
             uint32_t nd = 0, di[V];
@@ -663,21 +731,6 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                 }
                 wave_sync();
             }
-        } else if constexpr (CODEC == EXC_SEGLRC) {
-            // ---- one lane per candidate: segment record, predicted position, short walk
-#pragma unroll
-            for (int i = 0; i < V; i += 2) {
-                SegRec rec[2];
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-                    if (s[i + x] >= 0) rec[x] = A.ix.rec[R.segnum + s[i + x]];
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-                    if (s[i + x] >= 0)
-                        found[i + x] = seg_find(A.ix.words + R.woff + rec[x].off + 1,
-                                                seg_len(R, (uint32_t)s[i + x], SEG), rec[x], v[i + x],
-                                                search != SEARCH_BINARY);
-            }
         } else {
             // LRC: binary search inside the segment
 #pragma unroll
@@ -688,6 +741,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                                            seg_len(R, (uint32_t)s[i], SEG), R, v[i]);
                 }
             }
+        }
         }
     }
 
