@@ -333,6 +333,42 @@ __device__ __forceinline__ uint32_t win_value(const uint32_t* col, uint32_t lb, 
     return ((uint32_t)(cw >> (bit & 31)) & ((1u << lb) - 1u)) + (uint32_t)(seg_y(j, r.slope, r.icpt) + r.base);
 }
 
+// Exact search of v in a random-access SegLRC / HS block, one candidate per
+// lane, every lane taking part: the words of the positions the block fit
+// allows ([p - rl - 1, p + rr + 1], up to kWinWords words) arrive in one load
+// round, are staged in buf (word k of lane l at buf[64k + l]) and searched
+// there.  A v outside the window's values, a window too wide or the binary
+// search mode fall back to the search in HBM.  Lanes with act false return
+// false (their data / r must still be a valid block).
+__device__ bool win_search(uint32_t* buf, const uint32_t* data, uint32_t len, const SegRec& r, uint32_t v, bool act,
+                           uint32_t search)
+{
+    const uint32_t ln = lane_id(), lb = desc_lb(r.desc);
+    const float fp = ((float)v - r.icpt) * __builtin_amdgcn_rcpf(r.slope) + 0.5f;
+    const int last = (int)len - 1;
+    const int pp = (int)fminf(fmaxf(fp, 0.0f), (float)max(last, 0));
+    const int wl = max(0, pp - (int)min(r.rl, 64u) - 1), wh = max(wl, min(last, pp + (int)min(r.rr, 64u) + 1));
+    const uint32_t w0 = ((uint32_t)wl * lb) >> 5;
+    const bool ok = search != SEARCH_BINARY && lb > 0 && len > 0 && ((((uint32_t)wh * lb) >> 5) - w0) + 2 <= kWinWords;
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < (int)kWinWords; ++k) buf[64 * k + ln] = data[w0 + k];
+    wave_sync();
+    if (!act || len == 0) return false;
+    if (!ok) return seg_find(data, len, r, v, search != SEARCH_BINARY);
+    const uint32_t vlo = win_value(buf + ln, lb, w0, (uint32_t)wl, r), vhi = win_value(buf + ln, lb, w0, (uint32_t)wh, r);
+    if (v < vlo) return wl > 0 && seg_bsearch(data, lb, r, 0, wl - 1, v);
+    if (v > vhi) return wh < last && seg_bsearch(data, lb, r, wh + 1, last, v);
+    int a0 = wl, a1 = wh;
+    while (a0 <= a1) {
+        const int m = (a0 + a1) >> 1;
+        const uint32_t q = win_value(buf + ln, lb, w0, (uint32_t)m, r);
+        if (q == v) return true;
+        if (q < v) a0 = m + 1; else a1 = m - 1;
+    }
+    return false;
+}
+
 // Branchless search of v in the sorted LDS run buf[0..n) (n uniform).
 __device__ __forceinline__ bool lds_contains(const uint32_t* buf, uint32_t n, uint32_t v)
 {
@@ -659,40 +695,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                 if (!__ballot(sc >= 0)) continue;
                 const uint32_t si = (uint32_t)(sc >= 0 ? sc : (int32_t)sf);
                 const SegRec rc = A.ix.rec[R.segnum + si];
-                const uint32_t len = seg_len(R, si, SEG), lb = desc_lb(rc.desc);
+                const uint32_t len = seg_len(R, si, SEG);
                 const uint32_t* data = A.ix.words + R.woff + rc.off + 1;
-                const float fp = ((float)vi - rc.icpt) * __builtin_amdgcn_rcpf(rc.slope) + 0.5f;
-                const int last = (int)len - 1;
-                const int pp = (int)fminf(fmaxf(fp, 0.0f), (float)last);
-                const int wl = max(0, pp - (int)min(rc.rl, 64u) - 1), wh = min(last, pp + (int)min(rc.rr, 64u) + 1);
-                const uint32_t w0 = ((uint32_t)wl * lb) >> 5;
-                const bool ok = search != SEARCH_BINARY && lb > 0 && ((((uint32_t)wh * lb) >> 5) - w0) + 2 <= kWinWords;
-                wave_sync();
-#pragma unroll
-                for (int k = 0; k < (int)kWinWords; ++k) W.buf[64 * k + ln] = data[w0 + k];
-                wave_sync();
-                bool f = false;
-                if (sc >= 0) {
-                    if (!ok) {
-                        f = seg_find(data, len, rc, vi, search != SEARCH_BINARY);
-                    } else {
-                        const uint32_t vlo = win_value(W.buf + ln, lb, w0, (uint32_t)wl, rc);
-                        const uint32_t vhi = win_value(W.buf + ln, lb, w0, (uint32_t)wh, rc);
-                        if (vi < vlo) {
-                            f = wl > 0 && seg_bsearch(data, lb, rc, 0, wl - 1, vi);
-                        } else if (vi > vhi) {
-                            f = wh < last && seg_bsearch(data, lb, rc, wh + 1, last, vi);
-                        } else {
-                            int a0 = wl, a1 = wh;
-                            while (a0 <= a1) {
-                                const int m = (a0 + a1) >> 1;
-                                const uint32_t q = win_value(W.buf + ln, lb, w0, (uint32_t)m, rc);
-                                if (q == vi) { f = true; break; }
-                                if (q < vi) a0 = m + 1; else a1 = m - 1;
-                            }
-                        }
-                    }
-                }
+                const bool f = win_search(W.buf, data, len, rc, vi, sc >= 0, search);
                 if (f) fmask |= 1u << i;
             }
 #pragma unroll
@@ -808,46 +813,60 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
     const uint32_t ln = lane_id();
     const uint32_t shift = (uint32_t)R.icpt, nb = R.nseg;
     const uint32_t* starts = A.ix.hash + R.aux;
-    uint32_t v[V];
-    bool found[V];
+    uint32_t fmask = 0;
+    if (R.n > 512u * nb) {
+        // large buckets (fixed shifts on long lists): their fits are too loose
+        // for a one-load window; two candidates per lane in flight, each
+        // predicting its position and walking / binary-searching in HBM
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-        const uint32_t e = ln + 64 * i;
-        found[i] = false;
-        v[i] = e < nlive ? W.cand[e] : kNone;
-    }
+        for (int i = 0; i < V; i += 2) {
+            SegRec rec[2];
+            uint32_t len[2], vv[2];
+            bool ok[2];
 #pragma unroll
-    for (int i = 0; i < V; i += 2) {
-        SegRec rec[2];
-        uint32_t len[2];
-        bool ok[2];
-#pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            const uint32_t b = v[i + x] >> shift;
-            ok[x] = ln + 64 * (i + x) < nlive && b < nb;
-            len[x] = 0;
-            if (ok[x]) {
-                rec[x] = A.ix.rec[R.segnum + b];
-                len[x] = starts[b + 1] - rec[x].head;
+            for (int x = 0; x < 2; ++x) {
+                const uint32_t e = ln + 64 * (i + x);
+                vv[x] = e < nlive ? W.cand[e] : kNone;
+                const uint32_t b = vv[x] >> shift;
+                ok[x] = e < nlive && b < nb;
+                len[x] = 0;
+                if (ok[x]) {
+                    rec[x] = A.ix.rec[R.segnum + b];
+                    len[x] = starts[b + 1] - rec[x].head;
+                }
             }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+                if (ok[x] && len[x] > 0 &&
+                    seg_find(A.ix.words + R.woff + rec[x].off + 1, len[x], rec[x], vv[x], search != SEARCH_BINARY))
+                    fmask |= 1u << (i + x);
         }
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-            if (ok[x] && len[x] > 0)
-                found[i + x] = seg_find(A.ix.words + R.woff + rec[x].off + 1, len[x], rec[x], v[i + x],
-                                        search != SEARCH_BINARY);
+    } else for (int i = 0; i < V; ++i) {
+        // buckets of ~256 docIDs: row by row, bucket, record, windowed search
+        const uint32_t e = ln + 64u * (uint32_t)i;
+        if (!__ballot(e < nlive)) break;  // rows fill in order
+        const uint32_t vi = e < nlive ? W.cand[e] : kNone;
+        const uint32_t b = vi >> shift;
+        const bool inb = e < nlive && b < nb;
+        if (!__ballot(inb)) continue;
+        const uint32_t bb = inb ? b : 0u;
+        const SegRec rc = A.ix.rec[R.segnum + bb];
+        const uint32_t len = starts[bb + 1] - rc.head;
+        const bool f = win_search(W.buf, A.ix.words + R.woff + rc.off + 1, len, rc, vi, inb && len > 0, search);
+        if (f) fmask |= 1u << i;
     }
-    uint32_t cnt = 0, pos[V];
-#pragma unroll
-    for (int i = 0; i < V; ++i) {
-        const uint64_t m = __ballot(found[i]);
-        pos[i] = cnt + below_me(m);
-        cnt += popc(m);
-    }
+    // order-preserving compaction, row by row (rows read before written)
+    uint32_t cnt = 0;
     wave_sync();
 #pragma unroll
-    for (int i = 0; i < V; ++i)
-        if (found[i]) W.cand[pos[i]] = v[i];
+    for (int i = 0; i < V; ++i) {
+        const bool fi = (fmask >> i) & 1u;
+        const uint64_t m = __ballot(fi);
+        const uint32_t e = ln + 64 * i;
+        const uint32_t vi = e < nlive ? W.cand[e] : kNone;
+        if (fi) W.cand[cnt + below_me(m)] = vi;
+        cnt += popc(m);
+    }
     wave_sync();
     return uni(cnt);
 }
