@@ -13,7 +13,8 @@
 //    shortest list, i.e. 4 segments of 128 or 2 of 256);
 //    a 256-thread workgroup hosts 4 independent waves, each with a private LDS
 //    slice, synchronised with wave barriers only.  Persistent launches claim
-//    units in chunks from one device counter.
+//    chunks of units for the workgroup from one device counter; its waves
+//    take them one at a time through a shared LDS word.
 //  * the unit's docIDs are decoded cooperatively into LDS and stay there as the
 //    candidate set; each longer list (ascending length) filters it.
 //  * per list, one gather of <= 64 segment heads around the window that the
@@ -37,7 +38,7 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr uint32_t kNone = 0xffffffffu;
-constexpr uint32_t kClaim = 4;        // units per work-queue claim (persistent launch)
+constexpr uint32_t kClaim = 8;        // units per workgroup claim on the device counter (persistent launch)
 constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
 constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative LRC/SegLRC probe
 constexpr uint32_t kBitmapPiece = 768; // docIDs decoded per load round of the bitmap path
@@ -911,25 +912,81 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     wave_sync();
 }
 
+// Persistent work queue.  One returning device-scope atomic per unit costs
+// 2.9 ms on the C2 batch (0.8 ms with a static schedule): they are served at a
+// low device-wide rate whatever the address.  So the device counter is
+// claimed per workgroup, kClaim units at a time, and the workgroup's waves
+// take units one by one from a 64-bit LDS word packing (end << 32 | next) of
+// the chunk in hand.  One wave at a time refills (the LDS lock: 0 free,
+// 1 refilling, 2 device counter exhausted); the others sleep until the new
+// chunk is in, so no wave ever holds more than one unit of the tail.  Chunks
+// of 8 keep the units in flight close together in the unit order (neighbours
+// share cache lines of the same lists): C2 batch 0.77 ms at 8, 0.84 at 16,
+// 1.21 at 64, and 0.90 at 4, where the device atomics dominate again.
+// Returns the next unit of this wave, or kNone once the queue is drained.
+struct ClaimLds {
+    unsigned long long chunk;
+    uint32_t lock;
+};
+
+__device__ uint32_t claim_unit(const ProbeArgs& A, ClaimLds* q)
+{
+    uint32_t u = kNone;
+    if (lane_id() == 0) {
+        for (;;) {
+            const unsigned long long old = atomicAdd(&q->chunk, 1ull);
+            if ((uint32_t)old < (uint32_t)(old >> 32)) {
+                u = (uint32_t)old;
+                break;
+            }
+            if (atomicCAS(&q->lock, 0u, 1u) == 0u) {
+                // this wave refills, unless a chunk came in since its take
+                const unsigned long long cur = __hip_atomic_load(&q->chunk, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((uint32_t)cur < (uint32_t)(cur >> 32)) {
+                    __hip_atomic_store(&q->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    continue;
+                }
+                const uint32_t g = atomicAdd(A.counter, kClaim);
+                if (g >= A.nunits) {
+                    __hip_atomic_store(&q->lock, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    break;
+                }
+                const uint32_t ge = min(g + kClaim, A.nunits);
+                __hip_atomic_exchange(&q->chunk, ((unsigned long long)ge << 32) | (g + 1u), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&q->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                u = g;
+                break;
+            }
+            uint32_t l;
+            while ((l = __hip_atomic_load(&q->lock, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) == 1u)
+                __builtin_amdgcn_s_sleep(2);
+            if (l == 2u) break;
+        }
+    }
+    return uni(__shfl(u, 0));
+}
+
 // Eight waves per SIMD for the random-access codecs (64 VGPRs, 20 KB of LDS
 // per workgroup): the probe is bound by load latency, so resident waves pay
 // as long as the spilled registers stay few (C2 batch: 7 waves 0.88 ms,
-// 8 waves 0.85 ms with 32 spilled bytes per lane).
-template <int CODEC, int SEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : 8, 8))) void k_probe(ProbeArgs A)
+// 8 waves 0.85 ms with 32 spilled bytes per lane).  The persistent kernel's
+// 16-byte claim record takes it to 7 workgroups per CU: eight would need
+// 8 x 20,496 bytes of the 160 KiB.
+template <int CODEC, int SEG, bool PERSIST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : (PERSIST ? 7 : 8), PERSIST ? 7 : 8))) void k_probe(ProbeArgs A)
 {
     __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);
     uint32_t* L = lds + wid * wave_lds_words<CODEC>();
-    if (A.persistent) {
-        for (;;) {
-            uint32_t x = 0;
-            if (lane_id() == 0) x = atomicAdd(A.counter, kClaim);
-            const uint32_t u0 = uni(__shfl(x, 0));
-            if (u0 >= A.nunits) break;
-            const uint32_t u1 = min(u0 + kClaim, A.nunits);
-            for (uint32_t u = u0; u < u1; ++u) run_unit<CODEC, SEG>(A, u, L);
+    if constexpr (PERSIST) {
+        __shared__ ClaimLds q;
+        if (threadIdx.x == 0) {
+            q.chunk = 0;
+            q.lock = 0;
         }
+        __syncthreads();
+        for (uint32_t u; (u = claim_unit(A, &q)) != kNone;) run_unit<CODEC, SEG>(A, u, L);
     } else {
         const uint32_t u = blockIdx.x * kWaves + wid;
         if (u < A.nunits) run_unit<CODEC, SEG>(A, u, L);
@@ -1098,10 +1155,17 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* 
 template <int CODEC>
 hipError_t probe_for_seg(const ProbeArgs& a, uint32_t grid, hipStream_t s)
 {
-    if (a.ix.seg == 256)
-        hipLaunchKernelGGL((k_probe<CODEC, 256>), dim3(grid), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_probe<CODEC, 128>), dim3(grid), dim3(256), 0, s, a);
+    if (a.persistent) {
+        if (a.ix.seg == 256)
+            hipLaunchKernelGGL((k_probe<CODEC, 256, true>), dim3(grid), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_probe<CODEC, 128, true>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+        if (a.ix.seg == 256)
+            hipLaunchKernelGGL((k_probe<CODEC, 256, false>), dim3(grid), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_probe<CODEC, 128, false>), dim3(grid), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -1121,8 +1185,8 @@ int occ_for(uint32_t seg)
 {
     int nb = 0;
     hipError_t e = (seg == 256)
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 256>), 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 128>), 256, 0);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 256, true>), 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 128, true>), 256, 0);
     return (e == hipSuccess && nb > 0) ? nb : 4;
 }
 
