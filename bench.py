@@ -27,7 +27,7 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 # k_probe<SegLRC,128> HBM bytes per launch on the C2 batch with the tuned knobs, in KiB: rocprofv3
 # FETCH_SIZE (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
-TRAFFIC_KIB = 1573484.0
+TRAFFIC_KIB = 1437231.0
 C5_SCALE = 5.8e9  # C of df_k = min(2^25, C / k^1.1): about 25.4G postings, 40 GB resident
 META = {0: 8, 1: 28, 2: 32, 3: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD
 
@@ -281,8 +281,9 @@ def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
     """C3: the C2 lists in hash buckets on the top docID bits (shift 13 = the top
     12 bits of 2^25; also 12, 14 and the adaptive ~256-per-bucket width), 10,000
     2-term queries with long:short ratios 10^U(1,4).  Per bucket width: search
-    knobs autotuned per ratio class, the whole batch as one persistent launch
-    replayed from a hipGraph, and every class's sub-batch timed; the bucket
+    knobs autotuned per ratio class, the whole batch as one launch (persistent
+    work queue or one wave per unit, as the autotuner chose) replayed from a
+    hipGraph, and every class's sub-batch timed; the bucket
     width is then chosen per ratio class and the batch run routed that way."""
     qs = W.ratio_queries(df, 10_000, 3)
     sample = W.ratio_queries(df, 4000, 33)
@@ -296,12 +297,11 @@ def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
         dix = ctx.upload(hs)
         ctx.set_option(ea.OPT_GRAPH, 0)
         tune = ctx.autotune(dix, sample, reps=2)
-        row = {"device_bytes": dix.device_bytes(), "search": tune["search"]}
-        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        row = {"device_bytes": dix.device_bytes(), "search": tune["search"], "persistent": tune["persistent"]}
         ctx.set_option(ea.OPT_GRAPH, 1)
         dt, _, inv = time_batch(ctx, dix, qs, steps, 2)
         cum, res = ctx.dtohTransfer(ctx._bi)
-        row["graph_persistent_qps"] = len(qs) * steps / dt
+        row["graph_qps"] = len(qs) * steps / dt
         row["graph_step_ms"] = float(np.median(inv))
         row["matches_seglrc"] = bool(np.array_equal(cum, rcum) and np.array_equal(res, rres))
         ctx.set_option(ea.OPT_GRAPH, 0)
@@ -323,7 +323,7 @@ def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
     out["width_per_class"] = choice
     out["routed_probe_ms"] = routed_ms
     out["routed_qps"] = len(qs) / (routed_ms * 1e-3)
-    out["fixed_shift13_qps"] = keep[13][2]["graph_persistent_qps"]
+    out["fixed_shift13_qps"] = keep[13][2]["graph_qps"]
     for dix, hs, _ in keep.values():
         dix.close()
     ctx.set_tuning(None)

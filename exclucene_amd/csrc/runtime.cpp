@@ -707,8 +707,8 @@ int enqueue_invoke(exc_ctx* c, bool timed)
     } else {
         grid = std::max<uint32_t>(1, (c->nunits + wpb - 1) / wpb);
     }
-    if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     if (c->nunits) HIP_TRY(launch_unit_prologue(a, c->stream));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     if (c->nunits) HIP_TRY(launch_probe(a, grid, c->stream));
     if (timed) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     ScanArgs s;
