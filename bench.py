@@ -16,7 +16,9 @@ baseline (the oracle, on the host cores) on a bounded sample of the batch.
 import argparse
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -202,6 +204,33 @@ def decode_leg(ea, ctx, dix, built, docs, reps=5):
             "algorithmic_bytes": int(alg), "achieved_GBps": alg / (ms.mean() * 1e-3) / 1e9,
             "frac_of_peak": alg / (ms.mean() * 1e-3) / 1e9 / HBM_PEAK_GBS, "docids_per_s": total / (ms.mean() * 1e-3),
             "matches_lists": same}
+
+
+def loader_leg(ea, ctx, built, queries, ref_cum, ref_res, reps=3):
+    """The C2 SegLRC-128 index written as the reference's index files, then
+    loaded with exc_index_load_files (mapped files, validation, staged upload
+    into HBM) with the page cache warm from the write; file bytes per second
+    of the whole load, and the loaded index must answer the batch exactly as
+    the uploaded one."""
+    d = tempfile.mkdtemp(prefix="exc_load_")
+    try:
+        built.save(d, "c2")
+        nbytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        ts, lix = [], None
+        for _ in range(reps):
+            if lix is not None:
+                lix.close()
+            t0 = time.perf_counter()
+            lix = ctx.load_files(ea.SEGLRC, 128, d, "c2")
+            ts.append(time.perf_counter() - t0)
+        cum, res = ctx.intersect(lix, queries)
+        same = bool(np.array_equal(cum, ref_cum) and np.array_equal(res, ref_res))
+        lix.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    t = float(np.median(ts))
+    return {"files": "SegLRC-128 .ind1/.ind2/.segNum/.segOffset/.segHead/.regression_info(_list)/.ind2_distance_base",
+            "file_bytes": int(nbytes), "load_s": t, "GBps": nbytes / t / 1e9, "matches_upload": same}
 
 
 def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
@@ -404,6 +433,7 @@ def main():
     ap.add_argument("--nq", type=int, default=1000)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
+    ap.add_argument("--no-load", action="store_true", help="skip the index-file load leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the plain-method single-query leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the hash-bucket leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 1M-term, 35G-posting serving configuration")
@@ -443,7 +473,10 @@ def main():
     log("SegLRC-128 built: %.1f MB compressed, %.1f s" % (seg.compressed_bytes() / 1e6, time.perf_counter() - t0))
 
     ctx = ea.Context(0)
+    t0 = time.perf_counter()
     dix = ctx.upload(seg)
+    upload_s = time.perf_counter() - t0
+    log("uploaded %.1f MB of index arrays in %.3f s" % (dix.device_bytes() / 1e6, upload_s))
 
     models = None
     if args.models != "none":
@@ -524,6 +557,13 @@ def main():
     if not args.no_decode:
         extra["decode"] = decode_leg(ea, ctx, dix, seg, docs)
         log("decode: %s" % extra["decode"])
+
+    extra["upload"] = {"device_bytes": int(dix.device_bytes()), "s": upload_s,
+                       "GBps": dix.device_bytes() / upload_s / 1e9}
+    if not args.no_load:
+        extra["load_files"] = loader_leg(ea, ctx, seg, queries, cum, res)
+        headline_opts()
+        log("load files: %s" % extra["load_files"])
 
     if not args.no_serve:
         extra["serving"] = serving_leg(ea, ctx, dix, queries, res, cum)

@@ -25,7 +25,7 @@ SEARCH_BINARY, SEARCH_FIT, SEARCH_LEARNED = 0, 1, 2
 
 EXPORTS = [
     "exc_ctx_create", "exc_ctx_destroy", "exc_last_error", "exc_ctx_set_option", "exc_ctx_last_times",
-    "exc_index_upload", "exc_index_load_files", "exc_index_free", "exc_index_device_bytes",
+    "exc_index_upload", "exc_index_load_files", "exc_index_free", "exc_index_device_bytes", "exc_index_lengths",
     "exc_CPUPreprocess", "exc_htodTransfer", "exc_kernelInvoke", "exc_dtohTransfer", "exc_verify",
     "exc_intersect", "exc_plan", "exc_decompress",
     "exc_build_index", "exc_built_view", "exc_built_free", "exc_index_save_files",
@@ -130,6 +130,7 @@ def lib():
                                            ctypes.POINTER(_vp)]
         L.exc_index_free.argtypes = [_vp]
         L.exc_index_device_bytes.argtypes = [_vp]
+        L.exc_index_lengths.argtypes = [_vp, _vp, _u32, ctypes.POINTER(_u32)]
         L.exc_CPUPreprocess.argtypes = [_vp, _vp, ctypes.POINTER(_vp), _vp, _u64, ctypes.POINTER(BatchInfo)]
         L.exc_htodTransfer.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
         L.exc_kernelInvoke.argtypes = [_vp, ctypes.POINTER(BatchInfo)]
@@ -341,7 +342,11 @@ class Context:
     def load_files(self, codec, seg, directory, name):
         h = _vp()
         _check(lib().exc_index_load_files(self._h, codec, seg, directory.encode(), name.encode(), ctypes.byref(h)))
-        return DeviceIndex(self, h)
+        n = _u32()
+        _check(lib().exc_index_lengths(h, None, 0, ctypes.byref(n)))
+        lengths = np.zeros(max(int(n.value), 1), np.uint32)
+        _check(lib().exc_index_lengths(h, _p(lengths), len(lengths), ctypes.byref(n)))
+        return DeviceIndex(self, h, lengths=lengths[:int(n.value)])
 
     # -- reference stage API
     def CPUPreprocess(self, index, stream_bytes, pos, threshold):

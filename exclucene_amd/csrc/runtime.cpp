@@ -26,6 +26,11 @@
 #include <string>
 #include <vector>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "exc_format.h"
 #include "exc_internal.h"
 #include "exc_model.h"
@@ -162,6 +167,56 @@ template <class T>
 T* dptr(DevBuf& b) { return static_cast<T*>(b.p); }
 
 // upload `bytes` of src followed by kPadWords zero words
+// Host-to-device copy of a large host array (a mapped index file, for one)
+// through pinned staging.  kStreamThreads host threads each own two staging
+// buffers; thread t fills chunks t, t + T, ... from the source and queues each
+// chunk's DMA on the context stream, so reading the next chunks (page faults
+// of a mapped file included) overlaps the transfers of the previous ones.
+constexpr size_t kStreamChunk = size_t(8) << 20;
+constexpr int kStreamThreads = 4;
+
+int stream_to_device(exc_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    const size_t nchunk = (bytes + kStreamChunk - 1) / kStreamChunk;
+    const int T = (int)std::min<size_t>(kStreamThreads, nchunk);
+    std::vector<void*> stage(2 * T, nullptr);
+    std::vector<hipEvent_t> ev(2 * T, nullptr);
+    hipError_t err = hipSuccess;
+    for (size_t i = 0; i < stage.size() && err == hipSuccess; ++i) {
+        err = hipHostMalloc(&stage[i], kStreamChunk, hipHostMallocDefault);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+    }
+    if (err == hipSuccess) {
+        std::vector<hipError_t> terr(T, hipSuccess);
+        std::atomic<bool> stop{false};
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                hipError_t e = hipSetDevice(c->device);
+                for (size_t k = t, j = 0; e == hipSuccess && k < nchunk && !stop; k += T, ++j) {
+                    const int b = 2 * t + (int)(j & 1);
+                    if (j >= 2 && (e = hipEventSynchronize(ev[b])) != hipSuccess) break;  // its last DMA is done
+                    const size_t off = k * kStreamChunk, n = std::min(kStreamChunk, bytes - off);
+                    std::memcpy(stage[b], static_cast<const char*>(src) + off, n);
+                    e = hipMemcpyAsync(static_cast<char*>(dst) + off, stage[b], n, hipMemcpyHostToDevice, c->stream);
+                    if (e == hipSuccess) e = hipEventRecord(ev[b], c->stream);
+                }
+                if (e != hipSuccess) stop = true;
+                terr[t] = e;
+            });
+        for (auto& x : th) x.join();
+        err = hipStreamSynchronize(c->stream);
+        for (hipError_t e : terr)
+            if (e != hipSuccess) err = e;
+    }
+    for (size_t i = 0; i < stage.size(); ++i) {
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (stage[i]) (void)hipHostFree(stage[i]);
+    }
+    if (err != hipSuccess) return set_error(EXC_ERR_DEVICE, "staged upload: %s", hipGetErrorString(err));
+    return EXC_OK;
+}
+
 int upload_padded(exc_index* ix, const void* src, size_t bytes, const void** out)
 {
     DevBuf b;
@@ -169,7 +224,11 @@ int upload_padded(exc_index* ix, const void* src, size_t bytes, const void** out
     if (int rc = dev_alloc(ix->ctx, b, total)) return rc;
     ix->bufs.push_back(b);
     ix->bytes += b.bytes;
-    if (bytes) HIP_TRY(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    if (bytes > kStreamChunk) {
+        if (int rc = stream_to_device(ix->ctx, b.p, src, bytes)) return rc;
+    } else if (bytes) {
+        HIP_TRY(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemset(static_cast<char*>(b.p) + bytes, 0, total - bytes));
     *out = b.p;
     return EXC_OK;
@@ -472,20 +531,59 @@ extern "C" void exc_index_free(exc_index* ix)
 
 extern "C" uint64_t exc_index_device_bytes(const exc_index* ix) { return ix ? ix->bytes : 0; }
 
-static int read_file(const std::string& p, std::vector<char>& out, bool optional)
+extern "C" int exc_index_lengths(const exc_index* ix, uint32_t* out, uint32_t cap, uint32_t* nterms)
 {
-    FILE* f = std::fopen(p.c_str(), "rb");
-    if (!f) {
-        if (optional) { out.clear(); return EXC_OK; }
-        return set_error(EXC_ERR_IO, "cannot open %s", p.c_str());
+    if (!ix || !nterms) return set_error(EXC_ERR_ARG, "exc_index_lengths: null argument");
+    *nterms = ix->nterms;
+    if (!out) return EXC_OK;
+    if (cap < ix->nterms) return set_error(EXC_ERR_CAPACITY, "exc_index_lengths: %u lists, room for %u", ix->nterms, cap);
+    std::copy(ix->n.begin(), ix->n.end(), out);
+    return EXC_OK;
+}
+
+// A read-only private mapping of one index file: nothing is read until the
+// validation or the staged upload touches it, and a 40 GB .ind2 never needs
+// 40 GB of heap.
+struct MappedFile {
+    const char* p = nullptr;
+    size_t n = 0;
+    MappedFile() = default;
+    MappedFile(const MappedFile&) = delete;
+    MappedFile& operator=(const MappedFile&) = delete;
+    ~MappedFile()
+    {
+        if (p) munmap(const_cast<char*>(p), n);
     }
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    out.resize(sz > 0 ? (size_t)sz : 0);
-    size_t r = sz > 0 ? std::fread(out.data(), 1, (size_t)sz, f) : 0;
-    std::fclose(f);
-    return r == out.size() ? EXC_OK : set_error(EXC_ERR_IO, "short read on %s", p.c_str());
+    const char* data() const { return p; }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+};
+
+static int map_file(const std::string& path, MappedFile& out, bool optional)
+{
+    const int fd = open(path.c_str(), O_RDONLY);
+    if (fd < 0) {
+        if (optional) return EXC_OK;
+        return set_error(EXC_ERR_IO, "cannot open %s", path.c_str());
+    }
+    struct stat st;
+    if (fstat(fd, &st) != 0) {
+        close(fd);
+        return set_error(EXC_ERR_IO, "cannot stat %s", path.c_str());
+    }
+    out.n = (size_t)st.st_size;
+    if (out.n) {
+        void* m = mmap(nullptr, out.n, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m == MAP_FAILED) {
+            close(fd);
+            out.n = 0;
+            return set_error(EXC_ERR_IO, "cannot map %s", path.c_str());
+        }
+        (void)madvise(m, out.n, MADV_SEQUENTIAL);
+        out.p = static_cast<const char*>(m);
+    }
+    close(fd);
+    return EXC_OK;
 }
 
 extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const char* dir, const char* name,
@@ -493,18 +591,18 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
 {
     if (!c || !dir || !name || !out) return set_error(EXC_ERR_ARG, "exc_index_load_files: null argument");
     std::string p = std::string(dir) + "/" + name;
-    std::vector<char> ind1, ind2, segnum, segoff, head, median, reg, regl, base, hoff, hinfo;
+    MappedFile ind1, ind2, segnum, segoff, head, median, reg, regl, base, hoff, hinfo;
     int rc;
     const bool hs = codec == EXC_HS_SEGLRC;
-    if ((rc = read_file(p + ".ind1", ind1, false)) || (rc = read_file(p + ".ind2", ind2, false)) ||
-        (rc = read_file(p + ".segNum", segnum, false)) || (rc = read_file(p + ".segOffset", segoff, false)))
+    if ((rc = map_file(p + ".ind1", ind1, false)) || (rc = map_file(p + ".ind2", ind2, false)) ||
+        (rc = map_file(p + ".segNum", segnum, false)) || (rc = map_file(p + ".segOffset", segoff, false)))
         return rc;
-    if (!hs && (rc = read_file(p + ".segHead", head, false))) return rc;
-    if ((rc = read_file(p + ".segMedian", median, true))) return rc;
-    if (codec != EXC_PARAPFD && (rc = read_file(p + ".regression_info", reg, false))) return rc;
-    if ((rc = read_file(p + ".regression_info_list", regl, true))) return rc;
-    if ((codec == EXC_SEGLRC || hs) && (rc = read_file(p + ".ind2_distance_base", base, false))) return rc;
-    if (hs && ((rc = read_file(p + ".HashSeg_offset", hoff, false)) || (rc = read_file(p + ".HashSeg_info", hinfo, false))))
+    if (!hs && (rc = map_file(p + ".segHead", head, false))) return rc;
+    if ((rc = map_file(p + ".segMedian", median, true))) return rc;
+    if (codec != EXC_PARAPFD && (rc = map_file(p + ".regression_info", reg, false))) return rc;
+    if ((rc = map_file(p + ".regression_info_list", regl, true))) return rc;
+    if ((codec == EXC_SEGLRC || hs) && (rc = map_file(p + ".ind2_distance_base", base, false))) return rc;
+    if (hs && ((rc = map_file(p + ".HashSeg_offset", hoff, false)) || (rc = map_file(p + ".HashSeg_info", hinfo, false))))
         return rc;
     if (ind1.size() % sizeof(exc_term_ind1)) return set_error(EXC_ERR_FORMAT, ".ind1 size is not a multiple of 16");
     exc_index_host h;

@@ -128,6 +128,10 @@ int exc_index_load_files(exc_ctx *ctx, int codec, uint32_t seg_size, const char 
                          const char *name, exc_index **out);
 void exc_index_free(exc_index *ix);
 uint64_t exc_index_device_bytes(const exc_index *ix);
+/* Number of lists and, when out is given (cap >= that number), each list's
+ * length: the .ind1 m_urlcount column the reference's host side keeps to plan
+ * batches (GC/LRC/LRC_Intersection/LRC_Intersection.cu:126-246). */
+int exc_index_lengths(const exc_index *ix, uint32_t *out, uint32_t cap, uint32_t *nterms);
 
 /* ---- batch stages, one call per reference stage
  *      (GC/LRC/LRC_Intersection/LRC_Intersection.cu:126,249,256,348,354).
