@@ -13,6 +13,8 @@
 //   HS table    GC/HS_SegLRC/HS_SegLRC_Generator/HS_SegLRC_Generator.cu:503-594
 //   packing     GC/LRC/LRC_Compression/main.cpp:163-199, GI/ParaPFor.cpp:56-94,
 //               GC/ParaPFD/ParaPFD_Compression/main.cpp:151-189,
+//               GI/NewPFor.cpp:54-99, GI/Simple16.h:13-48,
+//               GC/NewPFD/NewPFD_Compression/main.cpp:151-189,
 //               GC/HS_SegLRC/HS_SegLRC_Compression/main.cpp:149-178
 // Lists are encoded in parallel (static partition over std::thread workers);
 // the result is identical for any thread count.
@@ -106,6 +108,71 @@ bool pfor_block(const uint32_t* v, uint32_t n, float frac, std::vector<uint32_t>
     return true;
 }
 
+// ---------------------------------------------------------------- NewPFor
+
+// Simple16 words for vals[0..n) appended to out: at each position the first
+// selector (in order 0..15) whose fields hold the next min(count, left)
+// values (GI/Simple16.h:13-48).  False if a value needs more than 28 bits,
+// which no selector holds (the reference's encoder never returns then).
+bool s16_encode(const uint32_t* vals, uint32_t n, std::vector<uint32_t>& out)
+{
+    for (uint32_t i = 0; i < n;) {
+        bool done = false;
+        for (uint32_t k = 0; k < 16 && !done; ++k) {
+            const uint32_t L = s16_layout(k);
+            uint32_t width[28], cnt = 0;
+            for (uint32_t r = 0; r < 3; ++r)
+                for (uint32_t q = 0; q < ((L >> (10 * r)) & 31u); ++q) width[cnt++] = (L >> (10 * r + 5)) & 31u;
+            const uint32_t vol = std::min(cnt, n - i);
+            uint32_t word = k << 28, sh = 0, j = 0;
+            for (; j < vol && (vals[i + j] >> width[j]) == 0; sh += width[j], ++j) word |= vals[i + j] << sh;
+            if (j == vol) {
+                out.push_back(word);
+                i += vol;
+                done = true;
+            }
+        }
+        if (!done) return false;
+    }
+    return true;
+}
+
+// One NewPFor block appended to out (GI/NewPFor.cpp:54-99): b by the ParaPFor
+// rule, low parts packed, exceptions as one Simple16 stream of position
+// deltas then high parts.  When a high part would need more than 28 bits b
+// grows until none does (the reference's encoder would not terminate).
+bool npfd_block(const uint32_t* v, uint32_t n, float frac, std::vector<uint32_t>& out)
+{
+    uint32_t b = pfor_best_b(v, n, frac), mx = 0;
+    for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, v[i]);
+    b = std::max(b, bits_of(mx) > 28 ? bits_of(mx) - 28 : 0u);
+    if (b > 31) return false;
+    std::vector<uint32_t> low(v, v + n), all;
+    std::vector<uint32_t> pos, hi;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (bits_of(v[i]) > b) {
+            pos.push_back(i);
+            hi.push_back(v[i] >> b);
+            low[i] = v[i] & ((1u << b) - 1u);
+        }
+    }
+    const uint32_t en = (uint32_t)pos.size();
+    if (en > 65535) return false;
+    const size_t at = out.size();
+    out.resize(at + 1 + words_for(n, b));
+    out[at] = (b << 26) | en;
+    put_bits(out.data() + at + 1, low.data(), n, b);
+    if (en) {
+        all.resize(2 * (size_t)en);
+        for (uint32_t i = 0; i < en; ++i) {
+            all[i] = i ? pos[i] - pos[i - 1] - 1 : pos[0];
+            all[en + i] = hi[i];
+        }
+        if (!s16_encode(all.data(), 2 * en, out)) return false;
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------- fits
 
 exc_lrc_info lrc_fit(const uint32_t* L, uint32_t n)
@@ -193,7 +260,7 @@ void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32
             vd[0] = 0;
         }
     }
-    if (codec == EXC_PARAPFD && n) {
+    if ((codec == EXC_PARAPFD || codec == EXC_NEWPFD) && n) {
         vd[0] = L[0];
         for (uint32_t j = 1; j < n; ++j) vd[j] = L[j] - L[j - 1];
     }
@@ -216,18 +283,20 @@ void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32
     uint32_t nseg = (n + seg - 1) / seg;
     o.seg_offset.resize(nseg);
     o.seg_head.resize(nseg);
-    if (codec == EXC_PARAPFD) o.seg_median.resize(nseg);
+    if (codec == EXC_PARAPFD || codec == EXC_NEWPFD) o.seg_median.resize(nseg);
     if (codec == EXC_SEGLRC) { o.seg_info.resize(nseg); o.base.resize(nseg); }
     for (uint32_t s = 0; s < nseg; ++s) {
         uint32_t st = s * seg, len = std::min(seg, n - st);
         o.seg_head[s] = L[st];
-        if (codec == EXC_PARAPFD) o.seg_median[s] = L[st + len / 2];
+        if (codec == EXC_PARAPFD || codec == EXC_NEWPFD) o.seg_median[s] = L[st + len / 2];
         if (codec == EXC_SEGLRC) {
             o.seg_info[s] = seg_fit(L + st, len);
             o.base[s] = seg_residuals(L + st, len, o.seg_info[s], vd.data() + st);
         }
         o.seg_offset[s] = (uint32_t)o.words.size();
-        if (!pfor_block(vd.data() + st, len, frac, o.words)) { o.ok = false; return; }
+        const bool ok = codec == EXC_NEWPFD ? npfd_block(vd.data() + st, len, frac, o.words)
+                                            : pfor_block(vd.data() + st, len, frac, o.words);
+        if (!ok) { o.ok = false; return; }
     }
 }
 
@@ -291,7 +360,7 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
                                int nthreads, exc_built** out)
 {
     if (!out || !list_off || (nterms && !docs)) return set_error(EXC_ERR_ARG, "exc_build_index: null argument");
-    if (codec < EXC_LRC || codec > EXC_PARAPFD) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
+    if (codec < EXC_LRC || codec > EXC_NEWPFD) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
     if (codec != EXC_HS_SEGLRC && (seg < 64 || seg > 1024 || (seg & 63)))
         return set_error(EXC_ERR_ARG, "exc_build_index: segment size must be a multiple of 64 in [64,1024]");
     if (hs_target == 0) hs_target = 256;
@@ -333,7 +402,7 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
     b->ind2.resize(woff[nterms]);
     b->seg_offset.resize(S);
     if (codec != EXC_HS_SEGLRC) b->seg_head.resize(S);
-    if (codec == EXC_PARAPFD) b->seg_median.resize(S);
+    if (codec == EXC_PARAPFD || codec == EXC_NEWPFD) b->seg_median.resize(S);
     if (codec == EXC_SEGLRC || codec == EXC_HS_SEGLRC) { b->seg_info.resize(S); b->base.resize(S); }
     if (codec == EXC_SEGLRC) b->list_info.resize(nterms);
     if (codec == EXC_LRC) b->lrc.resize(nterms);

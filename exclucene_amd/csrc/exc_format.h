@@ -9,6 +9,10 @@
 //          nvcc contracts j*s+b to one fused multiply-add, which is pinned here
 //          as an explicit fmaf on both sides.
 //   ParaPFor descriptor lb<<26 | eb<<21 | hb<<16 | en (GI/ParaPFor.cpp:88-91).
+//   NewPFor descriptor b<<26 | en, then the b-bit low parts, then the
+//          exceptions as one Simple16 stream [pos0, pos_i - pos_(i-1) - 1 ...,
+//          high parts ...] (GI/NewPFor.cpp:54-99); lb and en sit where the
+//          ParaPFor descriptor keeps them.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -29,5 +33,56 @@ EXC_HD uint32_t desc_hb(uint32_t d) { return (d >> 16) & 31u; }
 EXC_HD uint32_t desc_en(uint32_t d) { return d & 65535u; }
 EXC_HD uint32_t words_for(uint32_t n, uint32_t b) { return (uint32_t)(((uint64_t)n * b + 31) >> 5); }
 EXC_HD uint32_t bits_of(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
+
+// Simple16 (Zhang, Long and Suel, WWW 2008; the tables of GI/Simple16.h:13-48):
+// a word is a 4-bit selector k in bits 28-31 and fields packed from bit 0 up;
+// layout k is up to three runs of equal-width fields.  Packed per selector:
+// five bits each of (count, width) for runs 0, 1, 2.
+constexpr uint32_t s16_runs(uint32_t c0, uint32_t w0, uint32_t c1 = 0, uint32_t w1 = 0, uint32_t c2 = 0,
+                            uint32_t w2 = 0)
+{
+    return c0 | w0 << 5 | c1 << 10 | w1 << 15 | c2 << 20 | w2 << 25;
+}
+
+EXC_HD uint32_t s16_layout(uint32_t k)
+{
+    switch (k & 15u) {
+    case 0: return s16_runs(28, 1);
+    case 1: return s16_runs(7, 2, 14, 1);
+    case 2: return s16_runs(7, 1, 7, 2, 7, 1);
+    case 3: return s16_runs(14, 1, 7, 2);
+    case 4: return s16_runs(14, 2);
+    case 5: return s16_runs(1, 4, 8, 3);
+    case 6: return s16_runs(1, 3, 4, 4, 3, 3);
+    case 7: return s16_runs(7, 4);
+    case 8: return s16_runs(4, 5, 2, 4);
+    case 9: return s16_runs(2, 4, 4, 5);
+    case 10: return s16_runs(3, 6, 2, 5);
+    case 11: return s16_runs(2, 5, 3, 6);
+    case 12: return s16_runs(4, 7);
+    case 13: return s16_runs(1, 10, 2, 9);
+    case 14: return s16_runs(2, 14);
+    default: return s16_runs(1, 28);
+    }
+}
+
+// number of fields of a Simple16 word (all are decoded, used or not)
+EXC_HD uint32_t s16_count(uint32_t word)
+{
+    const uint32_t L = s16_layout(word >> 28);
+    return (L & 31u) + ((L >> 10) & 31u) + ((L >> 20) & 31u);
+}
+
+// the fields of one Simple16 word into out[0 .. min(count, lim))
+EXC_HD void s16_unpack(uint32_t word, uint32_t* out, uint32_t lim)
+{
+    const uint32_t L = s16_layout(word >> 28);
+    uint32_t sh = 0, o = 0;
+    for (uint32_t r = 0; r < 3; ++r) {
+        const uint32_t c = (L >> (10 * r)) & 31u, w = (L >> (10 * r + 5)) & 31u;
+        for (uint32_t i = 0; i < c; ++i, ++o, sh += w)
+            if (o < lim) out[o] = (word >> sh) & ((1u << w) - 1u);
+    }
+}
 
 }  // namespace exc

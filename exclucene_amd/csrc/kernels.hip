@@ -126,9 +126,12 @@ __device__ __forceinline__ uint32_t seg_len(const TermRec& R, uint32_t s, uint32
 
 // ------------------------------------------------------------ block decode
 
+// d-gap codecs: blocks decode by exception patch + prefix sum from segHead
+constexpr bool gap_codec(int codec) { return codec == EXC_PARAPFD || codec == EXC_NEWPFD; }
+
 // Raw lb-bit fields of one block for this lane: strided positions
 // (j = lane + 64 t) for the random-access codecs, consecutive (j = lane*V + t)
-// for ParaPFD whose prefix sum runs along a lane first.
+// for the d-gap codecs whose prefix sum runs along a lane first.
 template <int CODEC, int SEG>
 __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, uint32_t lb, uint32_t len,
                                             uint32_t (&f)[SEG / 64])
@@ -137,7 +140,7 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
     const uint32_t ln = lane_id();
 #pragma unroll
     for (int t = 0; t < V; ++t) {
-        const uint32_t j = (CODEC == EXC_PARAPFD) ? ln * V + t : ln + 64 * t;
+        const uint32_t j = gap_codec(CODEC) ? ln * V + t : ln + 64 * t;
         // every lane loads (from a clamped position inside the block) so that
         // all loads of a group are in flight before the first wait
         const uint32_t x = bits_at(data, lb, len ? min(j, len - 1u) : 0u);
@@ -145,11 +148,43 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
     }
 }
 
-// Finish decoding block s (fields already loaded) into dst[0..len).
+// The exceptions of a NewPFor block patched into dst[0..len) (low parts
+// already there).  The reference decodes the Simple16 stream and patches in
+// thread 0 alone (GC/NewPFD/NewPFD_Decompression/NewPFD_Decompression_kernel.cu:420-447);
+// here each lane takes one word of the stream, a wave scan of the words'
+// field counts places their fields in xs[0 .. 2 en), a scan of the position
+// deltas gives every exception its position, and the patch is lane-parallel.
+// Stream words are read 64 at a time: past the stream's end a round reads at
+// most 63 words, inside the list's successors or the array's zero padding.
+__device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint32_t en, uint32_t lb, uint32_t len,
+                                           uint32_t* xs, uint32_t* dst)
+{
+    const uint32_t ln = lane_id(), need = 2 * en;
+    for (uint32_t w0 = 0, got = 0; got < need; w0 += 64) {
+        const uint32_t word = sp[w0 + ln];
+        const uint32_t c = s16_count(word);
+        const uint32_t incl = wave_incl_scan(c);
+        const uint32_t at = got + incl - c;
+        if (at < need) s16_unpack(word, xs + at, need - at);
+        got += uni(__shfl(incl, 63));
+    }
+    wave_sync();
+    // position i = all[0] + sum over m = 1..i of (all[m] + 1)  (GI/NewPFor.cpp:119-124)
+    for (uint32_t e0 = 0, carry = 0; e0 < en; e0 += 64) {
+        const uint32_t e = e0 + ln;
+        const uint32_t d = e < en ? xs[e] + (e ? 1u : 0u) : 0u;
+        const uint32_t p = wave_incl_scan(d) + carry;
+        if (e < en && p < len) dst[p] += xs[en + e] << lb;
+        carry = uni(__shfl(p, 63));
+    }
+}
+
+// Finish decoding block s (fields already loaded) into dst[0..len); xs is
+// the NewPFor exception scratch (2 SEG words).
 template <int CODEC, int SEG>
 __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const SegRec& rec, uint32_t len,
                                              const uint32_t* __restrict__ data, uint32_t (&f)[SEG / 64],
-                                             uint32_t* dst)
+                                             uint32_t* dst, uint32_t* xs)
 {
     constexpr int V = SEG / 64;
     const uint32_t ln = lane_id();
@@ -169,8 +204,9 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
             if (j < len) dst[j] = f[t] + (uint32_t)(seg_y(j, rec.slope, rec.icpt) + rec.base);
         }
     } else {
-        // ParaPFD: patch exceptions, segHead, inclusive scan
-        // (GC/ParaPFD/ParaPFD_Decompression/ParaPFD_Decompression_kernel.cu:198-264).
+        // ParaPFD / NewPFD: patch exceptions, segHead, inclusive scan
+        // (GC/ParaPFD/ParaPFD_Decompression/ParaPFD_Decompression_kernel.cu:198-264,
+        // GC/NewPFD/NewPFD_Decompression/NewPFD_Decompression_kernel.cu:405-470).
         const uint32_t en = desc_en(rec.desc);
         if (en) {
 #pragma unroll
@@ -179,13 +215,17 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
                 if (j < len) dst[j] = f[t];
             }
             wave_sync();
-            const uint32_t eb = desc_eb(rec.desc), hb = desc_hb(rec.desc);
-            const uint32_t* pp = data + words_for(len, lb);
-            const uint32_t* hp = pp + words_for(en, eb);
-            for (uint32_t e = ln; e < en; e += 64) {
-                const uint32_t p = bits_at(pp, eb, e);
-                const uint32_t h = bits_at(hp, hb, e);
-                if (p < len) dst[p] |= h << lb;
+            if constexpr (CODEC == EXC_NEWPFD) {
+                npfd_patch(data + words_for(len, lb), en, lb, len, xs, dst);
+            } else {
+                const uint32_t eb = desc_eb(rec.desc), hb = desc_hb(rec.desc);
+                const uint32_t* pp = data + words_for(len, lb);
+                const uint32_t* hp = pp + words_for(en, eb);
+                for (uint32_t e = ln; e < en; e += 64) {
+                    const uint32_t p = bits_at(pp, eb, e);
+                    const uint32_t h = bits_at(hp, hb, e);
+                    if (p < len) dst[p] |= h << lb;
+                }
             }
             wave_sync();
 #pragma unroll
@@ -209,7 +249,8 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
 }
 
 // Decode segments listed by sid(g), g < r <= NB, of list R into dst + g*SEG
-// (sid is called with a lane-dependent g).  LANE_RECS: a group's segment
+// (sid is called with a lane-dependent g); NewPFD needs 2 SEG words of
+// scratch after them, at dst + NB*SEG.  LANE_RECS: a group's segment
 // records come in one lane-parallel load and are spread by readlane (fewer
 // registers: the whole-list decoder); otherwise one uniform load per block
 // (what the probe kernel's register budget prefers).
@@ -220,7 +261,7 @@ template <int CODEC, int SEG, int NB, bool LANE_RECS = false, class SegId>
 __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, SegId sid, uint32_t* dst)
 {
     constexpr int V = SEG / 64;
-    constexpr int G = (CODEC == EXC_PARAPFD) ? (NB < 2 ? NB : 2) : NB;
+    constexpr int G = gap_codec(CODEC) ? (NB < 2 ? NB : 2) : NB;
     for (uint32_t g0 = 0; g0 < r; g0 += G) {
         SegRec rec[G];
         uint32_t len[G], sg[G];
@@ -258,7 +299,8 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
 #pragma unroll
         for (int g = 0; g < G; ++g)
             if (g0 + g < r)
-                finish_block<CODEC, SEG>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG);
+                finish_block<CODEC, SEG>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG,
+                                         dst + NB * SEG);
     }
 }
 
@@ -645,7 +687,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 #pragma unroll
             for (int g = 0; g < P; ++g)
                 if ((uint32_t)g < np)
-                    finish_block<CODEC, SEG>(R, sf + p0 + g, rec[g], len[g], data[g], f[g], W.buf + g * SEG);
+                    finish_block<CODEC, SEG>(R, sf + p0 + g, rec[g], len[g], data[g], f[g], W.buf + g * SEG, nullptr);
             const uint32_t total = (np - 1) * SEG + seg_len(R, sf + p0 + np - 1, SEG);
             wave_sync();
             const uint32_t last = uni(W.buf[total - 1]);
@@ -1110,14 +1152,16 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
 // one list (the HS codec: one chunk of SEG elements).  Every block record and
 // packed word of the run is requested before any value is formed; LRC and
 // SegLRC values go from registers straight to HBM in coalesced stores,
-// ParaPFD values pass through LDS for the exception patch and the scan.
+// ParaPFD and NewPFD values pass through LDS for the exception patch and the
+// scan.
 template <int CODEC, int SEG>
 __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
 {
     constexpr bool HS = CODEC == EXC_HS_SEGLRC;
     constexpr int U = HS ? 1 : (int)(kDecodeUnit / SEG);
-    constexpr bool VIA_LDS = HS || CODEC == EXC_PARAPFD;
-    __shared__ uint32_t lds[VIA_LDS ? kWaves * U * SEG : 1];
+    constexpr bool VIA_LDS = HS || gap_codec(CODEC);
+    constexpr int W = U * SEG + (CODEC == EXC_NEWPFD ? 2 * SEG : 0);  // words per wave
+    __shared__ uint32_t lds[VIA_LDS ? kWaves * W : 1];
     const uint32_t wid = uni(threadIdx.x >> 6), ln = lane_id();
     const uint32_t u = blockIdx.x * kWaves + wid;
     if (u >= nunits) return;
@@ -1141,7 +1185,7 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* 
         const uint32_t nb = uni(min((uint32_t)U, R.nseg - k));
         const auto sid = [&](int g) { return k + (uint32_t)g; };
         if constexpr (VIA_LDS) {
-            uint32_t* L = lds + wid * U * SEG;
+            uint32_t* L = lds + wid * W;
             decode_blocks<CODEC, SEG, U, true>(ix, R, nb, sid, L);
             wave_sync();
             const uint32_t n = (nb - 1) * SEG + seg_len(R, k + nb - 1, SEG);
@@ -1252,6 +1296,7 @@ hipError_t launch_decompress(const DevIndex& ix, const DecUnit* units, uint32_t 
     case EXC_SEGLRC: return decomp_for_seg<EXC_SEGLRC>(ix, units, n, out, s);
     case EXC_HS_SEGLRC: return decomp_for_seg<EXC_HS_SEGLRC>(ix, units, n, out, s);
     case EXC_PARAPFD: return decomp_for_seg<EXC_PARAPFD>(ix, units, n, out, s);
+    case EXC_NEWPFD: return decomp_for_seg<EXC_NEWPFD>(ix, units, n, out, s);
     }
     return hipErrorInvalidValue;
 }

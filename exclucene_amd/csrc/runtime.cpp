@@ -241,6 +241,32 @@ uint32_t block_need_words(const uint32_t* blk, uint32_t len)
     return 1 + words_for(len, lb) + (en ? words_for(en, eb) + words_for(en, hb) : 0);
 }
 
+// Words of a NewPFor block of len values whose words are blk[0 .. avail), or 0
+// if it is malformed: its Simple16 stream must end inside avail, its
+// exception positions must lie inside the block and its descriptor must have
+// no bits between en and b.  Reads only words below avail.
+uint32_t npfd_block_words(const uint32_t* blk, uint32_t len, uint64_t avail)
+{
+    const uint32_t d = blk[0], b = desc_lb(d), en = desc_en(d);
+    if ((d >> 16) & 1023u || en > len) return 0;
+    uint64_t w = 1 + (uint64_t)words_for(len, b);
+    if (w > avail) return 0;
+    if (!en) return (uint32_t)w;
+    std::vector<uint32_t> all(2 * (size_t)en + 28);
+    for (uint32_t got = 0; got < 2 * en; ++w) {
+        if (w >= avail) return 0;
+        const uint32_t c = s16_count(blk[w]);
+        s16_unpack(blk[w], all.data() + got, c);
+        got += c;
+    }
+    uint64_t pos = all[0];
+    for (uint32_t i = 0; i < en; ++i) {
+        if (pos >= len) return 0;
+        if (i + 1 < en) pos += (uint64_t)all[i + 1] + 1;
+    }
+    return (uint32_t)w;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- context
@@ -365,7 +391,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
 {
     if (!c || !h || !out) return set_error(EXC_ERR_ARG, "exc_index_upload: null argument");
     const int codec = h->codec;
-    if (codec < EXC_LRC || codec > EXC_PARAPFD) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
+    if (codec < EXC_LRC || codec > EXC_NEWPFD) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
     const uint32_t seg = h->seg_size;
     const bool hs = codec == EXC_HS_SEGLRC;
     if (hs) {
@@ -429,6 +455,11 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
             if (blen[s] == 0) { if (o > lw) return set_error(EXC_ERR_FORMAT, "term %u: bad offset of empty bucket", t); continue; }
             if (o >= lw) return set_error(EXC_ERR_FORMAT, "term %u segment %u: offset outside the list", t, s);
             const uint32_t* blk = h->ind2 + w0 + o;
+            if (codec == EXC_NEWPFD) {
+                if (!npfd_block_words(blk, blen[s], lw - o))
+                    return set_error(EXC_ERR_FORMAT, "term %u segment %u: malformed NewPFor block", t, s);
+                continue;
+            }
             uint32_t need = block_need_words(blk, blen[s]);
             if (o + need > lw) return set_error(EXC_ERR_FORMAT, "term %u segment %u: block overruns the list", t, s);
             if (codec != EXC_PARAPFD && desc_en(blk[0]))
@@ -599,7 +630,8 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
         return rc;
     if (!hs && (rc = map_file(p + ".segHead", head, false))) return rc;
     if ((rc = map_file(p + ".segMedian", median, true))) return rc;
-    if (codec != EXC_PARAPFD && (rc = map_file(p + ".regression_info", reg, false))) return rc;
+    const bool gaps = codec == EXC_PARAPFD || codec == EXC_NEWPFD;
+    if (!gaps && (rc = map_file(p + ".regression_info", reg, false))) return rc;
     if ((rc = map_file(p + ".regression_info_list", regl, true))) return rc;
     if ((codec == EXC_SEGLRC || hs) && (rc = map_file(p + ".ind2_distance_base", base, false))) return rc;
     if (hs && ((rc = map_file(p + ".HashSeg_offset", hoff, false)) || (rc = map_file(p + ".HashSeg_info", hinfo, false))))
@@ -625,7 +657,7 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
     if (codec == EXC_LRC) {
         if (reg.size() != (size_t)h.nterms * sizeof(exc_lrc_info)) return set_error(EXC_ERR_FORMAT, "LRC .regression_info size mismatch");
         h.lrc_info = (const exc_lrc_info*)reg.data();
-    } else if (codec != EXC_PARAPFD) {
+    } else if (!gaps) {
         if (reg.size() != h.nseg * sizeof(exc_seg_info) || base.size() != h.nseg * 4)
             return set_error(EXC_ERR_FORMAT, ".regression_info / .ind2_distance_base size mismatch");
         h.seg_info = (const exc_seg_info*)reg.data();
@@ -671,6 +703,8 @@ uint32_t knobs_for(const exc_ctx* c, int codec, uint32_t k)
 
 int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq)
 {
+    if (ix->codec == EXC_NEWPFD)
+        return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only (as GC/NewPFD in the reference)");
     c->ix = ix;
     c->seg = ix->seg;
     c->queries.resize(nq);
@@ -1098,6 +1132,7 @@ extern "C" int exc_index_train_models(exc_ctx* c, exc_index* ix, int kind, uint3
     if (!c || !ix) return set_error(EXC_ERR_ARG, "exc_index_train_models: null argument");
     if (kind != EXC_MODEL_PLA && kind != EXC_MODEL_MLP) return set_error(EXC_ERR_ARG, "unknown model kind %d", kind);
     if (ix->codec == EXC_HS_SEGLRC) return set_error(EXC_ERR_ARG, "HS_SegLRC lists are searched by bucket, not by position");
+    if (ix->codec == EXC_NEWPFD) return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only");
     if (kind == EXC_MODEL_MLP && (epochs == 0 || samples == 0)) return set_error(EXC_ERR_ARG, "MLP needs epochs and samples");
     HIP_TRY(hipSetDevice(c->device));
     std::vector<uint32_t> terms;
@@ -1600,6 +1635,7 @@ extern "C" int exc_autotune(exc_ctx* c, exc_index* ix, const uint32_t* qterms, c
                             uint32_t reps, exc_tune_table* out)
 {
     if (!c || !ix || !qoff || (nq && !qterms) || !out) return set_error(EXC_ERR_ARG, "exc_autotune: bad argument");
+    if (ix->codec == EXC_NEWPFD) return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only");
     reps = std::max<uint32_t>(1, std::min<uint32_t>(reps, 50));
     // classify the sample
     std::vector<std::vector<uint32_t>> cls(kRatioClasses);
@@ -1950,6 +1986,7 @@ extern "C" int exc_server_create(exc_ctx* c, exc_index* ix, uint32_t nslots, uin
 {
     if (!c || !ix || !out || nslots == 0 || nslots > 16 || max_batch == 0)
         return set_error(EXC_ERR_ARG, "exc_server_create: bad argument");
+    if (ix->codec == EXC_NEWPFD) return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only");
     auto* sv = new exc_server();
     sv->ix = ix;
     sv->max_batch = max_batch;

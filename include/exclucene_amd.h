@@ -38,7 +38,10 @@ enum {
     EXC_LRC = 0,       /* GC/LRC       : one global fit per list          */
     EXC_SEGLRC = 1,    /* GC/SegLRC    : per-segment fit + base           */
     EXC_HS_SEGLRC = 2, /* GC/HS_SegLRC : hash buckets on top docID bits   */
-    EXC_PARAPFD = 3    /* GC/ParaPFD   : d-gaps in ParaPFor blocks        */
+    EXC_PARAPFD = 3,   /* GC/ParaPFD   : d-gaps in ParaPFor blocks        */
+    EXC_NEWPFD = 4     /* GC/NewPFD    : d-gaps in NewPFor blocks, Simple16
+                          exceptions; decompression only, as in the reference
+                          (GC/NewPFD/NewPFD_Decompression)                  */
 };
 
 /* ---- reference structs, byte-identical */
@@ -274,7 +277,7 @@ int exc_index_export(exc_ctx *ctx, exc_index *ix, exc_built **out);
  *      GC/LRC/LRC_Generator/LRC_Generator.cu:93-287, GC/LRC/LRC_Compression/main.cpp:86-238
  *      and their SegLRC / HS_SegLRC / ParaPFD counterparts).
  * docs: concatenated sorted distinct docID lists, list t = docs[list_off[t] .. list_off[t+1]).
- * frac: ParaPFor exception fraction (FRAC); hs_target: expected bucket size (256);
+ * frac: ParaPFor / NewPFor exception fraction (FRAC); hs_target: expected bucket size (256);
  * hs_shift: 0 = adaptive shift, else a fixed shift for every list. */
 int exc_build_index(int codec, uint32_t seg_size, float frac, uint32_t hs_target,
                     uint32_t hs_shift, const uint32_t *docs, const uint64_t *list_off,

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY.  This package is the checker: tests/, the smoke()
 entry and the cpu_baseline leg of bench.py use it; the exclucene_amd product
 never imports it.  The numerics live in oracle.c (a C restatement of the
-reference's ParaPFor / LRC / SegLRC / HS_SegLRC / ParaPFD codecs and of its
+reference's ParaPFor / LRC / SegLRC / HS_SegLRC / ParaPFD / NewPFD codecs and of its
 batched intersection semantics, with file:line citations); this module only
 binds it with ctypes and assembles whole indexes from per-list encoders the
 way the reference's *_Compression programs lay them out
@@ -17,8 +17,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_parapfor.so")
+REF_NPFOR_PATH = os.path.join(HERE, "_ref", "libref_newpfor.so")
 
-LRC, SEGLRC, HS, PFD = 0, 1, 2, 3
+LRC, SEGLRC, HS, PFD, NPFD = 0, 1, 2, 3, 4
 CODEC_NAMES = {LRC: "LRC", SEGLRC: "SegLRC", HS: "HS_SegLRC", PFD: "ParaPFD"}
 
 LRC_INFO = np.dtype([("fSlope", "<f4"), ("iIntercept", "<i4"), ("fSearchRange", "<f4")])
@@ -71,6 +72,14 @@ def lib():
         L.orc_pfd_encode_list.restype = ctypes.c_uint64
         L.orc_pfd_encode_list.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                                           _vp, _vp, _vp, _vp]
+        for f in ("orc_s16_encode", "orc_s16_decode", "orc_npfor_decode"):
+            getattr(L, f).restype = ctypes.c_uint32
+            getattr(L, f).argtypes = [_vp, ctypes.c_uint32, _vp]
+        L.orc_npfor_encode.restype = ctypes.c_uint32
+        L.orc_npfor_encode.argtypes = [_vp, ctypes.c_uint32, ctypes.c_float, _vp]
+        L.orc_npfd_encode_list.restype = ctypes.c_uint64
+        L.orc_npfd_encode_list.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                           _vp, _vp, _vp, _vp]
         L.orc_decode_term.restype = ctypes.c_int
         L.orc_decode_term.argtypes = [ctypes.POINTER(OrcIndex), ctypes.c_uint32, _vp]
         L.orc_checksum.argtypes = [_vp, _vp, ctypes.c_uint32, _vp]
@@ -125,6 +134,53 @@ def pfor_decode(words, n):
 def pfor_best_b(vals, frac=0.0):
     v = _u32(vals)
     return lib().orc_pfor_best_b(_p(v), len(v), frac)
+
+
+def s16_encode(vals):
+    v = _u32(vals)
+    out = np.zeros(len(v) + 2, np.uint32)
+    w = lib().orc_s16_encode(_p(v), len(v), _p(out))
+    if len(v) and w == 0:
+        raise ValueError("a value needs more than 28 bits")
+    return out[:w].copy()
+
+
+def s16_decode(words, n):
+    w = np.zeros(len(words) + 1, np.uint32)
+    w[:len(words)] = words
+    out = np.zeros(n + 28, np.uint32)
+    used = lib().orc_s16_decode(_p(w), n, _p(out))
+    return out[:n], used
+
+
+def npfor_encode(vals, frac=0.0):
+    v = _u32(vals)
+    out = np.zeros(4 * len(v) + 8, np.uint32)
+    w = lib().orc_npfor_encode(_p(v), len(v), frac, _p(out))
+    if w == 0:
+        raise ValueError("unrepresentable NewPFor block")
+    return out[:w].copy()
+
+
+def npfor_decode(words, n):
+    w = np.zeros(len(words) + 2, np.uint32)
+    w[:len(words)] = words
+    out = np.zeros(max(n, 1), np.uint32)
+    used = lib().orc_npfor_decode(_p(w), n, _p(out))
+    return out[:n], used
+
+
+def ref_npfor_lib():
+    """The reference's own NewPFor.cpp (+ Simple16.h), compiled into _ref/
+    (None if not built)."""
+    if not os.path.exists(REF_NPFOR_PATH):
+        return None
+    L = ctypes.CDLL(REF_NPFOR_PATH)
+    L.ref_npfor_encode.restype = ctypes.c_uint32
+    L.ref_npfor_encode.argtypes = [_vp, ctypes.c_uint32, ctypes.c_float, _vp]
+    L.ref_npfor_decode.restype = ctypes.c_uint32
+    L.ref_npfor_decode.argtypes = [_vp, ctypes.c_uint32, _vp]
+    return L
 
 
 def ref_lib():
@@ -221,9 +277,9 @@ def build_index(lists, codec, seg=128, frac=0.0, hs_target=256, hs_shift=0):
     uwlist = np.zeros(nt, np.uint64)
     seg_off = np.zeros(max(S, 1), np.uint32)
     seg_head = seg_median = lrc = fit = base = hash_off = hash_info = None
-    if codec in (LRC, SEGLRC, PFD):
+    if codec in (LRC, SEGLRC, PFD, NPFD):
         seg_head = np.zeros(max(S, 1), np.uint32)
-    if codec == PFD:
+    if codec in (PFD, NPFD):
         seg_median = np.zeros(max(S, 1), np.uint32)
     if codec == LRC:
         lrc = np.zeros(nt, LRC_INFO)
@@ -251,9 +307,10 @@ def build_index(lists, codec, seg=128, frac=0.0, hs_target=256, hs_shift=0):
         elif codec == HS:
             k = L_.orc_hs_encode_list(_p(x), n, _p(tabs[t]), frac, fit[sn:].ctypes.data,
                                       base[sn:].ctypes.data, wp, seg_off[sn:].ctypes.data)
-        elif codec == PFD:
-            k = L_.orc_pfd_encode_list(_p(x), n, seg, frac, wp, seg_head[sn:].ctypes.data,
-                                       seg_median[sn:].ctypes.data, seg_off[sn:].ctypes.data)
+        elif codec in (PFD, NPFD):
+            enc = L_.orc_pfd_encode_list if codec == PFD else L_.orc_npfd_encode_list
+            k = enc(_p(x), n, seg, frac, wp, seg_head[sn:].ctypes.data,
+                    seg_median[sn:].ctypes.data, seg_off[sn:].ctypes.data)
         else:
             raise ValueError(codec)
         if n and k == 0:

@@ -420,6 +420,164 @@ int orc_pfd_decode_list(const uint32_t *words, uint32_t n, uint32_t seg,
     return 0;
 }
 
+/* ------------------------------------------------------------- NewPFD */
+
+/* Simple16 field widths per selector, slot by slot (GI/Simple16.h:16-34). */
+static const uint8_t S16_W[16][28] = {
+    {1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1},
+    {2,2,2,2,2,2,2,1,1,1,1,1,1,1,1,1,1,1,1,1,1},
+    {1,1,1,1,1,1,1,2,2,2,2,2,2,2,1,1,1,1,1,1,1},
+    {1,1,1,1,1,1,1,1,1,1,1,1,1,1,2,2,2,2,2,2,2},
+    {2,2,2,2,2,2,2,2,2,2,2,2,2,2},
+    {4,3,3,3,3,3,3,3,3},
+    {3,4,4,4,4,3,3,3},
+    {4,4,4,4,4,4,4},
+    {5,5,5,5,4,4},
+    {4,4,5,5,5,5},
+    {6,6,6,5,5},
+    {5,5,6,6,6},
+    {7,7,7,7},
+    {10,9,9},
+    {14,14},
+    {28}};
+static const uint8_t S16_N[16] = {28, 21, 21, 21, 14, 9, 8, 7, 6, 6, 5, 5, 4, 3, 2, 1};
+
+/* GI/Simple16.h:13-48: per word, the first selector whose next min(count,
+ * left) slots hold the values.  Returns words written, 0 if a value needs
+ * more than 28 bits. */
+uint32_t orc_s16_encode(const uint32_t *in, uint32_t n, uint32_t *out)
+{
+    uint32_t w = 0;
+    for (uint32_t i = 0; i < n; ++w) {
+        uint32_t k = 0;
+        for (; k < 16; ++k) {
+            uint32_t vol = S16_N[k] < n - i ? S16_N[k] : n - i, j = 0, sh = 0, word = k << 28;
+            for (; j < vol && in[i + j] < (1u << S16_W[k][j]); ++j) {
+                word += in[i + j] << sh;
+                sh += S16_W[k][j];
+            }
+            if (j == vol) {
+                out[w] = word;
+                i += vol;
+                break;
+            }
+        }
+        if (k == 16) return 0;
+    }
+    return w;
+}
+
+/* S16_DECODE (GI/Simple16.h:52-258): whole words until at least n values;
+ * writes up to n + 27 values, returns words consumed. */
+uint32_t orc_s16_decode(const uint32_t *in, uint32_t n, uint32_t *out)
+{
+    uint32_t w = 0, got = 0;
+    while (got < n) {
+        uint32_t word = in[w++], k = word >> 28, sh = 0;
+        for (uint32_t j = 0; j < S16_N[k]; ++j) {
+            out[got++] = (word >> sh) & ((1u << S16_W[k][j]) - 1u);
+            sh += S16_W[k][j];
+        }
+    }
+    return w;
+}
+
+/* GI/NewPFor.cpp:54-99.  b as ParaPFor's rule, raised until every high part
+ * fits 28 bits (where the reference's encoder would not return).  Returns
+ * words written, 0 if unrepresentable. */
+uint32_t orc_npfor_encode(const uint32_t *in, uint32_t n, float frac, uint32_t *out)
+{
+    uint32_t b = orc_pfor_best_b(in, n, frac), mb = maxbits(in, n);
+    if (mb > 28 && b < mb - 28) b = mb - 28;
+    if (b > 31) return 0;
+    uint32_t *low = (uint32_t *)malloc(((size_t)n + 1) * 4);
+    uint32_t *all = (uint32_t *)malloc((2 * (size_t)n + 1) * 4);
+    uint32_t *pos = (uint32_t *)malloc(((size_t)n + 1) * 4);
+    uint32_t en = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (gccbits(in[i]) > b) {
+            low[i] = in[i] & ((1u << b) - 1u);
+            all[n + en] = in[i] >> b;
+            pos[en++] = i;
+        } else {
+            low[i] = in[i];
+        }
+    }
+    out[0] = (b << 26) + en;
+    pack_bits(low, n, b, out + 1);
+    uint32_t w = 1 + (uint32_t)div_roundup((uint64_t)n * b, 32);
+    if (en) {
+        /* all = [pos0, pos1 - pos0 - 1, ..., highs] */
+        for (uint32_t i = 0; i < en; ++i) all[i] = i ? pos[i] - pos[i - 1] - 1 : pos[0];
+        memmove(all + en, all + n, (size_t)en * 4);
+        uint32_t k = orc_s16_encode(all, 2 * en, out + w);
+        w = k ? w + k : 0;
+    }
+    free(low);
+    free(all);
+    free(pos);
+    return w;
+}
+
+/* GI/NewPFor.cpp:103-133.  Returns words consumed. */
+uint32_t orc_npfor_decode(const uint32_t *in, uint32_t n, uint32_t *out)
+{
+    uint32_t d = in[0], b = (d >> 26) % 32, en = d & 65535u;
+    unpack_bits(in + 1, n, b, out);
+    uint32_t w = 1 + (uint32_t)div_roundup((uint64_t)n * b, 32);
+    if (en) {
+        uint32_t *all = (uint32_t *)malloc((2 * (size_t)en + 28) * 4);
+        w += orc_s16_decode(in + w, 2 * en, all);
+        uint32_t p = all[0];
+        for (uint32_t i = 0; i < en; ++i) {
+            if (p < n) out[p] += all[en + i] << b;
+            if (i + 1 < en) p += all[i + 1] + 1;
+        }
+        free(all);
+    }
+    return w;
+}
+
+/* GC/NewPFD/NewPFD_Compression/main.cpp:151-189: d-gaps chained across
+ * blocks (gap 0 = docID 0), 256- (or seg-) element NewPFor blocks. */
+uint64_t orc_npfd_encode_list(const uint32_t *L, uint32_t n, uint32_t seg, float frac,
+                              uint32_t *words, uint32_t *seg_head, uint32_t *seg_median,
+                              uint32_t *seg_off)
+{
+    if (n == 0) return 0;
+    uint32_t *g = (uint32_t *)malloc((size_t)n * 4);
+    g[0] = L[0];
+    for (uint32_t j = 1; j < n; ++j) g[j] = L[j] - L[j - 1];
+    uint64_t w = 0;
+    uint32_t nseg = (uint32_t)div_roundup(n, seg);
+    for (uint32_t s = 0; s < nseg; ++s) {
+        uint32_t st = s * seg, len = (s + 1 < nseg) ? seg : n - st;
+        seg_head[s] = L[st];
+        seg_median[s] = L[st + len / 2];
+        seg_off[s] = (uint32_t)w;
+        uint32_t k = orc_npfor_encode(g + st, len, frac, words + w);
+        if (!k) { free(g); return 0; }
+        w += k;
+    }
+    free(g);
+    return w;
+}
+
+/* GC/NewPFD/NewPFD_Decompression/NewPFD_Decompression_kernel.cu:405-470:
+ * unpack + patch, gap 0 replaced by segHead, inclusive prefix sum. */
+int orc_npfd_decode_list(const uint32_t *words, uint32_t n, uint32_t seg,
+                         const uint32_t *seg_head, const uint32_t *seg_off, uint32_t *out)
+{
+    uint32_t nseg = (uint32_t)div_roundup(n, seg);
+    for (uint32_t s = 0; s < nseg; ++s) {
+        uint32_t st = s * seg, len = (s + 1 < nseg) ? seg : n - st;
+        orc_npfor_decode(words + seg_off[s], len, out + st);
+        out[st] = seg_head[s];
+        for (uint32_t j = 1; j < len; ++j) out[st + j] += out[st + j - 1];
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------- index + queries */
 
 static uint32_t term_nseg(const orc_index *ix, uint32_t t)
@@ -444,6 +602,7 @@ int orc_decode_term(const orc_index *ix, uint32_t t, uint32_t *out)
     case ORC_SEGLRC: return orc_seglrc_decode_list(w, n, ix->seg, ix->fit + sn, ix->base + sn, ix->seg_off + sn, out);
     case ORC_HS: return orc_hs_decode_list(w, n, ix->hash_info + ix->hash_off[t], ix->fit + sn, ix->base + sn, ix->seg_off + sn, out);
     case ORC_PFD: return orc_pfd_decode_list(w, n, ix->seg, ix->seg_head + sn, ix->seg_off + sn, out);
+    case ORC_NPFD: return orc_npfd_decode_list(w, n, ix->seg, ix->seg_head + sn, ix->seg_off + sn, out);
     }
     return -1;
 }

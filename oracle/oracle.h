@@ -17,6 +17,9 @@
  *   GC/HS_SegLRC/HS_SegLRC_Generator/HS_SegLRC_Generator.cu:243-288,503-594
  *   GC/HS_SegLRC/HS_SegLRC_Compression/main.cpp:149-178
  *   GC/ParaPFD/ParaPFD_Compression/main.cpp:151-189,235-311
+ *   GI/NewPFor.cpp:54-133, GI/Simple16.h:13-258,
+ *   GC/NewPFD/NewPFD_Compression/main.cpp:151-189,
+ *   GC/NewPFD/NewPFD_Decompression/NewPFD_Decompression_kernel.cu:405-470
  *   GC/LRC/LRC_Intersection/LRC_Intersection.cu:126-215 (query semantics),
  *   :354-379 (checksums)
  *
@@ -34,7 +37,7 @@
 extern "C" {
 #endif
 
-enum { ORC_LRC = 0, ORC_SEGLRC = 1, ORC_HS = 2, ORC_PFD = 3 };
+enum { ORC_LRC = 0, ORC_SEGLRC = 1, ORC_HS = 2, ORC_PFD = 3, ORC_NPFD = 4 };
 
 /* LRC per-list fit, 12 bytes (LRC_Intersection_kernel.cu:89-93). */
 typedef struct { float fSlope; int32_t iIntercept; float fSearchRange; } orc_lrc_info;
@@ -47,6 +50,17 @@ uint32_t orc_pfor_best_b(const uint32_t *in, uint32_t n, float frac);
 uint32_t orc_pfor_encode(const uint32_t *in, uint32_t n, float frac, uint32_t *out);
 /* returns words consumed */
 uint32_t orc_pfor_decode(const uint32_t *in, uint32_t n, uint32_t *out);
+
+/* ---- NewPFor block codec (GI/NewPFor.cpp) and Simple16 (GI/Simple16.h) ---- */
+uint32_t orc_s16_encode(const uint32_t *in, uint32_t n, uint32_t *out);   /* words, 0 if a value >= 2^28 */
+uint32_t orc_s16_decode(const uint32_t *in, uint32_t n, uint32_t *out);   /* words; writes <= n + 27 */
+uint32_t orc_npfor_encode(const uint32_t *in, uint32_t n, float frac, uint32_t *out);
+uint32_t orc_npfor_decode(const uint32_t *in, uint32_t n, uint32_t *out);
+uint64_t orc_npfd_encode_list(const uint32_t *L, uint32_t n, uint32_t seg, float frac,
+                              uint32_t *words, uint32_t *seg_head, uint32_t *seg_median,
+                              uint32_t *seg_off);
+int orc_npfd_decode_list(const uint32_t *words, uint32_t n, uint32_t seg,
+                         const uint32_t *seg_head, const uint32_t *seg_off, uint32_t *out);
 
 /* ---- LRC (one global fit per list, residual blocks of `seg`) ---- */
 void orc_lrc_fit(const uint32_t *L, uint32_t n, orc_lrc_info *info);
