@@ -31,7 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
 TRAFFIC_KIB = 1437231.0
 C5_SCALE = 5.8e9  # C of df_k = min(2^25, C / k^1.1): about 25.4G postings, 40 GB resident
-META = {0: 8, 1: 28, 2: 32, 3: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD
+META = {0: 8, 1: 28, 2: 32, 3: 8, 4: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD, NewPFD
+CODEC_NAME = {0: "LRC", 1: "SegLRC", 2: "HS_SegLRC", 3: "ParaPFD", 4: "NewPFD"}
 
 
 def log(*a):
@@ -200,7 +201,9 @@ def decode_leg(ea, ctx, dix, built, docs, reps=5):
     alg = len(built.ind2) * 4 + nblocks * META[built.codec] + total * 4
     same = bool(np.array_equal(ctx.from_device(out, total), docs))
     ctx.device_free(out)
-    return {"kernel": "k_decompress<SegLRC,128>", "docids": total, "kernel_ms": float(ms.mean()),
+    return {"kernel": "k_decompress<%s,%d>" % (CODEC_NAME[built.codec], built.seg), "docids": total,
+            "bits_per_docid": 8.0 * (len(built.ind2) * 4 + nblocks * META[built.codec]) / total,
+            "kernel_ms": float(ms.mean()),
             "algorithmic_bytes": int(alg), "achieved_GBps": alg / (ms.mean() * 1e-3) / 1e9,
             "frac_of_peak": alg / (ms.mean() * 1e-3) / 1e9 / HBM_PEAK_GBS, "docids_per_s": total / (ms.mean() * 1e-3),
             "matches_lists": same}
@@ -582,8 +585,20 @@ def main():
         pcum, pres = ctx.dtohTransfer(ctx._bi)
         extra["parapfd_matches_seglrc"] = bool(np.array_equal(pcum, cum) and np.array_equal(pres, res))
         log("ParaPFD batched: %.0f q/s" % extra["parapfd_batched_qps"])
+        if not args.no_decode:
+            extra["decode_parapfd"] = decode_leg(ea, ctx, pix, pfd, docs)
+            log("decode ParaPFD: %s" % extra["decode_parapfd"])
         pix.close()
         del pfd
+        if not args.no_decode:
+            # NewPFD (decompression only, as in the reference): 256-docID blocks, FRAC 0.1
+            npf = ea.build_index(docs, off, ea.NEWPFD, seg=256, frac=0.1)
+            nix = ctx.upload(npf)
+            extra["decode_newpfd"] = decode_leg(ea, ctx, nix, npf, docs)
+            log("decode NewPFD: %s" % extra["decode_newpfd"])
+            nix.close()
+            del npf
+        headline_opts()
 
     if not args.no_c1:
         extra["plain_c1"] = c1_leg(ea, W, ctx)
