@@ -156,12 +156,13 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
 // deltas gives every exception its position, and the patch is lane-parallel.
 // Stream words are read 64 at a time: past the stream's end a round reads at
 // most 63 words, inside the list's successors or the array's zero padding.
+// w_first: this lane's word of the first round, loaded with the block's fields.
 __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint32_t en, uint32_t lb, uint32_t len,
-                                           uint32_t* xs, uint32_t* dst)
+                                           uint32_t w_first, uint32_t* xs, uint32_t* dst)
 {
     const uint32_t ln = lane_id(), need = 2 * en;
     for (uint32_t w0 = 0, got = 0; got < need; w0 += 64) {
-        const uint32_t word = sp[w0 + ln];
+        const uint32_t word = w0 ? sp[w0 + ln] : w_first;
         const uint32_t c = s16_count(word);
         const uint32_t incl = wave_incl_scan(c);
         const uint32_t at = got + incl - c;
@@ -179,12 +180,37 @@ __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint
     }
 }
 
+// The first exception words of a d-gap block, loaded with its fields so that
+// the patch needs no second round trip: ParaPFD exception e = lane's position
+// and high part, NewPFD the lane's word of the first Simple16 round.
+struct ExcPrefetch {
+    uint32_t a = 0, b = 0;
+};
+
+template <int CODEC>
+__device__ __forceinline__ ExcPrefetch prefetch_exceptions(const uint32_t* __restrict__ data, uint32_t desc, uint32_t len)
+{
+    ExcPrefetch x;
+    const uint32_t en = desc_en(desc);
+    if (!gap_codec(CODEC) || !en) return x;
+    const uint32_t* xp = data + words_for(len, desc_lb(desc));
+    const uint32_t ln = lane_id();
+    if constexpr (CODEC == EXC_NEWPFD) {
+        x.a = xp[ln];
+    } else {
+        const uint32_t eb = desc_eb(desc), e = min(ln, en - 1u);
+        x.a = bits_at(xp, eb, e);
+        x.b = bits_at(xp + words_for(en, eb), desc_hb(desc), e);
+    }
+    return x;
+}
+
 // Finish decoding block s (fields already loaded) into dst[0..len); xs is
-// the NewPFor exception scratch (2 SEG words).
-template <int CODEC, int SEG>
+// the NewPFor exception scratch (2 SEG words), xp the prefetched exceptions.
+template <int CODEC, int SEG, bool PREFETCH = false>
 __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const SegRec& rec, uint32_t len,
                                              const uint32_t* __restrict__ data, uint32_t (&f)[SEG / 64],
-                                             uint32_t* dst, uint32_t* xs)
+                                             uint32_t* dst, uint32_t* xs, ExcPrefetch xp = ExcPrefetch())
 {
     constexpr int V = SEG / 64;
     const uint32_t ln = lane_id();
@@ -216,14 +242,15 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
             }
             wave_sync();
             if constexpr (CODEC == EXC_NEWPFD) {
-                npfd_patch(data + words_for(len, lb), en, lb, len, xs, dst);
+                const uint32_t* sp = data + words_for(len, lb);
+                npfd_patch(sp, en, lb, len, PREFETCH ? xp.a : sp[ln], xs, dst);
             } else {
                 const uint32_t eb = desc_eb(rec.desc), hb = desc_hb(rec.desc);
                 const uint32_t* pp = data + words_for(len, lb);
                 const uint32_t* hp = pp + words_for(en, eb);
                 for (uint32_t e = ln; e < en; e += 64) {
-                    const uint32_t p = bits_at(pp, eb, e);
-                    const uint32_t h = bits_at(hp, hb, e);
+                    const uint32_t p = (PREFETCH && e < 64) ? xp.a : bits_at(pp, eb, e);
+                    const uint32_t h = (PREFETCH && e < 64) ? xp.b : bits_at(hp, hb, e);
                     if (p < len) dst[p] |= h << lb;
                 }
             }
@@ -256,12 +283,15 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
 // (what the probe kernel's register budget prefers).
 // Blocks go in groups (all of them for the random-access codecs, pairs for
 // ParaPFD whose scans hold more registers); every load of a group is issued
-// before any of its values is consumed.
-template <int CODEC, int SEG, int NB, bool LANE_RECS = false, class SegId>
+// before any of its values is consumed.  PREFETCH: the d-gap codecs' first
+// exception words are loaded with the fields (two more registers per block:
+// the whole-list decoder takes them, the probe kernel's ParaPFD path not).
+template <int CODEC, int SEG, int NB, bool LANE_RECS = false, bool PREFETCH = false, class SegId>
 __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, SegId sid, uint32_t* dst)
 {
     constexpr int V = SEG / 64;
-    constexpr int G = gap_codec(CODEC) ? (NB < 2 ? NB : 2) : NB;
+    constexpr int GG = PREFETCH ? 4 : 2;  // d-gap blocks in flight: the decoder's register budget allows more
+    constexpr int G = gap_codec(CODEC) ? (NB < GG ? NB : GG) : NB;
     for (uint32_t g0 = 0; g0 < r; g0 += G) {
         SegRec rec[G];
         uint32_t len[G], sg[G];
@@ -294,13 +324,17 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
             len[g] = seg_len(R, sg[g], SEG);
             data[g] = ix.words + R.woff + rec[g].off + 1;
         }
+        ExcPrefetch xp[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
+        for (int g = 0; g < G; ++g) {
+            load_fields<CODEC, SEG>(data[g], desc_lb(rec[g].desc), len[g], f[g]);
+            if constexpr (PREFETCH) xp[g] = prefetch_exceptions<CODEC>(data[g], rec[g].desc, len[g]);
+        }
 #pragma unroll
         for (int g = 0; g < G; ++g)
             if (g0 + g < r)
-                finish_block<CODEC, SEG>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG,
-                                         dst + NB * SEG);
+                finish_block<CODEC, SEG, PREFETCH>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG,
+                                                   dst + NB * SEG, xp[g]);
     }
 }
 
@@ -1153,7 +1187,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
 // packed word of the run is requested before any value is formed; LRC and
 // SegLRC values go from registers straight to HBM in coalesced stores,
 // ParaPFD and NewPFD values pass through LDS for the exception patch and the
-// scan.
+// scan (values stored straight from registers, V consecutive per lane, and a
+// single patch area measured slower: 0.99 vs 0.91 ms for the C2 ParaPFD index).
 template <int CODEC, int SEG>
 __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
 {
@@ -1186,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* 
         const auto sid = [&](int g) { return k + (uint32_t)g; };
         if constexpr (VIA_LDS) {
             uint32_t* L = lds + wid * W;
-            decode_blocks<CODEC, SEG, U, true>(ix, R, nb, sid, L);
+            decode_blocks<CODEC, SEG, U, true, true>(ix, R, nb, sid, L);
             wave_sync();
             const uint32_t n = (nb - 1) * SEG + seg_len(R, k + nb - 1, SEG);
             for (uint32_t e = ln; e < n; e += 64) o[e] = L[e];
