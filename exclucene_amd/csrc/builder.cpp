@@ -244,6 +244,10 @@ struct ListOut {
 void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
                  const uint32_t* L, uint32_t n, ListOut& o)
 {
+    if (codec == EXC_RAW) {  // the docIDs as they are (the GU programs' .ind2)
+        o.words.assign(L, L + n);
+        return;
+    }
     std::vector<uint32_t> vd(n ? n : 1);
     if (codec == EXC_LRC) {
         o.lrc = lrc_fit(L, n);
@@ -360,7 +364,8 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
                                int nthreads, exc_built** out)
 {
     if (!out || !list_off || (nterms && !docs)) return set_error(EXC_ERR_ARG, "exc_build_index: null argument");
-    if (codec < EXC_LRC || codec > EXC_NEWPFD) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
+    if (codec < EXC_LRC || codec > EXC_RAW) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
+    if (codec == EXC_RAW) seg = 128;  // not stored: the unit planner's block size
     if (codec != EXC_HS_SEGLRC && (seg < 64 || seg > 1024 || (seg & 63)))
         return set_error(EXC_ERR_ARG, "exc_build_index: segment size must be a multiple of 64 in [64,1024]");
     if (hs_target == 0) hs_target = 256;

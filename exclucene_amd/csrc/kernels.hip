@@ -843,6 +843,102 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     return uni(cnt);
 }
 
+// ------------------------------------------------------------ uncompressed
+
+// First index in [lo, hi) of the sorted array L with L[idx] >= v (hi if none):
+// 64-ary narrowing by the whole wave, one load round per factor of 64.
+__device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t* __restrict__ L, uint32_t lo, uint32_t hi, uint32_t v)
+{
+    const uint32_t ln = lane_id();
+    while (hi - lo > 64) {
+        const uint32_t step = (hi - lo + 63) >> 6;  // 64 samples cover [lo, hi)
+        const uint32_t p = lo + ln * step;
+        const uint32_t c = popc(__ballot(p < hi && L[p] < v));  // samples below v: a prefix
+        if (c == 0) return lo;
+        const uint32_t nlo = lo + (c - 1) * step + 1, nhi = min(hi, lo + c * step);
+        lo = uni(nlo);
+        hi = uni(nhi);
+    }
+    const uint32_t p = lo + ln;
+    return lo + popc(__ballot(p < hi && L[p] < v));
+}
+
+// Is v in the sorted L[a .. b)?  interp: interpolation search
+// (GU/cuda_interpolationSearch/cuda_interpolation_kernel.cu:254-297), with
+// the probe position clamped to [left, right] (the reference's float
+// interpolation can land one past right on long lists, where its loop stops
+// progressing) and each step loading the probe and both of its neighbours in
+// one round; otherwise the plain binary search of mqSearch
+// (GU/cuda_binarySearch/cuda_binarySearch_kernel.cu:258-293).  Reads stay in
+// L[a .. b).
+__device__ __forceinline__ bool raw_find(const uint32_t* __restrict__ L, uint32_t a, uint32_t b, uint32_t t,
+                                         uint32_t r, uint32_t v, bool interp)
+{
+    if (a >= b || v < t || v > r) return false;
+    int32_t lo = (int32_t)a, hi = (int32_t)b - 1;
+    if (!interp) {
+        while (lo <= hi) {
+            const int32_t m = (lo + hi) >> 1;
+            const uint32_t q = L[m];
+            if (q == v) return true;
+            if (q < v) lo = m + 1; else hi = m - 1;
+        }
+        return false;
+    }
+    while (lo <= hi && t <= v && v <= r) {
+        int32_t m = lo;
+        if (r != t) {
+            const float fr = (float)(hi - lo) * (float)(v - t) / (float)(r - t);
+            m = min(hi, max(lo, lo + (int32_t)fr));
+        }
+        const uint32_t q = L[m];
+        const uint32_t qn = m < hi ? L[m + 1] : 0u, qp = m > lo ? L[m - 1] : 0u;
+        if (q == v) return true;
+        if (q < v) {
+            lo = m + 1;
+            t = qn;
+        } else {
+            hi = m - 1;
+            r = qp;
+        }
+    }
+    return false;
+}
+
+// Uncompressed list R filters the unit's candidates: the wave bounds the
+// part of R that can hold [vmin, vmax], then each lane searches it for its
+// candidates, row by row; order-preserving compaction as for the codecs.
+__device__ uint32_t probe_raw(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
+{
+    constexpr int V = (int)(kChunk / 64);
+    const uint32_t ln = lane_id();
+    const uint32_t* L = A.ix.words + R.woff;
+    const uint32_t vmin = uni(W.cand[0]), vmax = uni(W.cand[nlive - 1]);
+    const uint32_t a = wave_lower_bound(L, 0, R.n, vmin);
+    const uint32_t b = a < R.n ? wave_lower_bound(L, a, R.n, vmax + 1u) : a;
+    if (a >= b) return 0;
+    const uint32_t t = uni(L[a]), r = uni(L[b - 1]);
+    const bool interp = knob_search(knobs) != SEARCH_BINARY;
+    bool found[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        const uint32_t e = ln + 64 * i;
+        found[i] = e < nlive && raw_find(L, a, b, t, r, W.cand[e], interp);
+    }
+    uint32_t cnt = 0;
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        const uint64_t m = __ballot(found[i]);
+        const uint32_t e = ln + 64 * i;
+        const uint32_t vi = e < nlive ? W.cand[e] : kNone;
+        if (found[i]) W.cand[cnt + below_me(m)] = vi;
+        cnt += popc(m);
+    }
+    wave_sync();
+    return uni(cnt);
+}
+
 // HS_SegLRC: decode elements [k*CH, k*CH+CH) of list R into dst; returns count.
 // Bucket of each element = last bucket start <= its index
 // (HS_SegLRC_Intersection_kernel.cu:274-327).
@@ -949,7 +1045,10 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
 }
 
 template <int CODEC>
-constexpr int wave_lds_words() { return (int)(kChunk + kBufWords + (CODEC == EXC_PARAPFD ? kChunk : 0)); }
+constexpr int wave_lds_words()
+{
+    return CODEC == EXC_RAW ? (int)kChunk : (int)(kChunk + kBufWords + (CODEC == EXC_PARAPFD ? kChunk : 0));
+}
 
 template <int CODEC, int SEG>
 __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
@@ -966,6 +1065,11 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     uint32_t nlive;
     if constexpr (CODEC == EXC_HS_SEGLRC) {
         nlive = decode_hs_chunk<kChunk>(A.ix, R0, k, W.cand);
+    } else if constexpr (CODEC == EXC_RAW) {
+        const uint32_t b0 = k * kChunk;
+        nlive = min(kChunk, R0.n - b0);
+        const uint32_t* src = A.ix.words + R0.woff + b0;
+        for (uint32_t e = ln; e < nlive; e += 64) W.cand[e] = src[e];
     } else {
         const uint32_t s0 = k * U;
         const uint32_t nu = uni(min(U, R0.nseg - s0));
@@ -979,6 +1083,8 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
         const TermRec R = qr[j];
         if constexpr (CODEC == EXC_HS_SEGLRC)
             nlive = probe_hs(A, R, W, nlive, knobs);
+        else if constexpr (CODEC == EXC_RAW)
+            nlive = probe_raw(A, R, W, nlive, knobs);
         else
             nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs, j == 1 ? l1_first : 0u, j == 1 ? l1_count : kUnset);
     }
@@ -1203,7 +1309,11 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* 
     const DecUnit D = units[u];
     const uint32_t k = uni(D.k);
     uint32_t* o = out + D.out;
-    if constexpr (HS) {
+    if constexpr (CODEC == EXC_RAW) {
+        const uint32_t b0 = k * SEG, n = min((uint32_t)(U * SEG), D.n - b0);
+        const uint32_t* src = ix.words + D.woff + b0;
+        for (uint32_t e = ln; e < n; e += 64) o[e] = src[e];
+    } else if constexpr (HS) {
         const TermRec R = ix.terms[uni(D.term)];
         uint32_t* L = lds + wid * SEG;
         const uint32_t n = decode_hs_chunk<SEG>(ix, R, k, L);
@@ -1280,6 +1390,7 @@ int probe_blocks_per_cu(int codec, uint32_t seg)
     case EXC_SEGLRC: return occ_for<EXC_SEGLRC>(seg);
     case EXC_HS_SEGLRC: return occ_for<EXC_HS_SEGLRC>(seg);
     case EXC_PARAPFD: return occ_for<EXC_PARAPFD>(seg);
+    case EXC_RAW: return occ_for<EXC_RAW>(seg);
     }
     return 4;
 }
@@ -1297,13 +1408,15 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s)
     case EXC_SEGLRC: return probe_for_seg<EXC_SEGLRC>(a, grid, s);
     case EXC_HS_SEGLRC: return probe_for_seg<EXC_HS_SEGLRC>(a, grid, s);
     case EXC_PARAPFD: return probe_for_seg<EXC_PARAPFD>(a, grid, s);
+    case EXC_RAW: return probe_for_seg<EXC_RAW>(a, grid, s);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_unit_prologue(const ProbeArgs& a, hipStream_t s)
 {
-    if (!a.nunits || a.ix.codec == EXC_HS_SEGLRC) return hipSuccess;  // HS probes find buckets directly
+    // HS probes find buckets directly; raw lists have no block heads
+    if (!a.nunits || a.ix.codec == EXC_HS_SEGLRC || a.ix.codec == EXC_RAW) return hipSuccess;
     const uint32_t grid = (a.nunits + 255) / 256;
     if (a.ix.seg == 256)
         hipLaunchKernelGGL(k_unit_prologue<256>, dim3(grid), dim3(256), 0, s, a);
@@ -1331,6 +1444,7 @@ hipError_t launch_decompress(const DevIndex& ix, const DecUnit* units, uint32_t 
     case EXC_SEGLRC: return decomp_for_seg<EXC_SEGLRC>(ix, units, n, out, s);
     case EXC_HS_SEGLRC: return decomp_for_seg<EXC_HS_SEGLRC>(ix, units, n, out, s);
     case EXC_PARAPFD: return decomp_for_seg<EXC_PARAPFD>(ix, units, n, out, s);
+    case EXC_RAW: return decomp_for_seg<EXC_RAW>(ix, units, n, out, s);
     case EXC_NEWPFD: return decomp_for_seg<EXC_NEWPFD>(ix, units, n, out, s);
     }
     return hipErrorInvalidValue;

@@ -391,18 +391,20 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
 {
     if (!c || !h || !out) return set_error(EXC_ERR_ARG, "exc_index_upload: null argument");
     const int codec = h->codec;
-    if (codec < EXC_LRC || codec > EXC_NEWPFD) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
-    const uint32_t seg = h->seg_size;
+    if (codec < EXC_LRC || codec > EXC_RAW) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
+    const bool raw = codec == EXC_RAW;
+    const uint32_t seg = raw ? 128u : h->seg_size;  // raw lists: the unit planner's block size
     const bool hs = codec == EXC_HS_SEGLRC;
-    if (hs) {
+    if (hs || raw) {
         // the HS kernels walk the shortest list in chunks of 128 elements
     } else if (seg != 128 && seg != 256) {
         return set_error(EXC_ERR_ARG, "segment size %u unsupported (128 or 256)", seg);
     }
     const uint32_t nt = h->nterms;
-    if (nt && (!h->ind1 || !h->seg_num || !h->seg_offset || !h->ind2))
-        return set_error(EXC_ERR_FORMAT, "index is missing .ind1/.ind2/.segNum/.segOffset");
-    if (!hs && !h->seg_head) return set_error(EXC_ERR_FORMAT, "index is missing .segHead");
+    if (nt && (!h->ind1 || !h->ind2)) return set_error(EXC_ERR_FORMAT, "index is missing .ind1/.ind2");
+    if (nt && !raw && (!h->seg_num || !h->seg_offset))
+        return set_error(EXC_ERR_FORMAT, "index is missing .segNum/.segOffset");
+    if (!hs && !raw && !h->seg_head) return set_error(EXC_ERR_FORMAT, "index is missing .segHead");
     if (codec == EXC_LRC && !h->lrc_info) return set_error(EXC_ERR_FORMAT, "LRC index is missing .regression_info");
     if ((codec == EXC_SEGLRC || hs) && (!h->seg_info || !h->seg_base))
         return set_error(EXC_ERR_FORMAT, "index is missing .regression_info/.ind2_distance_base");
@@ -423,9 +425,21 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
         std::memset(&R, 0, sizeof R);
         R.woff = w0;
         R.n = n;
-        R.segnum = h->seg_num[t];
+        R.segnum = raw ? 0u : h->seg_num[t];
         R.model = kNoModel;
         R.term = t;
+        if (raw) {
+            // one word per docID, strictly increasing, below the kernels' empty-slot value
+            if (w1 - w0 < n) return set_error(EXC_ERR_FORMAT, "term %u: list words outside .ind2", t);
+            const uint32_t* L = h->ind2 + w0;
+            for (uint32_t i = 0; i < n; ++i)
+                if ((i && L[i] <= L[i - 1]) || L[i] == 0xffffffffu)
+                    return set_error(EXC_ERR_FORMAT, "term %u: raw list not strictly increasing at %u", t, i);
+            R.nseg = (n + seg - 1) / seg;
+            R.slope = 1.0f;
+            R.range_l = R.range_r = 4.0e9f;
+            continue;
+        }
         uint32_t nseg;
         std::vector<uint32_t> blen;
         if (hs) {
@@ -483,8 +497,8 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     }
 
     // one 32-byte record per segment / bucket: offset, descriptor, fit, base, head
-    std::vector<SegRec> srec(h->nseg);
-    for (uint32_t t = 0; t < nt; ++t) {
+    std::vector<SegRec> srec(raw ? 0 : h->nseg);
+    for (uint32_t t = 0; t < nt && !raw; ++t) {  // raw lists have no block records
         const TermRec& R = recs[t];
         const uint32_t* ht = hs ? h->hash_info + h->hash_offset[t] + 2 : nullptr;
         for (uint32_t s = 0; s < R.nseg; ++s) {
@@ -625,9 +639,20 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
     MappedFile ind1, ind2, segnum, segoff, head, median, reg, regl, base, hoff, hinfo;
     int rc;
     const bool hs = codec == EXC_HS_SEGLRC;
-    if ((rc = map_file(p + ".ind1", ind1, false)) || (rc = map_file(p + ".ind2", ind2, false)) ||
-        (rc = map_file(p + ".segNum", segnum, false)) || (rc = map_file(p + ".segOffset", segoff, false)))
-        return rc;
+    if ((rc = map_file(p + ".ind1", ind1, false)) || (rc = map_file(p + ".ind2", ind2, false))) return rc;
+    if (ind1.size() % sizeof(exc_term_ind1)) return set_error(EXC_ERR_FORMAT, ".ind1 size is not a multiple of 16");
+    if (codec == EXC_RAW) {  // the uncompressed programs' index: .ind1 + the docIDs in .ind2
+        exc_index_host h;
+        std::memset(&h, 0, sizeof h);
+        h.codec = codec;
+        h.seg_size = 128;
+        h.nterms = (uint32_t)(ind1.size() / sizeof(exc_term_ind1));
+        h.ind1 = (const exc_term_ind1*)ind1.data();
+        h.ind2 = (const uint32_t*)ind2.data();
+        h.ind2_words = ind2.size() / 4;
+        return exc_index_upload(c, &h, out);
+    }
+    if ((rc = map_file(p + ".segNum", segnum, false)) || (rc = map_file(p + ".segOffset", segoff, false))) return rc;
     if (!hs && (rc = map_file(p + ".segHead", head, false))) return rc;
     if ((rc = map_file(p + ".segMedian", median, true))) return rc;
     const bool gaps = codec == EXC_PARAPFD || codec == EXC_NEWPFD;
@@ -636,7 +661,6 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
     if ((codec == EXC_SEGLRC || hs) && (rc = map_file(p + ".ind2_distance_base", base, false))) return rc;
     if (hs && ((rc = map_file(p + ".HashSeg_offset", hoff, false)) || (rc = map_file(p + ".HashSeg_info", hinfo, false))))
         return rc;
-    if (ind1.size() % sizeof(exc_term_ind1)) return set_error(EXC_ERR_FORMAT, ".ind1 size is not a multiple of 16");
     exc_index_host h;
     std::memset(&h, 0, sizeof h);
     h.codec = codec;
@@ -1657,7 +1681,7 @@ extern "C" int exc_autotune(exc_ctx* c, exc_index* ix, const uint32_t* qterms, c
     // candidates: cooperative-run limits per codec x search modes
     std::vector<uint32_t> coops;
     if (ix->codec == EXC_PARAPFD) coops = {16, 32, 64};
-    else if (ix->codec == EXC_HS_SEGLRC) coops = {0};
+    else if (ix->codec == EXC_HS_SEGLRC || ix->codec == EXC_RAW) coops = {0};
     else coops = {0, 2, 4, 8, 16, 32, 64};
     std::vector<uint32_t> searches = {SEARCH_BINARY, SEARCH_FIT};
     if (!ix->models.empty()) searches.push_back(SEARCH_LEARNED);

@@ -39,9 +39,13 @@ enum {
     EXC_SEGLRC = 1,    /* GC/SegLRC    : per-segment fit + base           */
     EXC_HS_SEGLRC = 2, /* GC/HS_SegLRC : hash buckets on top docID bits   */
     EXC_PARAPFD = 3,   /* GC/ParaPFD   : d-gaps in ParaPFor blocks        */
-    EXC_NEWPFD = 4     /* GC/NewPFD    : d-gaps in NewPFor blocks, Simple16
+    EXC_NEWPFD = 4,    /* GC/NewPFD    : d-gaps in NewPFor blocks, Simple16
                           exceptions; decompression only, as in the reference
                           (GC/NewPFD/NewPFD_Decompression)                  */
+    EXC_RAW = 5        /* GU (src/gpu/Uncompressed): .ind2 holds the sorted
+                          docIDs themselves; EXC_OPT_SEARCH 0 = binary search
+                          (GU/cuda_binarySearch), else interpolation search
+                          (GU/cuda_interpolationSearch)                      */
 };
 
 /* ---- reference structs, byte-identical */
