@@ -437,6 +437,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
     ap.add_argument("--no-load", action="store_true", help="skip the index-file load leg")
+    ap.add_argument("--no-raw", action="store_true", help="skip the uncompressed-list leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the plain-method single-query leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the hash-bucket leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 1M-term, 35G-posting serving configuration")
@@ -598,6 +599,26 @@ def main():
             log("decode NewPFD: %s" % extra["decode_newpfd"])
             nix.close()
             del npf
+        headline_opts()
+
+    if not args.no_raw:
+        # the C2 lists uncompressed (the reference's src/gpu/Uncompressed programs):
+        # binary search (mqSearch) and interpolation search, one wave per unit
+        rb = ea.build_index(docs, off, ea.RAW)
+        rix = ctx.upload(rb)
+        ctx.set_tuning(None)
+        extra["uncompressed"] = {"device_bytes": int(rix.device_bytes())}
+        for mode, name in ((ea.SEARCH_BINARY, "binary"), (1, "interpolation")):
+            ctx.set_option(ea.OPT_SEARCH, mode)
+            ctx.set_option(ea.OPT_PERSISTENT, 0)
+            rdt, rprobe, _ = time_batch(ctx, rix, queries, max(3, args.steps // 2), 1)
+            rcum, rres = ctx.dtohTransfer(ctx._bi)
+            extra["uncompressed"][name] = {
+                "qps": nq * max(3, args.steps // 2) / rdt, "probe_ms": float(rprobe.mean()),
+                "matches_seglrc": bool(np.array_equal(rcum, cum) and np.array_equal(rres, res))}
+        log("uncompressed: %s" % extra["uncompressed"])
+        rix.close()
+        rb.close()
         headline_opts()
 
     if not args.no_c1:
