@@ -254,7 +254,7 @@ void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32
         if (n > 1) {
             int32_t mn = 0;
             for (uint32_t i = 0; i < n; ++i) {
-                int32_t d = (int32_t)(L[i] - (uint32_t)(lrc_y(i, o.lrc.fSlope) + o.lrc.iIntercept));
+                int32_t d = (int32_t)(L[i] - lrc_pred(i, o.lrc.fSlope, o.lrc.iIntercept));
                 vd[i] = (uint32_t)d;
                 if (i == 0 || d < mn) mn = d;
             }

@@ -24,8 +24,26 @@ namespace exc {
 
 constexpr uint32_t kPadWords = 64;   // zero words after every device array read as value pairs
 
-EXC_HD int32_t lrc_y(uint32_t i, float slope) { return (int32_t)((float)i * slope); }
-EXC_HD int32_t seg_y(uint32_t j, float slope, float icpt) { return (int32_t)fmaf((float)j, slope, icpt); }
+// float -> int32 as the reference's GPU code gets it (CUDA's cvt.rzi.s32.f32)
+// and as gfx950's v_cvt_i32_f32 does it: toward zero, out-of-range values
+// saturated, NaN to 0.  A C cast is undefined out of range (x86 gives
+// INT32_MIN), so encoder and decoder both go through this.
+EXC_HD int32_t f2i_sat(float x)
+{
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x < -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+
+EXC_HD int32_t lrc_y(uint32_t i, float slope) { return f2i_sat((float)i * slope); }
+EXC_HD int32_t seg_y(uint32_t j, float slope, float icpt) { return f2i_sat(fmaf((float)j, slope, icpt)); }
+// the predictions the residuals are taken against, added modulo 2^32
+EXC_HD uint32_t lrc_pred(uint32_t i, float slope, int32_t icpt) { return (uint32_t)lrc_y(i, slope) + (uint32_t)icpt; }
+EXC_HD uint32_t seg_pred(uint32_t j, float slope, float icpt, int32_t base)
+{
+    return (uint32_t)seg_y(j, slope, icpt) + (uint32_t)base;
+}
 
 EXC_HD uint32_t desc_lb(uint32_t d) { return (d >> 26) & 31u; }
 EXC_HD uint32_t desc_eb(uint32_t d) { return (d >> 21) & 31u; }

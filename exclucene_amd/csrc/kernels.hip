@@ -220,14 +220,14 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
 #pragma unroll
         for (int t = 0; t < V; ++t) {
             const uint32_t j = ln + 64 * t;
-            if (j < len) dst[j] = f[t] + (uint32_t)(lrc_y(s * SEG + j, R.slope) + R.icpt);
+            if (j < len) dst[j] = f[t] + lrc_pred(s * SEG + j, R.slope, R.icpt);
         }
     } else if constexpr (CODEC == EXC_SEGLRC) {
         // GC/SegLRC/SegLRC_Decompression/SegLRC_Decompression_kernel.cu:200-256
 #pragma unroll
         for (int t = 0; t < V; ++t) {
             const uint32_t j = ln + 64 * t;
-            if (j < len) dst[j] = f[t] + (uint32_t)(seg_y(j, rec.slope, rec.icpt) + rec.base);
+            if (j < len) dst[j] = f[t] + seg_pred(j, rec.slope, rec.icpt, rec.base);
         }
     } else {
         // ParaPFD / NewPFD: patch exceptions, segHead, inclusive scan
@@ -342,7 +342,7 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
 
 __device__ __forceinline__ uint32_t seg_val(const uint32_t* data, uint32_t lb, uint32_t j, const SegRec& r)
 {
-    return bits_at(data, lb, j) + (uint32_t)(seg_y(j, r.slope, r.icpt) + r.base);
+    return bits_at(data, lb, j) + seg_pred(j, r.slope, r.icpt, r.base);
 }
 
 __device__ __forceinline__ bool seg_bsearch(const uint32_t* data, uint32_t lb, const SegRec& r, int lo, int hi,
@@ -395,9 +395,9 @@ template <int CODEC, int SEG>
 __device__ __forceinline__ uint32_t block_value(const TermRec& R, uint32_t s, const SegRec& rec, uint32_t j, uint32_t f)
 {
     if constexpr (CODEC == EXC_LRC)
-        return f + (uint32_t)(lrc_y(s * SEG + j, R.slope) + R.icpt);
+        return f + lrc_pred(s * SEG + j, R.slope, R.icpt);
     else
-        return f + (uint32_t)(seg_y(j, rec.slope, rec.icpt) + rec.base);
+        return f + seg_pred(j, rec.slope, rec.icpt, rec.base);
 }
 
 // Value j of a SegLRC block whose packed words from w0 on are staged in LDS,
@@ -407,7 +407,7 @@ __device__ __forceinline__ uint32_t win_value(const uint32_t* col, uint32_t lb, 
 {
     const uint32_t bit = j * lb - w0 * 32u, k = bit >> 5;
     const uint64_t cw = (uint64_t)col[64u * k] | ((uint64_t)col[64u * (k + 1)] << 32);
-    return ((uint32_t)(cw >> (bit & 31)) & ((1u << lb) - 1u)) + (uint32_t)(seg_y(j, r.slope, r.icpt) + r.base);
+    return ((uint32_t)(cw >> (bit & 31)) & ((1u << lb) - 1u)) + seg_pred(j, r.slope, r.icpt, r.base);
 }
 
 // Exact search of v in a random-access SegLRC / HS block, one candidate per
@@ -463,7 +463,7 @@ __device__ __forceinline__ bool lrc_bsearch(const uint32_t* data, uint32_t lb, u
     int lo = 0, hi = (int)len - 1;
     while (lo <= hi) {
         const int m = (lo + hi) >> 1;
-        const uint32_t q = bits_at(data, lb, (uint32_t)m) + (uint32_t)(lrc_y(base_index + (uint32_t)m, R.slope) + R.icpt);
+        const uint32_t q = bits_at(data, lb, (uint32_t)m) + lrc_pred(base_index + (uint32_t)m, R.slope, R.icpt);
         if (q == v) return true;
         if (q < v) lo = m + 1; else hi = m - 1;
     }

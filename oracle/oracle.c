@@ -157,7 +157,17 @@ void orc_lrc_fit(const uint32_t *L, uint32_t n, orc_lrc_info *info)
     info->fSearchRange = (float)(rl - rr);
 }
 
-static inline int32_t lrc_y(uint32_t i, float slope) { return (int32_t)((float)i * slope); }
+/* float -> int32 as CUDA's cvt.rzi.s32.f32 (the reference's generator and
+ * decoders run on the GPU): toward zero, saturating, NaN -> 0.  A C cast is
+ * undefined out of range (x86 returns INT32_MIN). */
+static inline int32_t f2i_sat(float x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x < -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+static inline int32_t lrc_y(uint32_t i, float slope) { return f2i_sat((float)i * slope); }
 
 /* Residuals: GC/LRC/LRC_Generator/LRC_Generator_kernel.cu:68-88 (the search loop there
  * terminates at docID - y); min shift: LRC_Generator.cu:165-180; blocks of `seg`
@@ -172,7 +182,7 @@ uint64_t orc_lrc_encode_list(const uint32_t *L, uint32_t n, uint32_t seg, float 
     if (n > 1) {
         int32_t mn = 0;
         for (uint32_t i = 0; i < n; ++i) {
-            int32_t d = (int32_t)(L[i] - (uint32_t)(lrc_y(i, info->fSlope) + info->iIntercept));
+            int32_t d = (int32_t)(L[i] - ((uint32_t)lrc_y(i, info->fSlope) + (uint32_t)info->iIntercept));
             vd[i] = (uint32_t)d;
             if (i == 0 || d < mn) mn = d;
         }
@@ -205,7 +215,7 @@ int orc_lrc_decode_list(const uint32_t *words, uint32_t n, uint32_t seg,
         uint32_t st = s * seg, len = (s + 1 < nseg) ? seg : n - st;
         orc_pfor_decode(words + seg_off[s], len, out + st);
         for (uint32_t k = 0; k < len; ++k)
-            out[st + k] += (uint32_t)(lrc_y(st + k, info->fSlope) + info->iIntercept);
+            out[st + k] += ((uint32_t)lrc_y(st + k, info->fSlope) + (uint32_t)info->iIntercept);
     }
     return 0;
 }
@@ -241,7 +251,7 @@ void orc_seg_fit(const uint32_t *L, uint32_t n, orc_seg_info *info)
     info->nRangeRight = (uint32_t)(0 - rr) + 1;
 }
 
-static inline int32_t seg_y(uint32_t j, float s, float b) { return (int32_t)fmaf((float)j, s, b); }
+static inline int32_t seg_y(uint32_t j, float s, float b) { return f2i_sat(fmaf((float)j, s, b)); }
 
 /* residual against a segment fit, then per-unit min shift ("base"):
  * SegLRC_Generator_kernel.cu:62-100, SegLRC_Generator.cu:185-233. */
@@ -286,7 +296,7 @@ static void seg_block_decode(const uint32_t *blk, uint32_t len, const orc_seg_in
 {
     orc_pfor_decode(blk, len, out);
     for (uint32_t j = 0; j < len; ++j)
-        out[j] += (uint32_t)(seg_y(j, f->fSlope, f->fIntercept) + b);
+        out[j] += ((uint32_t)seg_y(j, f->fSlope, f->fIntercept) + (uint32_t)b);
 }
 
 int orc_seglrc_decode_list(const uint32_t *words, uint32_t n, uint32_t seg,
@@ -625,7 +635,7 @@ static uint32_t decode_block(const orc_index *ix, uint32_t t, uint32_t s, uint32
     case ORC_LRC:
         orc_pfor_decode(blk, len, out);
         for (uint32_t k = 0; k < len; ++k)
-            out[k] += (uint32_t)(lrc_y(st + k, ix->lrc[t].fSlope) + ix->lrc[t].iIntercept);
+            out[k] += ((uint32_t)lrc_y(st + k, ix->lrc[t].fSlope) + (uint32_t)ix->lrc[t].iIntercept);
         break;
     case ORC_SEGLRC: seg_block_decode(blk, len, &ix->fit[sn + s], ix->base[sn + s], out); break;
     case ORC_PFD: pfd_block_decode(blk, len, ix->seg_head[sn + s], out); break;

@@ -248,3 +248,22 @@ def test_c1_plain_query():
     assert np.array_equal(res[:cum[0]], want) and np.array_equal(res[cum[0]:], want)
     # expected size n_a * n_b / 2^24 = 59.6; a seeded draw lands near it
     assert 30 <= len(want) <= 95
+
+
+def test_lrc_prediction_saturates_like_the_gpu():
+    """float -> int32 of a fit prediction saturates (CUDA's cvt.rzi, the
+    reference's GPU generator and decoder; gfx950's v_cvt_i32_f32), where a C
+    cast on x86 would give INT32_MIN: for [0, 2^31 - 1] the slope is 2^31, so
+    y(1) = (float)1 * slope = 2^31 must become 2^31 - 1."""
+    top = (1 << 31) - 1
+    o = oracle.build_index([np.array([0, top], np.uint32)], oracle.LRC, seg=256)
+    slope = np.float32(o.lrc[0]["fSlope"])
+    icpt = int(o.lrc[0]["iIntercept"])
+    y1 = float(np.float32(1.0) * slope)
+    assert y1 >= 2.0 ** 31
+    w0 = int(o.uwlist[0]) + int(o.seg_off[0])
+    vd, _ = oracle.pfor_decode(o.words[w0:w0 + 8], 2)
+    sat = (int(vd[1]) + min(int(y1), top) + icpt) % (1 << 32)
+    wrap = (int(vd[1]) - (1 << 31) + icpt) % (1 << 32)  # the x86 cast's INT32_MIN
+    assert sat == top and wrap != top
+    assert np.array_equal(o.decode_term(0), [0, top])
