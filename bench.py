@@ -438,6 +438,7 @@ def main():
     ap.add_argument("--no-decode", action="store_true", help="skip the whole-index decode leg")
     ap.add_argument("--no-load", action="store_true", help="skip the index-file load leg")
     ap.add_argument("--no-raw", action="store_true", help="skip the uncompressed-list leg")
+    ap.add_argument("--no-lrc", action="store_true", help="skip the LRC leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the plain-method single-query leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the hash-bucket leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 1M-term, 35G-posting serving configuration")
@@ -599,6 +600,24 @@ def main():
             log("decode NewPFD: %s" % extra["decode_newpfd"])
             nix.close()
             del npf
+        headline_opts()
+
+    if not args.no_lrc:
+        # LRC (one fit per list, 256-docID residual blocks; the reference's LRC_Intersection)
+        lb_ = ea.build_index(docs, off, ea.LRC, seg=256, frac=0.0)
+        lix = ctx.upload(lb_)
+        ctx.set_tuning(None)
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        if args.autotune and not args.tune_table:
+            extra["lrc_tune"] = ctx.autotune(lix, W.df_weighted_queries(df, 2000, 2, 5, 23), reps=2)
+        ldt, lprobe, _ = time_batch(ctx, lix, queries, max(3, args.steps // 2), 1)
+        lcum, lres = ctx.dtohTransfer(ctx._bi)
+        extra["lrc"] = {"compressed_bytes": int(lb_.compressed_bytes()),
+                        "batched_qps": nq * max(3, args.steps // 2) / ldt, "probe_ms": float(lprobe.mean()),
+                        "matches_seglrc": bool(np.array_equal(lcum, cum) and np.array_equal(lres, res))}
+        log("LRC: %s" % extra["lrc"])
+        lix.close()
+        lb_.close()
         headline_opts()
 
     if not args.no_raw:
