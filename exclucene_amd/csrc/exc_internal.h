@@ -117,7 +117,10 @@ struct UnitRec {
     // 0: no docID of the slice can be in that list.
     uint32_t l1_first;
     uint32_t l1_count;
-    uint32_t pad[2];
+    // index of this unit in query order (its count and stage-scan slot): the
+    // probe runs units in launch order, the scan in query order
+    uint32_t slot;
+    uint32_t pad;
 };
 static_assert(sizeof(UnitRec) == 32, "UnitRec is 32 bytes");
 constexpr uint32_t kUnset = 0xffffffffu;

@@ -1090,7 +1090,7 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     }
     uint32_t* out = A.stage + short_off + (uint64_t)k * kChunk;
     for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];
-    if (ln == 0) A.ucount[u] = nlive;
+    if (ln == 0) A.ucount[uni(ur.slot)] = nlive;
     wave_sync();
 }
 

@@ -84,12 +84,14 @@ struct exc_ctx {
     std::vector<QueryRec> queries;
     std::vector<uint32_t> qterms;  // sorted terms of the batch
     std::vector<TermRec> qrecs;    // their TermRec copies
-    std::vector<UnitRec> units;
+    std::vector<UnitRec> units;   // query order: the scan's
+    std::vector<UnitRec> launch;  // launch order: the prologue's and the probe's
     uint64_t short_sum = 0;
     uint32_t nq = 0, nunits = 0, seg = 0;
     int state = 0;  // 0 none, 1 planned, 2 on device, 3 invoked
     // device batch buffers
     DevBuf d_queries, d_qrecs, d_units, d_stage, d_ucount, d_uoff, d_bsum, d_result, d_qcum, d_counter;
+    DevBuf d_launch;  // the units again, in launch order (build_launch)
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float probe_ms = 0, invoke_ms = 0;
@@ -299,7 +301,7 @@ extern "C" void exc_ctx_destroy(exc_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (DevBuf* b : {&c->d_queries, &c->d_qrecs, &c->d_units, &c->d_stage, &c->d_ucount, &c->d_uoff,
+    for (DevBuf* b : {&c->d_queries, &c->d_qrecs, &c->d_units, &c->d_launch, &c->d_stage, &c->d_ucount, &c->d_uoff,
                       &c->d_bsum, &c->d_result, &c->d_qcum, &c->d_counter})
         dev_free(c, *b);
     for (auto& kv : c->user_bufs) (void)hipFree(kv.first);
@@ -766,7 +768,8 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         for (uint32_t t : tmp) c->qrecs.push_back(ix->recs[t]);
         if (tc > 0xffff) return set_error(EXC_ERR_ARG, "query %u: more than 65535 terms", q);
         for (uint32_t k = 0; k < Q.nunits; ++k)
-            c->units.push_back(UnitRec{q, k, Q.toff, (uint16_t)tc, (uint16_t)Q.knobs, 0, kUnset, {0, 0}});
+            c->units.push_back(UnitRec{q, k, Q.toff, (uint16_t)tc, (uint16_t)Q.knobs, 0, kUnset,
+                                       (uint32_t)c->units.size(), 0});
         units += Q.nunits;
         if (units > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 work units in one batch");
     }
@@ -780,6 +783,34 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
     return EXC_OK;
 }
 
+// Launch order of the units: queries by descending |second-shortest| /
+// |shortest| (single-term queries last), each query's units together.  The
+// high-ratio units are the slowest ones (their probes search, the others
+// resolve whole runs through the bitmap), so running them first shortens the
+// launch's tail: C2 batch probe 0.653-0.661 ms whatever the callers' query
+// order, against 0.675 ms when units ran in the order given.
+void build_launch(exc_ctx* c)
+{
+    const exc_index* ix = c->ix;
+    std::vector<uint32_t> order(c->nq);
+    std::vector<double> key(c->nq, 0.0);
+    for (uint32_t q = 0; q < c->nq; ++q) {
+        order[q] = q;
+        const QueryRec& Q = c->queries[q];
+        if (Q.tc > 1) {
+            const double n0 = ix->n[c->qterms[Q.toff]], n1 = ix->n[c->qterms[Q.toff + 1]];
+            key[q] = n1 / std::max(1.0, n0);
+        }
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] > key[b]; });
+    c->launch.clear();
+    c->launch.reserve(c->units.size());
+    for (uint32_t q : order) {
+        const QueryRec& Q = c->queries[q];
+        c->launch.insert(c->launch.end(), c->units.begin() + Q.first_unit, c->units.begin() + Q.first_unit + Q.nunits);
+    }
+}
+
 int upload_plan(exc_ctx* c, bool sync = true)
 {
     if (c->state < 1) return set_error(EXC_ERR_STATE, "htodTransfer before CPUPreprocess");
@@ -789,6 +820,7 @@ int upload_plan(exc_ctx* c, bool sync = true)
     if ((rc = dev_alloc(c, c->d_queries, std::max<size_t>(1, c->nq) * sizeof(QueryRec))) ||
         (rc = dev_alloc(c, c->d_qrecs, std::max<size_t>(1, c->qrecs.size()) * sizeof(TermRec))) ||
         (rc = dev_alloc(c, c->d_units, std::max<size_t>(1, c->nunits) * sizeof(UnitRec))) ||
+        (rc = dev_alloc(c, c->d_launch, std::max<size_t>(1, c->nunits) * sizeof(UnitRec))) ||
         (rc = dev_alloc(c, c->d_stage, std::max<uint64_t>(1, c->short_sum) * 4)) ||
         (rc = dev_alloc(c, c->d_ucount, std::max<size_t>(1, c->nunits) * 4)) ||
         (rc = dev_alloc(c, c->d_uoff, std::max<size_t>(1, c->nunits) * 4)) ||
@@ -802,9 +834,13 @@ int upload_plan(exc_ctx* c, bool sync = true)
     if (!c->qrecs.empty())
         HIP_TRY(hipMemcpyAsync(c->d_qrecs.p, c->qrecs.data(), c->qrecs.size() * sizeof(TermRec), hipMemcpyHostToDevice,
                                c->stream));
-    if (c->nunits)
+    build_launch(c);
+    if (c->nunits) {
         HIP_TRY(hipMemcpyAsync(c->d_units.p, c->units.data(), (size_t)c->nunits * sizeof(UnitRec),
                                hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_launch.p, c->launch.data(), (size_t)c->nunits * sizeof(UnitRec),
+                               hipMemcpyHostToDevice, c->stream));
+    }
     if (sync) HIP_TRY(hipStreamSynchronize(c->stream));
     c->state = 2;
     return EXC_OK;
@@ -829,12 +865,16 @@ int refresh_knobs(exc_ctx* c)
     if (c->plan_gen == c->opt_gen) return EXC_OK;
     for (auto& Q : c->queries) Q.knobs = knobs_for(c, c->ix->codec, Q.klass);
     for (auto& U : c->units) U.knobs = (uint16_t)c->queries[U.q].knobs;
+    for (auto& U : c->launch) U.knobs = (uint16_t)c->queries[U.q].knobs;
     if (c->nq)
         HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice,
                                c->stream));
-    if (c->nunits)
+    if (c->nunits) {
         HIP_TRY(hipMemcpyAsync(c->d_units.p, c->units.data(), (size_t)c->nunits * sizeof(UnitRec),
                                hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_launch.p, c->launch.data(), (size_t)c->nunits * sizeof(UnitRec),
+                               hipMemcpyHostToDevice, c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->plan_gen = c->opt_gen;
     return EXC_OK;
@@ -848,7 +888,7 @@ int enqueue_invoke(exc_ctx* c, bool timed)
     a.ix = ix->dix;
     a.queries = dptr<QueryRec>(c->d_queries);
     a.qrecs = dptr<TermRec>(c->d_qrecs);
-    a.units = dptr<UnitRec>(c->d_units);
+    a.units = dptr<UnitRec>(c->d_launch);
     a.nunits = c->nunits;
     a.persistent = c->persistent;
     a.stage = dptr<uint32_t>(c->d_stage);
@@ -870,7 +910,7 @@ int enqueue_invoke(exc_ctx* c, bool timed)
     ScanArgs s;
     std::memset(&s, 0, sizeof s);
     s.queries = a.queries;
-    s.units = a.units;
+    s.units = dptr<UnitRec>(c->d_units);  // query order
     s.ucount = a.ucount;
     s.stage = a.stage;
     s.nunits = c->nunits;
