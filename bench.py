@@ -236,6 +236,32 @@ def loader_leg(ea, ctx, built, queries, ref_cum, ref_res, reps=3):
             "file_bytes": int(nbytes), "load_s": t, "GBps": nbytes / t / 1e9, "matches_upload": same}
 
 
+def plain_leg(ea, ctx, dix, qs, ref_cum, ref_res, n=300):
+    """The plain single-query GPU path on the first n queries of a config:
+    one query per plan / copy / kernels / results sequence, plain binary
+    search with one lane per shortest-list element, no work queue.  Its
+    results must equal the batched run's (ref_cum / ref_res, cumulative
+    counts and results of the same queries).  Leaves the context's options
+    at the plain settings: the caller restores its own."""
+    ctx.set_tuning(None)
+    ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_BINARY)
+    ctx.set_option(ea.OPT_PERSISTENT, 0)
+    ctx.set_option(ea.OPT_TPE, 1)
+    ctx.set_option(ea.OPT_TIMING, 0)
+    n = min(n, len(qs))
+    lat, same = [], True
+    for i in range(n):
+        t1 = time.perf_counter()
+        _, r = ctx.intersect(dix, [qs[i]])
+        lat.append(time.perf_counter() - t1)
+        lo = int(ref_cum[i - 1]) if i else 0
+        same = same and np.array_equal(r, ref_res[lo:int(ref_cum[i])])
+    ctx.set_option(ea.OPT_TIMING, 1)
+    ctx.set_option(ea.OPT_TPE, 0)
+    return {"queries": n, "qps": n / sum(lat), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "matches_batched": bool(same)}
+
+
 def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
     """C4: 10,000 queries of 2-4 terms (seed 44); the probe kernel with plain
     binary search, the list-fit window and the learned window (PLA-256, then
@@ -264,6 +290,8 @@ def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
                                              else max(1, int(np.sum(cum != ref[0]))))
         row["learned_gain_vs_binary"] = row["binary_probe_ms"] / row["learned_probe_ms"]
         out[name] = row
+    out["plain_single_query"] = plain_leg(ea, ctx, dix, qs, ref[0], ref[1])
+    out["batched_learned_vs_plain"] = out["pla"]["learned_qps"] / out["plain_single_query"]["qps"]
     if tune:
         ctx.set_tuning(tune)
     return out
@@ -356,6 +384,8 @@ def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
     out["routed_probe_ms"] = routed_ms
     out["routed_qps"] = len(qs) / (routed_ms * 1e-3)
     out["fixed_shift13_qps"] = keep[13][2]["graph_qps"]
+    out["plain_single_query_shift13"] = plain_leg(ea, ctx, keep[13][0], qs, rcum, rres)
+    ctx.set_option(ea.OPT_SEARCH, 1)
     for dix, hs, _ in keep.values():
         dix.close()
     ctx.set_tuning(None)
@@ -407,6 +437,10 @@ def c5_leg(ea, W, ctx, qps=20000.0, seconds=3.0, check=200):
             row["alone_vs_mix_equal"] = bool(same)
         out[mode] = row
         log("C5 %s: %s" % (mode, row))
+    bcum, bres = ctx.intersect(dix, queries[:300])  # batched, for the plain path's check
+    out["plain_single_query"] = plain_leg(ea, ctx, dix, queries, bcum, bres)
+    ctx.set_option(ea.OPT_SEARCH, 1)
+    ctx.set_tuning(None)
     dix.close()
     return out
 

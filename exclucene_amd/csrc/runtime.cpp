@@ -246,15 +246,15 @@ uint32_t block_need_words(const uint32_t* blk, uint32_t len)
 // Words of a NewPFor block of len values whose words are blk[0 .. avail), or 0
 // if it is malformed: its Simple16 stream must end inside avail, its
 // exception positions must lie inside the block and its descriptor must have
-// no bits between en and b.  Reads only words below avail.
-uint32_t npfd_block_words(const uint32_t* blk, uint32_t len, uint64_t avail)
+// no bits between en and b.  Reads only words below avail; all is scratch.
+uint32_t npfd_block_words(const uint32_t* blk, uint32_t len, uint64_t avail, std::vector<uint32_t>& all)
 {
     const uint32_t d = blk[0], b = desc_lb(d), en = desc_en(d);
     if ((d >> 16) & 1023u || en > len) return 0;
     uint64_t w = 1 + (uint64_t)words_for(len, b);
     if (w > avail) return 0;
     if (!en) return (uint32_t)w;
-    std::vector<uint32_t> all(2 * (size_t)en + 28);
+    if (all.size() < 2 * (size_t)en + 28) all.resize(2 * (size_t)en + 28);
     for (uint32_t got = 0; got < 2 * en; ++w) {
         if (w >= avail) return 0;
         const uint32_t c = s16_count(blk[w]);
@@ -415,6 +415,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
 
     // ---- validate the layout so that every device read stays inside its array
     std::vector<TermRec> recs(nt);
+    std::vector<uint32_t> s16_scratch;  // NewPFor exception streams, decoded for the check
     const uint64_t W = h->ind2_words;
     for (uint32_t t = 0; t < nt; ++t) {
         const exc_term_ind1& r = h->ind1[t];
@@ -472,7 +473,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
             if (o >= lw) return set_error(EXC_ERR_FORMAT, "term %u segment %u: offset outside the list", t, s);
             const uint32_t* blk = h->ind2 + w0 + o;
             if (codec == EXC_NEWPFD) {
-                if (!npfd_block_words(blk, blen[s], lw - o))
+                if (!npfd_block_words(blk, blen[s], lw - o, s16_scratch))
                     return set_error(EXC_ERR_FORMAT, "term %u segment %u: malformed NewPFor block", t, s);
                 continue;
             }
