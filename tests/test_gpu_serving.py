@@ -98,3 +98,27 @@ def test_counts_only_server(ctx, served):
     finally:
         srv.close()
     assert np.array_equal(cnt, alone) and len(res) == 0 and len(got) == 0 and n == alone[0]
+
+
+@pytest.mark.parametrize("codec,seg,frac", [(ea.LRC, 256, 0.0), (ea.HS_SEGLRC, 128, 0.0), (ea.PARAPFD, 128, 0.1),
+                                            (ea.RAW, 128, 0.0)])
+def test_server_other_codecs(ctx, codec, seg, frac):
+    """The serving loop over every probe codec: each query's docIDs in the mix
+    equal its docIDs alone."""
+    lists = _corpus(43, 40)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, codec, seg=seg, frac=frac)
+    dix = ctx.upload(built)
+    try:
+        qs = _queries(len(lists), 200, 5)
+        alone = [ctx.intersect(dix, [q])[1] for q in qs]
+        srv = ea.Server(ctx, dix, nslots=3, max_batch=32)
+        try:
+            lat, cnt, res = srv.replay(qs, np.cumsum(np.full(len(qs), 1e-5)), sum(len(a) for a in alone))
+        finally:
+            srv.close()
+        assert np.array_equal(cnt, [len(a) for a in alone])
+        assert np.array_equal(res, np.concatenate(alone))
+    finally:
+        dix.close()
+        built.close()

@@ -22,7 +22,8 @@ def _queries(nl, nq, seed):
     return [[int(x) for x in rng.choice(nl, int(rng.integers(1, 6)), replace=False)] for _ in range(nq)]
 
 
-@pytest.mark.parametrize("codec,frac", [(ea.SEGLRC, 0.0), (ea.PARAPFD, 0.1), (ea.HS_SEGLRC, 0.0)])
+@pytest.mark.parametrize("codec,frac", [(ea.SEGLRC, 0.0), (ea.PARAPFD, 0.1), (ea.HS_SEGLRC, 0.0), (ea.LRC, 0.0),
+                                        (ea.RAW, 0.0)])
 def test_stage_sequence_with_thresholds(ctx, codec, frac):
     """Run()-style loop: CPUPreprocess -> htod -> kernelInvoke -> dtoh -> verify
     over the whole stream at several thresholds; every threshold must give the
@@ -30,7 +31,8 @@ def test_stage_sequence_with_thresholds(ctx, codec, frac):
     lists = _corpus()
     docs, off = ea.lists_to_flat(lists)
     built = ea.build_index(docs, off, codec, seg=128, frac=frac)
-    ref = oracle.build_index(lists, codec, seg=128, frac=frac)
+    # the uncompressed lists' oracle: the same intersections over a SegLRC encoding
+    ref = oracle.build_index(lists, oracle.SEGLRC if codec == ea.RAW else codec, seg=128, frac=frac)
     qs = _queries(len(lists), 300, 5)
     ocum, ores = oracle.eval_queries(ref, qs)
     ocs = oracle.checksum(ocum, ores)
