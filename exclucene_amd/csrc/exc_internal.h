@@ -120,7 +120,9 @@ struct UnitRec {
     // index of this unit in query order (its count and stage-scan slot): the
     // probe runs units in launch order, the scan in query order
     uint32_t slot;
-    uint32_t pad;
+    // stage-buffer offset of this unit's survivors (the query's shortest-list
+    // offset + k * kChunk; the planner keeps the summed length below 2^32)
+    uint32_t soff;
 };
 static_assert(sizeof(UnitRec) == 32, "UnitRec is 32 bytes");
 constexpr uint32_t kUnset = 0xffffffffu;
@@ -149,9 +151,13 @@ struct ScanArgs {
     uint32_t* bsum;     // per scan block sum, then exclusive block offsets (+ total)
     uint32_t* result;
     uint32_t* qcum;     // per query inclusive cumulative count
+    uint32_t* ticket;   // blocks of k_scan_local done (zero between launches)
 };
 
 constexpr uint32_t kScanBlockItems = 4096;  // units per scan block (256 threads x 16)
+constexpr uint32_t kScatterUnits = 8;       // consecutive units per wave of k_scatter
+constexpr uint32_t kScatterRows = 8;        // 64-element rows k_scatter reads before writing
+constexpr uint32_t kScanTicketWord = 32;     // word of the context's counter buffer holding the scan ticket
 constexpr uint32_t kChunk = 512;            // shortest-list docIDs per work unit
 constexpr uint32_t kDecodeUnit = 1024;      // docIDs per wave of whole-list decompression (HS: one chunk)
 

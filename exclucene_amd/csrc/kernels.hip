@@ -1061,7 +1061,6 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     const uint32_t l1_first = uni(ur.l1_first), l1_count = uni(ur.l1_count);
     const TermRec* qr = A.qrecs + uni(ur.toff);
     const TermRec R0 = qr[0];
-    const uint64_t short_off = A.queries[uni(ur.q)].short_off;  // used at the end only
     uint32_t nlive;
     if constexpr (CODEC == EXC_HS_SEGLRC) {
         nlive = decode_hs_chunk<kChunk>(A.ix, R0, k, W.cand);
@@ -1088,7 +1087,7 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
         else
             nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs, j == 1 ? l1_first : 0u, j == 1 ? l1_count : kUnset);
     }
-    uint32_t* out = A.stage + short_off + (uint64_t)k * kChunk;
+    uint32_t* out = A.stage + uni(ur.soff);
     for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];
     if (ln == 0) A.ucount[uni(ur.slot)] = nlive;
     wave_sync();
@@ -1208,9 +1207,46 @@ __global__ __launch_bounds__(256) void k_unit_prologue(ProbeArgs A)
 __global__ void k_reset(uint32_t* c) { if (threadIdx.x == 0) *c = 0; }
 
 // ---- scan over unit counts (replaces cudppScan + ucntResult)
+
+// Exclusive scan in place of v[0, n) by the whole block of NT threads;
+// v[n] = total.
+template <int NT>
+__device__ void block_excl_scan(uint32_t* v, uint32_t n)
+{
+    __shared__ uint32_t wsum[NT / 64];
+    const uint32_t t = threadIdx.x, ln = lane_id(), wid = t >> 6;
+    const uint32_t per = (n + NT - 1) / NT;
+    const uint32_t b0 = t * per;
+    uint32_t tot = 0;
+    for (uint32_t i = 0; i < per; ++i)
+        if (b0 + i < n) tot += v[b0 + i];
+    const uint32_t incl = wave_incl_scan(tot);
+    if (ln == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t w = 0; w < wid; ++w) woff += wsum[w];
+    uint32_t run = woff + incl - tot;
+    __syncthreads();
+    for (uint32_t i = 0; i < per; ++i) {
+        if (b0 + i < n) {
+            const uint32_t x = v[b0 + i];
+            v[b0 + i] = run;
+            run += x;
+        }
+    }
+    if (t == NT - 1) v[n] = woff + incl;
+}
+
+// Each block scans kScanBlockItems unit counts and publishes its sum, then
+// takes a ticket (device-scope fence first, so the sum is visible before the
+// ticket is).  The block holding the last ticket turns the block sums into
+// exclusive offsets in place (bsum[nblk] = total) and resets the ticket for
+// the next launch.  No block waits for another; the ticket is zeroed when the
+// plan is uploaded and every launch leaves it at zero.
 __global__ __launch_bounds__(256) void k_scan_local(ScanArgs a)
 {
     __shared__ uint32_t wsum[4];
+    __shared__ uint32_t last;
     const uint32_t ln = lane_id(), wid = threadIdx.x >> 6;
     const uint32_t base = blockIdx.x * kScanBlockItems + threadIdx.x * 16;
     uint32_t x[16], tot = 0;
@@ -1232,34 +1268,22 @@ __global__ __launch_bounds__(256) void k_scan_local(ScanArgs a)
         if (u < a.nunits) a.uoff[u] = run;
         run += x[k];
     }
-    if (threadIdx.x == 255) a.bsum[blockIdx.x] = woff + incl;
+    if (threadIdx.x == 255) {
+        a.bsum[blockIdx.x] = woff + incl;
+        __threadfence();
+        last = atomicAdd(a.ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every block's sum is visible past the last ticket
+    block_excl_scan<256>(a.bsum, gridDim.x);
+    if (threadIdx.x == 0) *a.ticket = 0;
 }
 
-// exclusive scan of the block sums in place; bsum[nblk] = total.
-__global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* bsum, uint32_t nblk)
+// bsum[0] = 0 for a batch without units
+__global__ __launch_bounds__(64) void k_scan_empty(uint32_t* bsum)
 {
-    __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x, ln = lane_id(), wid = t >> 6;
-    const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t b0 = t * per;
-    uint32_t tot = 0;
-    for (uint32_t i = 0; i < per; ++i)
-        if (b0 + i < nblk) tot += bsum[b0 + i];
-    const uint32_t incl = wave_incl_scan(tot);
-    if (ln == 63) wsum[wid] = incl;
-    __syncthreads();
-    uint32_t woff = 0;
-    for (uint32_t w = 0; w < wid; ++w) woff += wsum[w];
-    uint32_t run = woff + incl - tot;
-    __syncthreads();
-    for (uint32_t i = 0; i < per; ++i) {
-        if (b0 + i < nblk) {
-            const uint32_t x = bsum[b0 + i];
-            bsum[b0 + i] = run;
-            run += x;
-        }
-    }
-    if (t == 1023) bsum[nblk] = woff + incl;
+    if (threadIdx.x == 0) bsum[0] = 0;
 }
 
 __device__ __forceinline__ uint32_t unit_excl(const ScanArgs& a, uint32_t u)
@@ -1267,17 +1291,54 @@ __device__ __forceinline__ uint32_t unit_excl(const ScanArgs& a, uint32_t u)
     return u < a.nunits ? a.uoff[u] + a.bsum[u / kScanBlockItems] : a.bsum[(a.nunits + kScanBlockItems - 1) / kScanBlockItems];
 }
 
+// One wave per kScatterUnits consecutive units (query order).  Their outputs
+// form one contiguous range, the scan's offsets being exclusive sums: the wave
+// loads the units' counts and stage slots and the first unit's offset in one
+// load round, then copies the range in coalesced rows, each lane taking its
+// element from the last unit whose offset inside the range is not above it,
+// with kScatterRows rows read before any is written.  (One wave per unit was
+// two dependent load rounds for ~37 survivors on average.)
 __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
 {
     const uint32_t ln = lane_id();
-    const uint32_t u = blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (u < a.nunits) {
-        const uint32_t cnt = a.ucount[u];
-        if (cnt) {
-            const UnitRec ur = a.units[u];
-            const uint32_t* src = a.stage + a.queries[ur.q].short_off + (uint64_t)ur.k * kChunk;
-            uint32_t* dst = a.result + unit_excl(a, u);
-            for (uint32_t e = ln; e < cnt; e += 64) dst[e] = src[e];
+    const uint32_t u0 = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * kScatterUnits;
+    if (u0 < a.nunits) {
+        // loads are unconditional (clamped indexes, results selected after):
+        // exec-masked loads make the compiler wait for every outstanding
+        // access before each later store
+        const uint32_t base = uni(a.uoff[u0] + a.bsum[u0 / kScanBlockItems]);
+        const uint32_t u = u0 + ln;
+        const bool mine = ln < kScatterUnits && u < a.nunits;
+        const uint32_t uc = min(u0 + min(ln, kScatterUnits - 1), a.nunits - 1);
+        const uint32_t c_raw = a.ucount[uc], s_raw = a.units[uc].soff;
+        const uint32_t cnt = mine ? c_raw : 0u;
+        const uint32_t soff = mine ? s_raw : 0u;
+        // each unit's offset inside the range and its stage slot minus that
+        // offset (modulo 2^32: the sum with an element index is the slot)
+        uint32_t ex[kScatterUnits], sh[kScatterUnits], total = 0;
+#pragma unroll
+        for (int j = 0; j < (int)kScatterUnits; ++j) {
+            ex[j] = total;
+            sh[j] = uni(__builtin_amdgcn_readlane(soff, j)) - total;
+            total += uni(__builtin_amdgcn_readlane(cnt, j));
+        }
+        const uint32_t* __restrict__ src = a.stage;
+        uint32_t* __restrict__ dst = a.result + base;
+        for (uint32_t r0 = 0; r0 < total; r0 += 64u * kScatterRows) {
+            uint32_t x[kScatterRows];
+#pragma unroll
+            for (int i = 0; i < (int)kScatterRows; ++i) {
+                const uint32_t e = r0 + ln + 64u * (uint32_t)i;
+                uint32_t d = sh[0];
+#pragma unroll
+                for (int j = 1; j < (int)kScatterUnits; ++j) d = e >= ex[j] ? sh[j] : d;
+                x[i] = src[e < total ? e + d : 0u];
+            }
+#pragma unroll
+            for (int i = 0; i < (int)kScatterRows; ++i) {
+                const uint32_t e = r0 + ln + 64u * (uint32_t)i;
+                if (e < total) dst[e] = x[i];
+            }
         }
     }
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1428,9 +1489,12 @@ hipError_t launch_unit_prologue(const ProbeArgs& a, hipStream_t s)
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s)
 {
     const uint32_t nblk = (a.nunits + kScanBlockItems - 1) / kScanBlockItems;
-    if (nblk) hipLaunchKernelGGL(k_scan_local, dim3(nblk), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, a.bsum, nblk);
-    const uint32_t g1 = (a.nunits + kWaves - 1) / kWaves, g2 = (a.nq + 255) / 256;
+    if (nblk)
+        hipLaunchKernelGGL(k_scan_local, dim3(nblk), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(64), 0, s, a.bsum);
+    const uint32_t per_block = kWaves * kScatterUnits;
+    const uint32_t g1 = (a.nunits + per_block - 1) / per_block, g2 = (a.nq + 255) / 256;
     const uint32_t grid = g1 > g2 ? g1 : g2;
     if (grid) hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();

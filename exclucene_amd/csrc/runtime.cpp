@@ -770,7 +770,7 @@ int plan_queries(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64
         if (tc > 0xffff) return set_error(EXC_ERR_ARG, "query %u: more than 65535 terms", q);
         for (uint32_t k = 0; k < Q.nunits; ++k)
             c->units.push_back(UnitRec{q, k, Q.toff, (uint16_t)tc, (uint16_t)Q.knobs, 0, kUnset,
-                                       (uint32_t)c->units.size(), 0});
+                                       (uint32_t)c->units.size(), (uint32_t)(Q.short_off + (uint64_t)k * kChunk)});
         units += Q.nunits;
         if (units > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 work units in one batch");
     }
@@ -828,9 +828,11 @@ int upload_plan(exc_ctx* c, bool sync = true)
         (rc = dev_alloc(c, c->d_bsum, (size_t)(nblk + 1) * 4)) ||
         (rc = dev_alloc(c, c->d_result, std::max<uint64_t>(1, c->short_sum) * 4)) ||
         (rc = dev_alloc(c, c->d_qcum, std::max<size_t>(1, c->nq) * 4)) ||
-        (rc = dev_alloc(c, c->d_counter, 64)))
+        (rc = dev_alloc(c, c->d_counter, 256)))
         return rc;
     drop_graph(c);
+    // the scan's ticket starts at zero (every launch leaves it there)
+    HIP_TRY(hipMemsetAsync(dptr<uint32_t>(c->d_counter) + kScanTicketWord, 0, 4, c->stream));
     if (c->nq) HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice, c->stream));
     if (!c->qrecs.empty())
         HIP_TRY(hipMemcpyAsync(c->d_qrecs.p, c->qrecs.data(), c->qrecs.size() * sizeof(TermRec), hipMemcpyHostToDevice,
@@ -920,6 +922,7 @@ int enqueue_invoke(exc_ctx* c, bool timed)
     s.bsum = dptr<uint32_t>(c->d_bsum);
     s.result = dptr<uint32_t>(c->d_result);
     s.qcum = dptr<uint32_t>(c->d_qcum);
+    s.ticket = dptr<uint32_t>(c->d_counter) + kScanTicketWord;
     HIP_TRY(launch_scan(s, c->stream));
     return EXC_OK;
 }

@@ -163,3 +163,50 @@ def test_decompress_into_device_memory(ctx):
                 ctx.device_free(out)
         finally:
             dix.close()
+
+
+def test_scan_over_many_blocks(ctx):
+    """A batch of more than three scan blocks of units (4,096 units each): the
+    block that finishes last turns the block sums into offsets and leaves the
+    ticket at zero, so eager launches, graph replays, a one-block batch and a
+    batch without units that follow all give the exact intersections
+    (np.intersect1d over the lists themselves)."""
+    rng = np.random.default_rng(31)
+    lists = [np.sort(rng.choice(1 << 23, int(n), replace=False)).astype(np.uint32)
+             for n in rng.integers(400_000, 1_500_000, 12)]
+    lists.append(np.zeros(0, np.uint32))
+    docs, off = ea.lists_to_flat(lists)
+    dix = ctx.upload(ea.build_index(docs, off, ea.SEGLRC, seg=128))
+
+    def expect(qs):
+        cum, res, c = [], [], 0
+        for q in qs:
+            r = lists[q[0]]
+            for t in q[1:]:
+                r = np.intersect1d(r, lists[t], assume_unique=True)
+            res.append(r)
+            c += len(r)
+            cum.append(c)
+        return np.array(cum, np.int64), np.concatenate(res).astype(np.uint32)
+
+    qs = [[int(x) for x in rng.choice(12, int(rng.integers(2, 4)), replace=False)] for _ in range(24)]
+    units = sum(-(-min(len(lists[t]) for t in q) // 512) for q in qs)
+    assert units > 3 * 4096
+    batches = [qs, qs[:3], [[12, 0], [5, 12]]]
+    try:
+        for graph in (0, 1):
+            for persistent in (0, 1):
+                ctx.set_option(ea.OPT_GRAPH, graph)
+                ctx.set_option(ea.OPT_PERSISTENT, persistent)
+                for b in batches:
+                    ecum, eres = expect(b)
+                    bi = ctx.plan(dix, b)
+                    for rep in range(3):
+                        ctx.kernelInvoke(bi)
+                        cum, res = ctx.dtohTransfer(bi)
+                        assert np.array_equal(np.asarray(cum, np.int64), ecum), (graph, persistent, len(b), rep)
+                        assert np.array_equal(res, eres), (graph, persistent, len(b), rep)
+    finally:
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        dix.close()
