@@ -559,7 +559,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         hb0 = pre_first;
         rng = pre_count;
         small = rng <= 64;
-        H = (small && (int64_t)ln < rng) ? head[hb0 + ln] : kNone;
+        H = (small && ln < (uint32_t)rng) ? (head + hb0)[ln] : kNone;
     } else if (search != SEARCH_BINARY && nseg > 64) {
         int64_t wa, t0, t1, wb;
         if (search == SEARCH_LEARNED && R.model != kNoModel) {
@@ -572,8 +572,10 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         wb = min(max(wb, wa), nseg);
         if (wb - wa + 2 <= 64) {
             hb0 = wa - 1;
-            const int64_t idx = hb0 + ln;
-            H = (idx < 0) ? 0u : ((idx < nseg && (int64_t)ln <= wb - wa + 1) ? head[idx] : kNone);
+            // lanes below lim read head[hb0 + lane] (lane 0 of hb0 = -1 reads
+            // as 0): 32-bit lane arithmetic against uniform bounds
+            const uint32_t lim = uni((uint32_t)min(nseg - hb0, wb - wa + 2));
+            H = (hb0 < 0 && ln == 0) ? 0u : (ln < lim ? (head + hb0)[ln] : kNone);
             const uint32_t h0 = uni(__builtin_amdgcn_readlane(H, 0));
             const uint32_t hend = uni(__builtin_amdgcn_readlane(H, (int)(wb - wa + 1)));
             small = h0 <= vmin && hend > vmax;
@@ -586,7 +588,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         hb0 = ub_lo ? ub_lo - 1 : 0;
         rng = (int64_t)ub_hi - hb0;
         small = rng <= 64;
-        H = (small && (int64_t)ln < rng) ? head[hb0 + ln] : kNone;
+        H = (small && ln < (uint32_t)rng) ? (head + hb0)[ln] : kNone;
     }
     // segments spanned by the candidate set
     int32_t s_first, s_last;
