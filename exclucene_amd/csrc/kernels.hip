@@ -1196,10 +1196,42 @@ __global__ __launch_bounds__(256) void k_unit_prologue(ProbeArgs A)
     const uint32_t vmax = (b0 + nb < R0.nseg) ? h0[b0 + nb] - 1u : 0xffffffffu;
     const uint32_t* h1 = A.ix.seg_head + R1.segnum;
     const uint32_t n = R1.nseg;
-    uint32_t a = 0, b = 0;  // upper bounds of vmin and vmax
-    for (uint32_t step = n ? 1u << (31 - __builtin_clz(n)) : 0u; step; step >>= 1) {
-        if (a + step <= n && h1[a + step - 1] <= vmin) a += step;
-        if (b + step <= n && h1[b + step - 1] <= vmax) b += step;
+    // upper bounds of vmin and vmax (heads <= v) are searched in [lo, lo + len]:
+    // the whole list, or, for the fitted codecs, the window the list fit gives
+    // once two bracket loads confirm it holds the bound (a window that misses
+    // falls back to the whole list, so the fit's rounding never matters)
+    uint32_t lo_a = 0, len_a = n, lo_b = 0, len_b = n;
+    if ((A.ix.codec == EXC_SEGLRC || A.ix.codec == EXC_LRC) && n > 64) {
+        int64_t wa0, wb0, wa1, wb1;
+        if (A.ix.codec == EXC_SEGLRC) {
+            fit_window<EXC_SEGLRC, SEG>(R1, vmin, wa0, wb0);
+            fit_window<EXC_SEGLRC, SEG>(R1, vmax, wa1, wb1);
+        } else {
+            fit_window<EXC_LRC, SEG>(R1, vmin, wa0, wb0);
+            fit_window<EXC_LRC, SEG>(R1, vmax, wa1, wb1);
+        }
+        wa0 = min(wa0, (int64_t)n);
+        wb0 = min(max(wb0, wa0), (int64_t)n);
+        wa1 = min(wa1, (int64_t)n);
+        wb1 = min(max(wb1, wa1), (int64_t)n);
+        // the bound of v lies in [wa, wb] iff head[wa - 1] <= v (or wa = 0)
+        // and head[wb] > v (or wb = n)
+        const uint32_t l0 = wa0 ? h1[wa0 - 1] : 0u, r0 = wb0 < n ? h1[wb0] : 0xffffffffu;
+        const uint32_t l1 = wa1 ? h1[wa1 - 1] : 0u, r1 = wb1 < n ? h1[wb1] : 0xffffffffu;
+        if (l0 <= vmin && (wb0 == n || r0 > vmin)) {
+            lo_a = (uint32_t)wa0;
+            len_a = (uint32_t)(wb0 - wa0);
+        }
+        if (l1 <= vmax && (wb1 == n || r1 > vmax)) {
+            lo_b = (uint32_t)wa1;
+            len_b = (uint32_t)(wb1 - wa1);
+        }
+    }
+    uint32_t a = lo_a, b = lo_b;
+    const uint32_t len = max(len_a, len_b);
+    for (uint32_t step = len ? 1u << (31 - __builtin_clz(len)) : 0u; step; step >>= 1) {
+        if (a + step <= lo_a + len_a && h1[a + step - 1] <= vmin) a += step;
+        if (b + step <= lo_b + len_b && h1[b + step - 1] <= vmax) b += step;
     }
     const uint32_t first = a ? a - 1 : 0;
     A.units[u].l1_first = first;
