@@ -94,6 +94,10 @@ struct exc_ctx {
     DevBuf d_launch;  // the units again, in launch order (build_launch)
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // around the probe inside a step: timing only, without the system-scope
+    // fence (its L2 writeback and invalidation otherwise stall the kernels
+    // around it); the step's own events ev[0] / ev[3] keep the fence
+    hipEvent_t pev[2] = {nullptr, nullptr};
     float probe_ms = 0, invoke_ms = 0;
     // graph of the current plan
     hipGraphExec_t gexec = nullptr;
@@ -291,6 +295,7 @@ extern "C" int exc_ctx_create(int device, uint64_t budget, exc_ctx** out)
         return set_error(EXC_ERR_DEVICE, "hipStreamCreate failed");
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
+    for (auto& e : c->pev) (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     *out = c;
     return EXC_OK;
 }
@@ -306,6 +311,8 @@ extern "C" void exc_ctx_destroy(exc_ctx* c)
         dev_free(c, *b);
     for (auto& kv : c->user_bufs) (void)hipFree(kv.first);
     for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->pev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -907,9 +914,9 @@ int enqueue_invoke(exc_ctx* c, bool timed)
         grid = std::max<uint32_t>(1, (c->nunits + wpb - 1) / wpb);
     }
     if (c->nunits) HIP_TRY(launch_unit_prologue(a, c->stream));
-    if (timed) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    if (timed) HIP_TRY(hipEventRecord(c->pev[0], c->stream));
     if (c->nunits) HIP_TRY(launch_probe(a, grid, c->stream));
-    if (timed) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    if (timed) HIP_TRY(hipEventRecord(c->pev[1], c->stream));
     ScanArgs s;
     std::memset(&s, 0, sizeof s);
     s.queries = a.queries;
@@ -957,7 +964,7 @@ int invoke(exc_ctx* c)
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->probe_ms = c->invoke_ms = 0;
     if (timed) {
-        (void)hipEventElapsedTime(&c->probe_ms, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&c->probe_ms, c->pev[0], c->pev[1]);
         (void)hipEventElapsedTime(&c->invoke_ms, c->ev[0], c->ev[3]);
     } else if (c->timing) {
         (void)hipEventElapsedTime(&c->invoke_ms, c->ev[0], c->ev[3]);
