@@ -29,7 +29,7 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 # k_probe<SegLRC,128> HBM bytes per launch on the C2 batch with the tuned knobs, in KiB: rocprofv3
 # FETCH_SIZE (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
-TRAFFIC_KIB = 1427963.0
+TRAFFIC_KIB = 1302880.0
 C5_SCALE = 5.8e9  # C of df_k = min(2^25, C / k^1.1): about 25.4G postings, 40 GB resident
 META = {0: 8, 1: 28, 2: 32, 3: 8, 4: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD, NewPFD
 CODEC_NAME = {0: "LRC", 1: "SegLRC", 2: "HS_SegLRC", 3: "ParaPFD", 4: "NewPFD"}
