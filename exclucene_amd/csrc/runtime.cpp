@@ -98,6 +98,7 @@ struct exc_ctx {
     // fence (its L2 writeback and invalidation otherwise stall the kernels
     // around it); the step's own events ev[0] / ev[3] keep the fence
     hipEvent_t pev[2] = {nullptr, nullptr};
+    void* ticket_zeroed = nullptr;  // d_counter allocation whose scan ticket was zeroed
     float probe_ms = 0, invoke_ms = 0;
     // graph of the current plan
     hipGraphExec_t gexec = nullptr;
@@ -838,8 +839,12 @@ int upload_plan(exc_ctx* c, bool sync = true)
         (rc = dev_alloc(c, c->d_counter, 256)))
         return rc;
     drop_graph(c);
-    // the scan's ticket starts at zero (every launch leaves it there)
-    HIP_TRY(hipMemsetAsync(dptr<uint32_t>(c->d_counter) + kScanTicketWord, 0, 4, c->stream));
+    // the scan's ticket starts at zero when its buffer is new; every launch
+    // leaves it at zero
+    if (c->d_counter.p != c->ticket_zeroed) {
+        HIP_TRY(hipMemsetAsync(dptr<uint32_t>(c->d_counter) + kScanTicketWord, 0, 4, c->stream));
+        c->ticket_zeroed = c->d_counter.p;
+    }
     if (c->nq) HIP_TRY(hipMemcpyAsync(c->d_queries.p, c->queries.data(), c->nq * sizeof(QueryRec), hipMemcpyHostToDevice, c->stream));
     if (!c->qrecs.empty())
         HIP_TRY(hipMemcpyAsync(c->d_qrecs.p, c->qrecs.data(), c->qrecs.size() * sizeof(TermRec), hipMemcpyHostToDevice,
