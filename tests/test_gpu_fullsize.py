@@ -1,11 +1,12 @@
 """GPU parity at the full C2 size: 100,000 Zipf-distributed lists over 2^25
-docIDs (369 M postings), SegLRC-128, the 1,000-query C2 batch of 2-5 terms.
+docIDs (369 M postings), the 1,000-query C2 batch of 2-5 terms.
 
-The device's cumulative counts and results must equal the oracle's exactly
-(its C restatement of decode + intersect, multithreaded, over the same
-encoded arrays), for both launch modes, and must satisfy properties checked
-against the raw lists themselves: each query's results ascend strictly and
-every result is in every list of its query."""
+For SegLRC-128, LRC-256 and ParaPFD-128 (FRAC 0.1) the device's cumulative
+counts and results must equal the oracle's exactly (its C restatement of
+decode + intersect, multithreaded, over the same encoded arrays), in both
+launch modes; the uncompressed lists must give the same output.  The SegLRC
+results are further checked against the raw lists themselves: each query's
+results ascend strictly and every result is in every list of its query."""
 import os
 
 import numpy as np
@@ -27,31 +28,60 @@ def _oracle_index(built):
                         hash_info=built.hash_info)
 
 
-def test_c2_full_size_matches_oracle(ctx):
+def _threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+@pytest.fixture(scope="module")
+def c2():
     df, docs, off = W.c2_corpus()
     assert len(docs) > 369_000_000
     queries = W.c2_queries(df)
     built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=threads)
+    cum, res = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
+    return docs, off, queries, built, np.asarray(cum, np.int64), res
+
+
+def _gpu_both_modes(ctx, built, queries):
     dix = ctx.upload(built)
+    out = []
     try:
         for persistent in (0, 1):
             ctx.set_option(ea.OPT_PERSISTENT, persistent)
             cum, res = ctx.intersect(dix, queries)
-            assert np.array_equal(np.asarray(cum, np.int64), np.asarray(ocum, np.int64)), persistent
-            assert np.array_equal(res, ores), persistent
+            out.append((np.asarray(cum, np.int64), res))
     finally:
         ctx.set_option(ea.OPT_PERSISTENT, 1)
         dix.close()
+    return out
 
+
+def test_c2_full_size_seglrc(ctx, c2):
+    docs, off, queries, built, ocum, ores = c2
+    for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
+        assert np.array_equal(cum, ocum), mode
+        assert np.array_equal(res, ores), mode
     # properties against the raw lists
-    cum = np.asarray(cum, np.int64)
-    start = np.concatenate([[0], cum[:-1]])
+    start = np.concatenate([[0], ocum[:-1]])
     for q, terms in enumerate(queries):
-        r = res[start[q]:cum[q]]
+        r = ores[start[q]:ocum[q]]
         assert np.all(np.diff(r.astype(np.int64)) > 0), q
         for t in terms:
             lst = docs[off[t]:off[t + 1]]
             pos = np.minimum(np.searchsorted(lst, r), max(len(lst) - 1, 0))
             assert len(r) == 0 or np.array_equal(lst[pos], r), (q, t)
+
+
+@pytest.mark.parametrize("codec,seg,frac", [(ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1), (ea.RAW, 128, 0.0)])
+def test_c2_full_size_other_codecs(ctx, c2, codec, seg, frac):
+    docs, off, queries, _, scum, sres = c2
+    built = ea.build_index(docs, off, codec, seg=seg, frac=frac)
+    if codec == ea.RAW:
+        ocum, ores = scum, sres  # the uncompressed lists answer like any encoding of them
+    else:
+        ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
+        ocum = np.asarray(ocum, np.int64)
+        assert np.array_equal(ocum, scum) and np.array_equal(ores, sres)
+    for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
+        assert np.array_equal(cum, ocum), mode
+        assert np.array_equal(res, ores), mode
