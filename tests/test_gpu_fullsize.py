@@ -4,7 +4,8 @@ docIDs (369 M postings), the 1,000-query C2 batch of 2-5 terms.
 For SegLRC-128, LRC-256 and ParaPFD-128 (FRAC 0.1) the device's cumulative
 counts and results must equal the oracle's exactly (its C restatement of
 decode + intersect, multithreaded, over the same encoded arrays), in both
-launch modes; the uncompressed lists must give the same output.  The SegLRC
+launch modes; the uncompressed lists must give the same output, and so must
+the C3 hash buckets (shift 13) for a share of the ratio queries.  The SegLRC
 results are further checked against the raw lists themselves: each query's
 results ascend strictly and every result is in every list of its query."""
 import os
@@ -39,7 +40,7 @@ def c2():
     queries = W.c2_queries(df)
     built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
     cum, res = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
-    return docs, off, queries, built, np.asarray(cum, np.int64), res
+    return docs, off, queries, built, np.asarray(cum, np.int64), res, df
 
 
 def _gpu_both_modes(ctx, built, queries):
@@ -57,7 +58,7 @@ def _gpu_both_modes(ctx, built, queries):
 
 
 def test_c2_full_size_seglrc(ctx, c2):
-    docs, off, queries, built, ocum, ores = c2
+    docs, off, queries, built, ocum, ores, _ = c2
     for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
         assert np.array_equal(cum, ocum), mode
         assert np.array_equal(res, ores), mode
@@ -74,7 +75,7 @@ def test_c2_full_size_seglrc(ctx, c2):
 
 @pytest.mark.parametrize("codec,seg,frac", [(ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1), (ea.RAW, 128, 0.0)])
 def test_c2_full_size_other_codecs(ctx, c2, codec, seg, frac):
-    docs, off, queries, _, scum, sres = c2
+    docs, off, queries, _, scum, sres, _ = c2
     built = ea.build_index(docs, off, codec, seg=seg, frac=frac)
     if codec == ea.RAW:
         ocum, ores = scum, sres  # the uncompressed lists answer like any encoding of them
@@ -82,6 +83,20 @@ def test_c2_full_size_other_codecs(ctx, c2, codec, seg, frac):
         ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
         ocum = np.asarray(ocum, np.int64)
         assert np.array_equal(ocum, scum) and np.array_equal(ores, sres)
+    for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
+        assert np.array_equal(cum, ocum), mode
+        assert np.array_equal(res, ores), mode
+
+
+def test_c3_full_size_hash_buckets(ctx, c2):
+    """C3: the full-size lists in hash buckets of shift 13 (the top 12 bits of
+    2^25), the first 2,000 of the 10,000 ratio queries (long:short 10^U(1,4)):
+    equal to the oracle over the same buckets, in both launch modes."""
+    docs, off, _, _, _, _, df = c2
+    queries = W.ratio_queries(df)[:2000]
+    built = ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=13)
+    ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
+    ocum = np.asarray(ocum, np.int64)
     for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
         assert np.array_equal(cum, ocum), mode
         assert np.array_equal(res, ores), mode
