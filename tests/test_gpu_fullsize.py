@@ -7,7 +7,9 @@ decode + intersect, multithreaded, over the same encoded arrays), in both
 launch modes; the uncompressed lists must give the same output, and so must
 the C3 hash buckets (shift 13) for a share of the ratio queries.  The SegLRC
 results are further checked against the raw lists themselves: each query's
-results ascend strictly and every result is in every list of its query."""
+results ascend strictly and every result is in every list of its query.
+Whole-list decoding of all 100,000 lists must return them unchanged for every
+codec, NewPFD included."""
 import os
 
 import numpy as np
@@ -100,3 +102,18 @@ def test_c3_full_size_hash_buckets(ctx, c2):
     for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
         assert np.array_equal(cum, ocum), mode
         assert np.array_equal(res, ores), mode
+
+
+@pytest.mark.parametrize("codec,seg,frac", [(ea.SEGLRC, 128, 0.0), (ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1),
+                                            (ea.NEWPFD, 256, 0.1)])
+def test_c2_full_size_decode(ctx, c2, codec, seg, frac):
+    """Every one of the 100,000 full-size lists decoded on the device in one
+    call equals the list it was built from."""
+    docs, off, _, built0, _, _, _ = c2
+    built = built0 if codec == ea.SEGLRC else ea.build_index(docs, off, codec, seg=seg, frac=frac)
+    dix = ctx.upload(built)
+    try:
+        out = ctx.decompress(dix, np.arange(len(off) - 1, dtype=np.uint32))
+    finally:
+        dix.close()
+    assert np.array_equal(out, docs)
