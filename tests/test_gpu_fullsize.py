@@ -9,7 +9,8 @@ the C3 hash buckets (shift 13) for a share of the ratio queries.  The SegLRC
 results are further checked against the raw lists themselves: each query's
 results ascend strictly and every result is in every list of its query.
 Whole-list decoding of all 100,000 lists must return them unchanged for every
-codec, NewPFD included."""
+codec, NewPFD included, and the device encoder must produce the host
+builder's segment arrays byte for byte."""
 import os
 
 import numpy as np
@@ -117,3 +118,24 @@ def test_c2_full_size_decode(ctx, c2, codec, seg, frac):
     finally:
         dix.close()
     assert np.array_equal(out, docs)
+
+
+def test_c2_full_size_device_builder(ctx, c2):
+    """The device encoder over the full-size lists: the index's segment-level
+    arrays byte-identical to the host builder's, and the C2 batch answered on
+    the device-built index exactly as the oracle answers it."""
+    docs, off, queries, built, ocum, ores, _ = c2
+    d = ctx.device_alloc(len(docs) * 4)
+    try:
+        ctx.to_device(d, docs)
+        dix = ctx.build_index_device(d, off, seg=128)
+        try:
+            dev = ctx.export(dix)
+            for name in ("ind1", "ind2", "seg_num", "seg_offset", "seg_head", "seg_info", "seg_base"):
+                assert np.array_equal(getattr(dev, name), getattr(built, name)), name
+            cum, res = ctx.intersect(dix, queries)
+            assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
+        finally:
+            dix.close()
+    finally:
+        ctx.device_free(d)
