@@ -9,8 +9,9 @@ the C3 hash buckets (shift 13) for a share of the ratio queries.  The SegLRC
 results are further checked against the raw lists themselves: each query's
 results ascend strictly and every result is in every list of its query.
 Whole-list decoding of all 100,000 lists must return them unchanged for every
-codec, NewPFD included, and the device encoder must produce the host
-builder's segment arrays byte for byte."""
+codec, NewPFD included, the device encoder must produce the host builder's
+segment arrays byte for byte, and the index written to the reference's files
+must answer the same after loading."""
 import os
 
 import numpy as np
@@ -139,3 +140,17 @@ def test_c2_full_size_device_builder(ctx, c2):
             dix.close()
     finally:
         ctx.device_free(d)
+
+
+def test_c2_full_size_from_files(ctx, c2, tmp_path):
+    """The full-size SegLRC-128 index written in the reference's files and
+    loaded back through the mapped, validated, staged loader answers the C2
+    batch as the oracle does."""
+    docs, off, queries, built, ocum, ores, _ = c2
+    built.save(str(tmp_path), "c2")
+    lix = ctx.load_files(ea.SEGLRC, 128, str(tmp_path), "c2")
+    try:
+        cum, res = ctx.intersect(lix, queries)
+        assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
+    finally:
+        lix.close()
