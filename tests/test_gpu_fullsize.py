@@ -10,8 +10,9 @@ results are further checked against the raw lists themselves: each query's
 results ascend strictly and every result is in every list of its query.
 Whole-list decoding of all 100,000 lists must return them unchanged for every
 codec, NewPFD included, the device encoder must produce the host builder's
-segment arrays byte for byte, and the index written to the reference's files
-must answer the same after loading."""
+segment arrays byte for byte, the index written to the reference's files
+must answer the same after loading, and so must the learned window search
+(PLA and MLP models on every list longer than 65,536)."""
 import os
 
 import numpy as np
@@ -154,3 +155,24 @@ def test_c2_full_size_from_files(ctx, c2, tmp_path):
         assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
     finally:
         lix.close()
+
+
+@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP])
+def test_c2_full_size_learned_search(ctx, c2, kind):
+    """C4's learned window search at full size: a position model on every list
+    longer than 65,536 (511 lists), the C2 batch answered through the models'
+    windows exactly as the oracle answers it."""
+    docs, off, queries, built, ocum, ores, _ = c2
+    dix = ctx.upload(built)
+    try:
+        if kind == ea.MODEL_PLA:
+            st = ctx.train_models(dix, kind, min_len=65536)
+        else:
+            st = ctx.train_models(dix, kind, min_len=65536, epochs=200, samples=4096, lr=1e-3)
+        assert st["nmodels"] == int((np.diff(off.astype(np.int64)) > 65536).sum())
+        ctx.set_option(ea.OPT_SEARCH, ea.SEARCH_LEARNED)
+        cum, res = ctx.intersect(dix, queries)
+        assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
+    finally:
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        dix.close()
