@@ -12,7 +12,8 @@ Whole-list decoding of all 100,000 lists must return them unchanged for every
 codec, NewPFD included, the device encoder must produce the host builder's
 segment arrays byte for byte, the index written to the reference's files
 must answer the same after loading, and so must the learned window search
-(PLA and MLP models on every list longer than 65,536)."""
+(PLA and MLP models on every list longer than 65,536) and the
+continuous-batching server."""
 import os
 
 import numpy as np
@@ -175,4 +176,23 @@ def test_c2_full_size_learned_search(ctx, c2, kind):
         assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
     finally:
         ctx.set_option(ea.OPT_SEARCH, 1)
+        dix.close()
+
+
+def test_c2_full_size_served(ctx, c2):
+    """The C2 batch served by the continuous-batching loop at full size (three
+    slots, Poisson arrivals at 20 k queries/s): every query's count and
+    results equal the oracle's."""
+    docs, off, queries, built, ocum, ores, _ = c2
+    dix = ctx.upload(built)
+    try:
+        arrival = np.cumsum(np.random.default_rng(5).exponential(1 / 20000.0, len(queries)))
+        srv = ea.Server(ctx, dix, nslots=3, max_batch=256)
+        try:
+            lat, cnt, res = srv.replay(queries, arrival, int(ocum[-1]))
+        finally:
+            srv.close()
+        assert np.array_equal(cnt.astype(np.int64), np.diff(np.concatenate([[0], ocum])))
+        assert np.array_equal(res, ores) and np.all(lat > 0)
+    finally:
         dix.close()
