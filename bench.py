@@ -495,8 +495,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard by "
-              "replicating the index or by docID range; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (a batch would shard across "
+              "GPUs by replicating the index or by docID range)", file=sys.stderr)
         sys.exit(2)
 
     import exclucene_amd as ea
