@@ -5,7 +5,8 @@ For SegLRC-128, LRC-256 and ParaPFD-128 (FRAC 0.1) the device's cumulative
 counts and results must equal the oracle's exactly (its C restatement of
 decode + intersect, multithreaded, over the same encoded arrays), in both
 launch modes; the uncompressed lists must give the same output, and so must
-the C3 hash buckets (shift 13) for a share of the ratio queries.  The SegLRC
+the C3 hash buckets (shift 13 and the adaptive width) for a share of the
+ratio queries.  The SegLRC
 results are further checked against the raw lists themselves: each query's
 results ascend strictly and every result is in every list of its query.
 Whole-list decoding of all 100,000 lists must return them unchanged for every
@@ -94,13 +95,15 @@ def test_c2_full_size_other_codecs(ctx, c2, codec, seg, frac):
         assert np.array_equal(res, ores), mode
 
 
-def test_c3_full_size_hash_buckets(ctx, c2):
+@pytest.mark.parametrize("shift", [13, 0])
+def test_c3_full_size_hash_buckets(ctx, c2, shift):
     """C3: the full-size lists in hash buckets of shift 13 (the top 12 bits of
-    2^25), the first 2,000 of the 10,000 ratio queries (long:short 10^U(1,4)):
-    equal to the oracle over the same buckets, in both launch modes."""
+    2^25) or of the adaptive width (about 256 docIDs per bucket, per list), the
+    first 2,000 of the 10,000 ratio queries (long:short 10^U(1,4)): equal to
+    the oracle over the same buckets, in both launch modes."""
     docs, off, _, _, _, _, df = c2
     queries = W.ratio_queries(df)[:2000]
-    built = ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=13)
+    built = ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=shift)
     ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
     ocum = np.asarray(ocum, np.int64)
     for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
