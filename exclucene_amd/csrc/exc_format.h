@@ -99,6 +99,27 @@ EXC_HD uint32_t s16_count(uint32_t word)
     return (L & 31u) + ((L >> 10) & 31u) + ((L >> 20) & 31u);
 }
 
+// field k of a Simple16 word, k < s16_count(word) (larger k: some value,
+// never a shift of 32 or more)
+EXC_HD uint32_t s16_field(uint32_t word, uint32_t k)
+{
+    const uint32_t L = s16_layout(word >> 28);
+    const uint32_t c0 = L & 31u, w0 = (L >> 5) & 31u, c1 = (L >> 10) & 31u, w1 = (L >> 15) & 31u;
+    const uint32_t w2 = (L >> 25) & 31u;
+    uint32_t sh, w;
+    if (k < c0) {
+        sh = k * w0;
+        w = w0;
+    } else if (k < c0 + c1) {
+        sh = c0 * w0 + (k - c0) * w1;
+        w = w1;
+    } else {
+        sh = c0 * w0 + c1 * w1 + (k - c0 - c1) * w2;
+        w = w2;
+    }
+    return (word >> (sh < 31u ? sh : 31u)) & ((1u << w) - 1u);
+}
+
 // the fields of one Simple16 word into out[0 .. min(count, lim))
 EXC_HD void s16_unpack(uint32_t word, uint32_t* out, uint32_t lim)
 {

@@ -152,8 +152,10 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
 // already there).  The reference decodes the Simple16 stream and patches in
 // thread 0 alone (GC/NewPFD/NewPFD_Decompression/NewPFD_Decompression_kernel.cu:420-447);
 // here each lane takes one word of the stream, a wave scan of the words'
-// field counts places their fields in xs[0 .. 2 en), a scan of the position
-// deltas gives every exception its position, and the patch is lane-parallel.
+// field counts gives every word its first field, each lane then extracts the
+// fields l, l + 64, ... of xs[0 .. 2 en) from the words that hold them, a
+// scan of the position deltas gives every exception its position, and the
+// patch is lane-parallel.
 // Stream words are read 64 at a time: past the stream's end a round reads at
 // most 63 words, inside the list's successors or the array's zero padding.
 // w_first: this lane's word of the first round, loaded with the block's fields.
@@ -165,9 +167,24 @@ __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint
         const uint32_t word = w0 ? sp[w0 + ln] : w_first;
         const uint32_t c = s16_count(word);
         const uint32_t incl = wave_incl_scan(c);
-        const uint32_t at = got + incl - c;
-        if (at < need) s16_unpack(word, xs + at, need - at);
-        got += uni(__shfl(incl, 63));
+        const uint32_t ex = incl - c;  // the round's first field in this lane's word
+        const uint32_t tot = uni(__shfl(incl, 63));
+        const uint32_t nf = min(tot, need - got);
+        // field-parallel unpack: lane l takes fields l, l + 64, ... of the
+        // round; its word is the last lane whose first field is <= f (binary
+        // lifting over the lanes, ex strictly increases since every word holds
+        // a field), so no lane loops over a long word while the others idle
+        for (uint32_t f0 = 0; f0 < nf; f0 += 64) {
+            const uint32_t f = f0 + ln;
+            uint32_t o = 0;
+            for (uint32_t step = 32; step; step >>= 1)
+                if ((uint32_t)__shfl((int)ex, (int)(o + step)) <= f) o += step;
+            const uint32_t wo = (uint32_t)__shfl((int)word, (int)o);
+            const uint32_t k = f - (uint32_t)__shfl((int)ex, (int)o);
+            const uint32_t v = s16_field(wo, k);
+            if (f < nf) xs[got + f] = v;
+        }
+        got += tot;
     }
     wave_sync();
     // position i = all[0] + sum over m = 1..i of (all[m] + 1)  (GI/NewPFor.cpp:119-124)
