@@ -72,14 +72,24 @@ __device__ __forceinline__ uint32_t bits_at(const uint32_t* __restrict__ p, uint
     return (uint32_t)(cw >> (bit & 31)) & ((1u << lb) - 1u);
 }
 
+// Inclusive sum over the wave's lanes in DPP moves (VALU only, no LDS-unit
+// permutes): row_shr 1, 2, 4, 8 within each row of 16 lanes (sources before
+// the row start read 0), then rows 1 and 3 add lane 15 of the row before
+// (row_bcast:15) and rows 2 and 3 add lane 31 (row_bcast:31).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_add(uint32_t v)
+{
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-    const uint32_t ln = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o);
-        if (ln >= (uint32_t)o) v += t;
-    }
+    v = dpp_add<0x111, 0xf>(v);  // row_shr:1
+    v = dpp_add<0x112, 0xf>(v);  // row_shr:2
+    v = dpp_add<0x114, 0xf>(v);  // row_shr:4
+    v = dpp_add<0x118, 0xf>(v);  // row_shr:8
+    v = dpp_add<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -168,7 +178,7 @@ __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint
         const uint32_t c = s16_count(word);
         const uint32_t incl = wave_incl_scan(c);
         const uint32_t ex = incl - c;  // the round's first field in this lane's word
-        const uint32_t tot = uni(__shfl(incl, 63));
+        const uint32_t tot = uni(__builtin_amdgcn_readlane(incl, 63));
         const uint32_t nf = min(tot, need - got);
         // field-parallel unpack: lane l takes fields l, l + 64, ... of the
         // round; its word is the last lane whose first field is <= f (binary
@@ -193,7 +203,7 @@ __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint
         const uint32_t d = e < en ? xs[e] + (e ? 1u : 0u) : 0u;
         const uint32_t p = wave_incl_scan(d) + carry;
         if (e < en && p < len) dst[p] += xs[en + e] << lb;
-        carry = uni(__shfl(p, 63));
+        carry = uni(__builtin_amdgcn_readlane(p, 63));
     }
 }
 
