@@ -820,8 +820,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             uint32_t nd = 0, di[V];
 #pragma unroll
             for (int i = 0; i < V; ++i) {
-                // segment of the previous candidate: lane - 1, or lane 63 of the previous row
-                int32_t sp = __shfl_up(s[i], 1);
+                // segment of the previous candidate: lane - 1 (a DPP wave_shr:1
+                // move), or lane 63 of the previous row
+                int32_t sp = __builtin_amdgcn_update_dpp(0, s[i], 0x138, 0xf, 0xf, false);
                 if (ln == 0) sp = (i > 0) ? (int32_t)__builtin_amdgcn_readlane((uint32_t)s[i > 0 ? i - 1 : 0], 63) : -2;
                 const bool first = s[i] >= 0 && s[i] != sp;
                 const uint64_t m = __ballot(first);
