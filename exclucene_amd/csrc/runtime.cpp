@@ -1062,6 +1062,8 @@ extern "C" int exc_dtohTransfer(exc_ctx* c, const exc_batchInfo* bi, uint32_t* c
 extern "C" int exc_verify(const exc_batchInfo* bi, const uint32_t* cum, const uint32_t* res, exc_checkSum* cs)
 {
     if (!bi || !cs || (bi->nTotalQueryNum && !cum)) return set_error(EXC_ERR_ARG, "exc_verify: null argument");
+    if (bi->nTotalQueryNum && cum[bi->nTotalQueryNum - 1] && !res)
+        return set_error(EXC_ERR_ARG, "exc_verify: null result buffer for %u results", cum[bi->nTotalQueryNum - 1]);
     for (uint32_t i = 0; i < bi->nTotalQueryNum; ++i) {
         const uint32_t b = i ? cum[i - 1] : 0, n = cum[i] - b;
         if (n) {
@@ -1888,8 +1890,8 @@ struct exc_server {
     uint64_t next_ticket = 0;
     std::thread loop;
     std::atomic<bool> stop{false};
-    // stats
-    uint64_t batches = 0, queries_done = 0, max_inflight_batches = 0, overlap_launches = 0;
+    // stats: written by the request loop, read by exc_server_stats from any thread
+    std::atomic<uint64_t> batches{0}, queries_done{0}, max_inflight_batches{0}, overlap_launches{0};
 };
 
 namespace {
@@ -2034,7 +2036,7 @@ void server_loop(exc_server* sv)
             uint32_t busy_now = 0;
             for (auto& o : sv->slots) busy_now += o.busy;
             if (busy_now) ++sv->overlap_launches;
-            sv->max_inflight_batches = std::max<uint64_t>(sv->max_inflight_batches, busy_now + 1);
+            if (busy_now + 1 > sv->max_inflight_batches.load()) sv->max_inflight_batches.store(busy_now + 1);
             const int rc = slot_launch(sv, sl);
             ++sv->batches;
             sl.busy = true;
@@ -2124,27 +2126,32 @@ extern "C" int exc_server_wait(exc_server* sv, uint64_t ticket, uint32_t* result
     std::unique_lock<std::mutex> lk(sv->mu);
     auto it = sv->reqs.find(ticket);
     if (it == sv->reqs.end()) return set_error(EXC_ERR_ARG, "unknown or already collected ticket");
-    sv->cv_done.wait(lk, [&] { return it->second.done; });
-    auto& r = it->second;
+    // the wait releases the mutex and a concurrent submit may rehash the map:
+    // iterators die with a rehash, references to the element do not
+    exc_server::Req& r = it->second;
+    sv->cv_done.wait(lk, [&] { return r.done; });
     if (n) *n = r.count;
     if (latency_ms) *latency_ms = r.latency_ms;
     if (r.err) {
         const int e = r.err;
-        sv->reqs.erase(it);
+        sv->reqs.erase(ticket);
         return set_error(e, "query failed on the device");
     }
     // a too-small buffer keeps the request so the caller can retry with *n
     if (!(sv->flags & EXC_SERVER_COUNTS) && r.count > cap)
         return set_error(EXC_ERR_CAPACITY, "result buffer too small");
     if (results && r.buf && r.count) std::memcpy(results, r.buf->p + r.off, r.count * 4);
-    sv->reqs.erase(it);
+    sv->reqs.erase(ticket);
     return EXC_OK;
 }
 
 extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const uint64_t* qoff, const double* arrival_s,
                                  uint32_t nq, double* lat_ms, uint32_t* counts, uint32_t* results, uint64_t res_cap)
 {
-    if (!sv || !qoff || !arrival_s) return set_error(EXC_ERR_ARG, "exc_server_replay: bad argument");
+    if (!sv || !qoff || !arrival_s || (qoff[nq] && !qterms))
+        return set_error(EXC_ERR_ARG, "exc_server_replay: bad argument");
+    for (uint32_t q = 0; q < nq; ++q)
+        if (qoff[q + 1] < qoff[q]) return set_error(EXC_ERR_ARG, "exc_server_replay: query %u: bad offsets", q);
     for (uint64_t k = 0; k < qoff[nq]; ++k)
         if (qterms[k] >= sv->ix->nterms) return set_error(EXC_ERR_ARG, "query term %u out of range", qterms[k]);
     std::vector<uint64_t> tickets(nq);
@@ -2157,9 +2164,9 @@ extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const u
         for (uint32_t q = 0; q < nq; ++q) {
             while (submitted.load(std::memory_order_acquire) <= q) std::this_thread::yield();
             std::unique_lock<std::mutex> lk(sv->mu);
-            auto it = sv->reqs.find(tickets[q]);
-            sv->cv_done.wait(lk, [&] { return it->second.done; });
-            auto& r = it->second;
+            // a reference, not the iterator, across the wait (submits rehash)
+            exc_server::Req& r = sv->reqs.at(tickets[q]);
+            sv->cv_done.wait(lk, [&] { return r.done; });
             if (r.err && !crc) {
                 crc = r.err;
                 cq = (int)q;
@@ -2177,7 +2184,7 @@ extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const u
                 }
             }
             pos += r.buf ? r.count : 0;
-            sv->reqs.erase(it);
+            sv->reqs.erase(tickets[q]);
         }
     });
     const auto t0 = std::chrono::steady_clock::now();
@@ -2208,10 +2215,10 @@ extern "C" int exc_server_stats(exc_server* sv, exc_server_info* st)
 {
     if (!sv || !st) return set_error(EXC_ERR_ARG, "exc_server_stats: bad argument");
     std::lock_guard<std::mutex> g(sv->mu);
-    st->batches = sv->batches;
-    st->queries = sv->queries_done;
-    st->max_inflight_batches = sv->max_inflight_batches;
-    st->overlapped_launches = sv->overlap_launches;
+    st->batches = sv->batches.load();
+    st->queries = sv->queries_done.load();
+    st->max_inflight_batches = sv->max_inflight_batches.load();
+    st->overlapped_launches = sv->overlap_launches.load();
     return EXC_OK;
 }
 

@@ -406,9 +406,13 @@ def mlp_scale(L):
     return float(x0), float(xs)
 
 
-def mlp_loss_grad(params, L, samples):
+def mlp_loss_grad(params, L, samples, _plant=None):
     """Mean squared position error of the 1-32-1 ReLU MLP over `samples`
-    strided pairs of L and its analytic gradient, in float64."""
+    strided pairs of L and its analytic gradient, in float64.
+
+    _plant (tests only) plants a known slip in the gradient so that a test can
+    show its finite-difference check catches it: "relu_mask" drops the ReLU
+    derivative, "scale" drops the 2/S factor."""
     L = _u32(L)
     n = len(L)
     S = min(samples, n)
@@ -423,10 +427,10 @@ def mlp_loss_grad(params, L, samples):
     out = h @ w2 + b2
     e = out - t
     loss = float(np.mean(e * e))
-    g = 2.0 * e / S
+    g = e if _plant == "scale" else 2.0 * e / S
     gw2 = h.T @ g
     gb2 = g.sum()
-    dh = np.outer(g, w2) * (pre > 0)
+    dh = np.outer(g, w2) * (1.0 if _plant == "relu_mask" else (pre > 0))
     gw1 = dh.T @ x
     gb1 = dh.sum(axis=0)
     return loss, np.concatenate([gw1, gb1, gw2, [gb2]])

@@ -54,8 +54,25 @@ def test_alone_vs_in_mix(ctx, served, nslots, budget):
     assert np.array_equal(res, np.concatenate(alone))
     assert np.all(lat > 0) and st["queries"] == len(qs)
     assert st["batches"] >= (len(qs) + 127) // 128
-    if nslots > 1 and budget == 0:
-        assert st["max_inflight_batches"] >= 1
+
+
+def test_batches_overlap_on_the_device(ctx, served):
+    """Queries that are all waiting when the loop starts fill every free
+    slot in one pass, so later batches launch while earlier ones are still in
+    flight; each query's docIDs are still those it gets alone."""
+    lists, built, dix = served
+    qs = _queries(len(lists), 400, 21)
+    alone = [ctx.intersect(dix, [q])[1] for q in qs]
+    srv = ea.Server(ctx, dix, nslots=3, max_batch=16)
+    try:
+        lat, cnt, res = srv.replay(qs, np.zeros(len(qs)), sum(len(a) for a in alone))
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert np.array_equal(res, np.concatenate(alone))
+    assert st["max_inflight_batches"] >= 2
+    assert st["overlapped_launches"] > 0
+    assert st["batches"] >= len(qs) // 16
 
 
 def test_submit_wait_from_threads(ctx, served):

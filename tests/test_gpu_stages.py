@@ -127,13 +127,14 @@ def test_errors_are_reported_not_fatal(ctx):
 def test_budget_admission():
     c = ea.Context(0, budget=1 << 20)
     try:
-        lists = _corpus(25, 30)
+        lists = _corpus(25, 50)
         docs, off = ea.lists_to_flat(lists)
         built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
-        if built.compressed_bytes() > (1 << 20):
-            with pytest.raises(ea.ExcError) as e:
-                c.upload(built)
-            assert e.value.code == -6
+        # the corpus is sized to exceed the 1 MiB budget
+        assert built.compressed_bytes() > (1 << 20)
+        with pytest.raises(ea.ExcError) as e:
+            c.upload(built)
+        assert e.value.code == -6
     finally:
         c.close()
 

@@ -267,3 +267,135 @@ def test_lrc_prediction_saturates_like_the_gpu():
     wrap = (int(vd[1]) - (1 << 31) + icpt) % (1 << 32)  # the x86 cast's INT32_MIN
     assert sat == top and wrap != top
     assert np.array_equal(o.decode_term(0), [0, top])
+
+
+# ---------------------------------------------------------------- fits off an exact line
+
+def test_seg_fit_is_least_squares():
+    """SegLRC segment and list fits on random lists against numpy's least
+    squares: slope within float rounding of polyfit's, intercept within the
+    float rounding of ya - slope * xa, and the recorded ranges cover every
+    element's position from the fit's own float prediction."""
+    rng = np.random.default_rng(41)
+    fx = fixture_list()
+    cases = [rand_list(rng, n, 1 << 25) for n in (2, 3, 17, 128, 256, 4096, 100000)]
+    cases += [fx[:128], fx[5000:5256], fx]
+    for L in cases:
+        n = len(L)
+        f = oracle.seg_fit(L)
+        i = np.arange(n, dtype=np.float64)
+        slope, icpt = np.polyfit(i, L.astype(np.float64), 1)
+        fs, fi = float(f["fSlope"]), float(f["fIntercept"])
+        assert abs(fs - slope) <= 2e-7 * abs(slope) + 1e-12
+        # intercept = mean(L) - float(slope) * mean(i), rounded to float
+        ref_i = L.astype(np.float64).mean() - fs * i.mean()
+        assert abs(fi - ref_i) <= abs(ref_i) * 2 ** -23 + 1e-6
+        assert abs(fi - icpt) <= abs(slope - fs) * i.mean() + abs(icpt) * 2 ** -23 + 1e-6
+        # coverage: every element lies inside [pred - nRangeLeft, pred + nRangeRight]
+        px = (L.astype(np.float32) - np.float32(fi)) / np.float32(fs)
+        dev = px.astype(np.float64) - i
+        assert (dev <= int(f["nRangeLeft"])).all() and (-dev <= int(f["nRangeRight"])).all()
+        # and the ranges are the tightest such integers (+1 as the reference records them)
+        assert int(f["nRangeLeft"]) == int(max(dev.max(), 0.0)) + 1
+        assert int(f["nRangeRight"]) == int(max(-dev.min(), 0.0)) + 1
+
+
+def test_seg_fit_planted_slope_slip_fails_polyfit():
+    """A plausible slip (the slope's denominator off by one term) would move
+    the slope off polyfit's by far more than the check's tolerance."""
+    rng = np.random.default_rng(42)
+    L = rand_list(rng, 128, 1 << 25).astype(np.float64)
+    i = np.arange(128, dtype=np.float64)
+    xa, ya = i.mean(), L.mean()
+    good = ((i - xa) * (L - ya)).sum() / ((i - xa) ** 2).sum()
+    slip = ((i - xa) * (L - ya)).sum() / ((i - xa) ** 2).sum()[()] / (1 + 1 / 127)
+    slope = np.polyfit(i, L, 1)[0]
+    assert abs(good - slope) <= 2e-7 * abs(slope)
+    assert abs(slip - slope) > 2e-7 * abs(slope)
+
+
+# ---------------------------------------------------------------- learned models
+
+def test_pla_knots_are_the_empirical_cdf():
+    """PLA-256 knots equal np.searchsorted of the knot positions in the
+    fp32-scaled docIDs (elements strictly below each knot), with x0 = first
+    docID and xs = 256 / (span + 1) in fp32."""
+    rng = np.random.default_rng(43)
+    lists = [rand_list(rng, n, 1 << 25) for n in (1, 2, 300, 70000)]
+    lists += [fixture_list(), (np.arange(4096, dtype=np.uint32) * 7 + 11)]
+    for L in lists:
+        x0, xs, y = oracle.pla_fit(L)
+        assert np.float32(x0) == np.float32(L[0])
+        assert np.float32(xs) == np.float32(256.0) / (np.float32(L[-1]) - np.float32(L[0]) + np.float32(1.0))
+        t = (L.astype(np.float32) - np.float32(x0)) * np.float32(xs)
+        assert (np.diff(t) >= 0).all()
+        want = np.searchsorted(t, np.arange(257, dtype=np.float32), side="left")
+        assert np.array_equal(y.astype(np.int64), want)
+        # (the last docID's scaled value can round up to 256 in fp32, so
+        # y[256] counts the elements below it, not always all of them)
+        assert y[0] == 0 and y[256] >= len(L) - 1
+    # an exactly uniform list: knot p holds ceil(p * n / 256) elements
+    L = np.arange(0, 256 * 64, 64, dtype=np.uint32)
+    _, _, y = oracle.pla_fit(L)
+    assert np.array_equal(y[:256], np.arange(256))
+
+
+def test_pla_prediction_interpolates_the_knots():
+    """The PLA model's prediction is the piecewise-linear interpolation of
+    its knots (np.interp), so its recorded max error is the exhaustive max of
+    |interp - position| up to fp32 rounding."""
+    rng = np.random.default_rng(44)
+    L = rand_list(rng, 50000, 1 << 25)
+    x0, xs, y = oracle.pla_fit(L)
+    t = np.clip((L.astype(np.float32) - np.float32(x0)) * np.float32(xs), 0, 256).astype(np.float64)
+    pred = np.interp(t, np.arange(257, dtype=np.float64), y.astype(np.float64))
+    err = np.abs(pred - np.arange(len(L))).max()
+    got = oracle.model_max_err(1, y, x0, xs, L)
+    assert abs(got - err) <= 1e-3 * max(1.0, err)
+
+
+def _fd_max_rel_err(params, L, samples, grad_fn, h=1e-6):
+    """Largest relative difference between grad_fn's gradient and float64
+    central finite differences of the oracle's loss."""
+    _, g = grad_fn(params)
+    fd = np.zeros_like(g)
+    for k in range(len(params)):
+        p1 = params.copy()
+        p1[k] += h
+        p0 = params.copy()
+        p0[k] -= h
+        fd[k] = (oracle.mlp_loss_grad(p1, L, samples)[0] - oracle.mlp_loss_grad(p0, L, samples)[0]) / (2 * h)
+    scale = np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    return float((np.abs(g - fd) / scale).max())
+
+
+def _mlp_params(rng):
+    # weights drawn so that every hidden unit is active on part of [0, 1]
+    w1 = rng.normal(0.0, 2.0, 32)
+    b1 = rng.uniform(-1.0, 1.0, 32)
+    w2 = rng.normal(0.0, 0.5, 32)
+    return np.concatenate([w1, b1, w2, [rng.normal(0.0, 0.1)]]).astype(np.float64)
+
+
+def test_mlp_gradient_matches_finite_differences():
+    """oracle.mlp_loss_grad's analytic gradient (the judge of the GPU
+    training kernel's backward pass) equals float64 central differences of
+    its own loss at random parameters."""
+    rng = np.random.default_rng(45)
+    for n, S in ((5000, 256), (70000, 1024), (300, 300)):
+        L = rand_list(rng, n, 1 << 25)
+        for _ in range(2):
+            params = _mlp_params(rng)
+            err = _fd_max_rel_err(params, L, S, lambda p: oracle.mlp_loss_grad(p, L, S))
+            assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("slip", ["relu_mask", "scale"])
+def test_mlp_gradient_planted_slip_is_caught(slip):
+    """A gradient with the ReLU mask dropped, or without the 2/S factor,
+    fails the same finite-difference check."""
+    rng = np.random.default_rng(46)
+    L = rand_list(rng, 5000, 1 << 25)
+    params = _mlp_params(rng)
+    err = _fd_max_rel_err(params, L, 256, lambda p: oracle.mlp_loss_grad(p, L, 256, _plant=slip))
+    assert err > 1e-2, err
