@@ -30,7 +30,6 @@ HBM_PEAK_GBS = 8000.0
 # k_probe<SegLRC,128> HBM bytes per launch on the C2 batch with the tuned knobs, in KiB: rocprofv3
 # FETCH_SIZE (x1.9999, the copy-kernel scale) + WRITE_SIZE (x1.0), profiles/k_probe_seglrc128_c2.md
 TRAFFIC_KIB = 1302880.0
-C5_SCALE = 5.8e9  # C of df_k = min(2^25, C / k^1.1): about 25.4G postings, 40 GB resident
 META = {0: 8, 1: 28, 2: 32, 3: 8, 4: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD, NewPFD
 CODEC_NAME = {0: "LRC", 1: "SegLRC", 2: "HS_SegLRC", 3: "ParaPFD", 4: "NewPFD"}
 
@@ -399,10 +398,10 @@ def c5_leg(ea, W, ctx, qps=20000.0, seconds=3.0, check=200):
     process at `qps` and go through the continuous-batching server (4 slots,
     16 GiB admission budget).  A sample of queries is checked alone against the
     mix."""
-    U = 1 << 26
-    df = W.zipf_df(1_000_000, universe=U, cap=1 << 25, s=1.1, scale=C5_SCALE)
+    U = W.C5_UNIVERSE
+    df = W.c5_df()
     t0 = time.perf_counter()
-    dix, off = ctx.synth_index_device(5, U, df, seg=128, batch_postings=1 << 30)
+    dix, off = ctx.synth_index_device(W.C5_SEED, U, df, seg=128, batch_postings=1 << 30)
     build_s = time.perf_counter() - t0
     lengths = np.diff(off)
     log("C5 index: %d postings, %.1f GB resident, built on the GPU in %.1f s"
@@ -412,7 +411,7 @@ def c5_leg(ea, W, ctx, qps=20000.0, seconds=3.0, check=200):
     rng = np.random.default_rng(55)
     arrival = np.cumsum(rng.exponential(1.0 / qps, nq))
     tune = ctx.autotune(dix, W.df_weighted_queries_cdf(lengths, 2000, 2, 6, 56), reps=2)
-    out = {"postings": int(off[-1]), "terms": len(df), "universe": U, "C": C5_SCALE,
+    out = {"postings": int(off[-1]), "terms": len(df), "universe": U, "C": W.C5_SCALE,
            "resident_bytes": dix.device_bytes(),
            "build_s_on_gpu": build_s, "offered_qps": qps, "queries": nq, "tune": tune}
     pick = np.random.default_rng(57).choice(nq, check, replace=False)

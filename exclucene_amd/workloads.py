@@ -34,6 +34,18 @@ def zipf_df(nterms=C2_TERMS, universe=C2_UNIVERSE, cap=C2_CAP, s=1.0, scale=None
     return np.minimum(df, cap).astype(np.uint32)
 
 
+C5_UNIVERSE = 1 << 26
+C5_TERMS = 1_000_000
+C5_SCALE = 5.8e9  # C of df_k = min(2^25, C / k^1.1): about 25.4G postings, 40 GB resident
+C5_SEED = 5
+
+
+def c5_df():
+    """Expected list lengths of the C5 index (the device synthesizer draws
+    each list as a Bernoulli(df / universe) sample, so lengths vary around df)."""
+    return zipf_df(C5_TERMS, universe=C5_UNIVERSE, cap=1 << 25, s=1.1, scale=C5_SCALE)
+
+
 def c1_lists(seed=1):
     rng = np.random.default_rng(seed)
     a = np.sort(rng.choice(1 << 24, 1000, replace=False)).astype(np.uint32)

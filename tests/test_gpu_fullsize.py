@@ -98,17 +98,62 @@ def test_c2_full_size_other_codecs(ctx, c2, codec, seg, frac):
 @pytest.mark.parametrize("shift", [13, 0])
 def test_c3_full_size_hash_buckets(ctx, c2, shift):
     """C3: the full-size lists in hash buckets of shift 13 (the top 12 bits of
-    2^25) or of the adaptive width (about 256 docIDs per bucket, per list), the
-    first 2,000 of the 10,000 ratio queries (long:short 10^U(1,4)): equal to
-    the oracle over the same buckets, in both launch modes."""
+    2^25) or of the adaptive width (about 256 docIDs per bucket, per list), all
+    10,000 ratio queries (long:short 10^U(1,4), seed 3): equal to the oracle
+    over the same buckets in both launch modes, and as the configuration runs
+    them -- one persistent work-queue launch replayed from a hipGraph, knobs
+    autotuned per ratio class -- on two replays."""
     docs, off, _, _, _, _, df = c2
-    queries = W.ratio_queries(df)[:2000]
+    queries = W.ratio_queries(df, 10_000, 3)
     built = ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=shift)
     ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
     ocum = np.asarray(ocum, np.int64)
-    for mode, (cum, res) in enumerate(_gpu_both_modes(ctx, built, queries)):
-        assert np.array_equal(cum, ocum), mode
-        assert np.array_equal(res, ores), mode
+    dix = ctx.upload(built)
+    try:
+        for persistent in (0, 1):
+            ctx.set_option(ea.OPT_PERSISTENT, persistent)
+            cum, res = ctx.intersect(dix, queries)
+            assert np.array_equal(np.asarray(cum, np.int64), ocum), persistent
+            assert np.array_equal(res, ores), persistent
+        ctx.autotune(dix, W.ratio_queries(df, 2000, 33), reps=1)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        ctx.set_option(ea.OPT_GRAPH, 1)
+        ctx.plan(dix, queries)
+        bi = ctx._bi
+        for rep in range(2):
+            ctx.kernelInvoke(bi)
+            cum, res = ctx.dtohTransfer(bi)
+            assert np.array_equal(np.asarray(cum, np.int64), ocum), rep
+            assert np.array_equal(res, ores), rep
+    finally:
+        ctx.set_option(ea.OPT_GRAPH, 0)
+        ctx.set_tuning(None)
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        dix.close()
+
+
+def test_c4_full_query_set(ctx, c2):
+    """C4's own query set: 10,000 queries of 2-4 terms (seed 44) over the
+    full-size SegLRC index, with PLA-256 and then MLP 1-32-1 models on every
+    list longer than 65,536; binary, list-fit and learned window search each
+    equal to the oracle (zero mismatches)."""
+    docs, off, _, built, _, _, df = c2
+    queries = W.df_weighted_queries(df, 10_000, 2, 4, 44)
+    ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
+    ocum = np.asarray(ocum, np.int64)
+    dix = ctx.upload(built)
+    try:
+        for kind in (ea.MODEL_PLA, ea.MODEL_MLP):
+            ctx.train_models(dix, kind, min_len=65536, epochs=200, samples=4096, lr=1e-3)
+            for search in (ea.SEARCH_BINARY, 1, ea.SEARCH_LEARNED):
+                ctx.set_option(ea.OPT_SEARCH, search)
+                cum, res = ctx.intersect(dix, queries)
+                assert np.array_equal(np.asarray(cum, np.int64), ocum), (kind, search)
+                assert np.array_equal(res, ores), (kind, search)
+    finally:
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        dix.close()
 
 
 @pytest.mark.parametrize("codec,seg,frac", [(ea.SEGLRC, 128, 0.0), (ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1),
