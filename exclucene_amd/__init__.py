@@ -35,6 +35,7 @@ EXPORTS = [
     "exc_server_destroy", "exc_autotune", "exc_ctx_set_tuning", "exc_decompress_device",
     "exc_device_alloc", "exc_device_free", "exc_memcpy",
     "exc_synth_lists_device", "exc_build_index_device", "exc_synth_index_device", "exc_index_export",
+    "exc_intersect_noalloc",
 ]
 MODEL_PLA, MODEL_MLP = 1, 2
 SERVER_COUNTS = 1
@@ -138,6 +139,7 @@ def lib():
         L.exc_dtohTransfer.argtypes = [_vp, ctypes.POINTER(BatchInfo), _vp, _vp, _u64]
         L.exc_verify.argtypes = [ctypes.POINTER(BatchInfo), _vp, _vp, ctypes.POINTER(CheckSum)]
         L.exc_intersect.argtypes = [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+        L.exc_intersect_noalloc.argtypes = [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _u64, ctypes.POINTER(_u64)]
         L.exc_plan.argtypes = [_vp, _vp, _vp, _vp, _u32, ctypes.POINTER(BatchInfo)]
         L.exc_decompress.argtypes = [_vp, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]
         L.exc_build_index.argtypes = [ctypes.c_int, _u32, ctypes.c_float, _u32, _u32, _vp, _vp, _u32,
@@ -395,6 +397,14 @@ class Context:
 
     def intersect(self, index, queries):
         """Exact results of every query: (cum_counts, results)."""
+        return self._intersect_with(lib().exc_intersect, index, queries)
+
+    def intersect_noalloc(self, index, queries):
+        """As intersect, through exc_intersect_noalloc: no allocation, the
+        batch planned on the device inside the library's fixed scratch area."""
+        return self._intersect_with(lib().exc_intersect_noalloc, index, queries)
+
+    def _intersect_with(self, fn, index, queries):
         qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
         nq = len(qoff) - 1
         ucnt = index.ucnt
@@ -410,8 +420,7 @@ class Context:
         cum = np.zeros(max(nq, 1), np.uint32)
         res = np.zeros(max(cap, 1), np.uint32)
         n = _u64()
-        _check(lib().exc_intersect(self._h, index.handle, _p(qterms), _p(qoff), nq, _p(cum), _p(res), cap,
-                                   ctypes.byref(n)))
+        _check(fn(self._h, index.handle, _p(qterms), _p(qoff), nq, _p(cum), _p(res), cap, ctypes.byref(n)))
         return cum[:nq], res[:int(n.value)]
 
     # -- autotuner

@@ -161,6 +161,25 @@ constexpr uint32_t kScanTicketWord = 32;     // word of the context's counter bu
 constexpr uint32_t kChunk = 512;            // shortest-list docIDs per work unit
 constexpr uint32_t kDecodeUnit = 1024;      // docIDs per wave of whole-list decompression (HS: one chunk)
 
+// Batch planning on the device, for the entry point that allocates nothing
+// (exc_intersect_noalloc): the query list is copied into a fixed device
+// scratch area and these kernels write the records plan_queries writes on the
+// host -- terms sorted by ascending length (stable), their TermRec copies,
+// one QueryRec per query, the work units in query order.
+constexpr uint32_t kNaMaxTerms = 64;  // terms per query the device planner sorts
+struct NaPlanArgs {
+    const TermRec* terms;    // the index's term records
+    const uint32_t* qterms;  // the batch's terms as given
+    const uint64_t* qoff;    // nq + 1 offsets into qterms (qoff[0] need not be 0)
+    uint32_t nq;
+    uint32_t knobs[kRatioClasses];  // search knobs per ratio class
+    QueryRec* queries;
+    TermRec* qrecs;          // one per batch term, sorted per query
+    UnitRec* units;          // query order
+};
+hipError_t launch_noalloc_plan(const NaPlanArgs& a, hipStream_t s);
+hipError_t launch_zero_words(uint32_t* p, uint32_t n, hipStream_t s);
+
 // launchers (kernels.hip); all return hipError_t of the launch
 hipError_t launch_reset(uint32_t* counter, hipStream_t s);
 hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s);

@@ -1268,6 +1268,120 @@ __global__ __launch_bounds__(256) void k_unit_prologue(ProbeArgs A)
 
 __global__ void k_reset(uint32_t* c) { if (threadIdx.x == 0) *c = 0; }
 
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* p, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
+}
+
+// ---- batch planning on the device (the allocation-free entry point)
+
+// Thread per query: the query's terms sorted by ascending list length with a
+// stable insertion sort (the rule of LRC_Intersection.cu:157-179, as
+// plan_queries applies it on the host), their term records copied in that
+// order, and its QueryRec with the unit count, ratio class and knobs.  The
+// shortest-list length is parked in short_off until k_na_scan turns the
+// counts into offsets.
+__global__ __launch_bounds__(256) void k_na_queries(NaPlanArgs a)
+{
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= a.nq) return;
+    const uint64_t o = a.qoff[q] - a.qoff[0];
+    const uint32_t tc = (uint32_t)(a.qoff[q + 1] - a.qoff[q]);  // <= kNaMaxTerms, checked on the host
+    uint32_t t[kNaMaxTerms], n[kNaMaxTerms];
+    for (uint32_t i = 0; i < tc; ++i) {
+        const uint32_t x = a.qterms[o + i], nx = a.terms[x].n;
+        uint32_t k = i;
+        while (k && nx < n[k - 1]) {
+            t[k] = t[k - 1];
+            n[k] = n[k - 1];
+            --k;
+        }
+        t[k] = x;
+        n[k] = nx;
+    }
+    for (uint32_t i = 0; i < tc; ++i) a.qrecs[o + i] = a.terms[t[i]];
+    uint32_t klass = kRatioClasses - 1;
+    if (tc >= 2) {
+        // ratio_class of the host planner, in the same double arithmetic
+        double r = (double)n[1] / (double)max(n[0], 1u);
+        klass = 0;
+        while (klass < 4 && r >= 10.0) {
+            r /= 10.0;
+            ++klass;
+        }
+    }
+    const uint32_t n0 = tc ? n[0] : 0u;
+    QueryRec Q;
+    Q.toff = (uint32_t)o;
+    Q.tc = tc;
+    Q.first_unit = 0;
+    Q.nunits = (n0 + kChunk - 1) / kChunk;
+    Q.short_off = n0;
+    Q.knobs = a.knobs[klass];
+    Q.klass = klass;
+    a.queries[q] = Q;
+}
+
+// One workgroup: exclusive sums of the queries' unit counts and shortest-list
+// lengths into first_unit and short_off (each thread a contiguous range of
+// queries, a block scan over the threads' totals in LDS).
+__global__ __launch_bounds__(1024) void k_na_scan(NaPlanArgs a)
+{
+    __shared__ uint32_t su[1024];
+    __shared__ uint64_t ss[1024];
+    const uint32_t t = threadIdx.x, per = (a.nq + 1023) / 1024;
+    const uint32_t q0 = min(a.nq, t * per), q1 = min(a.nq, q0 + per);
+    uint32_t u = 0;
+    uint64_t s = 0;
+    for (uint32_t q = q0; q < q1; ++q) {
+        u += a.queries[q].nunits;
+        s += a.queries[q].short_off;
+    }
+    su[t] = u;
+    ss[t] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t xu = t >= d ? su[t - d] : 0u;
+        const uint64_t xs = t >= d ? ss[t - d] : 0ull;
+        __syncthreads();
+        su[t] += xu;
+        ss[t] += xs;
+        __syncthreads();
+    }
+    u = su[t] - u;
+    s = ss[t] - s;
+    for (uint32_t q = q0; q < q1; ++q) {
+        QueryRec& Q = a.queries[q];
+        const uint32_t nu = Q.nunits;
+        const uint64_t n0 = Q.short_off;
+        Q.first_unit = u;
+        Q.short_off = s;
+        u += nu;
+        s += n0;
+    }
+}
+
+// One wave per query: its work units, in query order (also the launch order).
+__global__ __launch_bounds__(256) void k_na_units(NaPlanArgs a)
+{
+    const uint32_t q = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (q >= a.nq) return;
+    const QueryRec Q = a.queries[q];
+    for (uint32_t k = lane_id(); k < Q.nunits; k += 64) {
+        UnitRec U;
+        U.q = q;
+        U.k = k;
+        U.toff = Q.toff;
+        U.tc = (uint16_t)Q.tc;
+        U.knobs = (uint16_t)Q.knobs;
+        U.l1_first = 0;
+        U.l1_count = kUnset;
+        U.slot = Q.first_unit + k;
+        U.soff = (uint32_t)(Q.short_off + (uint64_t)k * kChunk);
+        a.units[Q.first_unit + k] = U;
+    }
+}
+
 // ---- scan over unit counts (replaces cudppScan + ucntResult)
 
 // Exclusive scan in place of v[0, n) by the whole block of NT threads;
@@ -1521,6 +1635,23 @@ int probe_blocks_per_cu(int codec, uint32_t seg)
 hipError_t launch_reset(uint32_t* counter, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(uint32_t* p, uint32_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    const uint32_t grid = min((n + 255) / 256, 1024u);
+    hipLaunchKernelGGL(k_zero_words, dim3(grid), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_noalloc_plan(const NaPlanArgs& a, hipStream_t s)
+{
+    if (!a.nq) return hipSuccess;
+    hipLaunchKernelGGL(k_na_queries, dim3((a.nq + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_na_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_na_units, dim3((a.nq + kWaves - 1) / kWaves), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -278,6 +278,10 @@ uint32_t npfd_block_words(const uint32_t* blk, uint32_t len, uint64_t avail, std
 
 // ---------------------------------------------------------------- context
 
+namespace {
+void noalloc_setup(int device);  // exc_intersect_noalloc's scratch address, once per device
+}  // namespace
+
 extern "C" int exc_ctx_create(int device, uint64_t budget, exc_ctx** out)
 {
     if (!out) return set_error(EXC_ERR_ARG, "exc_ctx_create: null out");
@@ -297,6 +301,7 @@ extern "C" int exc_ctx_create(int device, uint64_t budget, exc_ctx** out)
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
     for (auto& e : c->pev) (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    noalloc_setup(device);  // the allocation-free entry point's scratch address on this device
     *out = c;
     return EXC_OK;
 }
@@ -1098,6 +1103,178 @@ extern "C" int exc_intersect(exc_ctx* c, exc_index* ix, const uint32_t* qterms, 
     if (int rc = exc_plan(c, ix, qterms, qoff, nq, &bi)) return rc;
     if (int rc = invoke(c)) return rc;
     return fetch(c, cum, res, cap, nres);
+}
+
+// ---------------------------------------------------------------- allocation-free intersect
+
+// Device scratch of exc_intersect_noalloc: a fixed array in the library's code
+// object, one copy per GPU, held until the process ends.  Host code never
+// names it in a kernel argument or a copy (the name is a host-side stand-in);
+// it uses the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNaScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kNaScratchBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};        // scratch's device address on each GPU
+hipError_t g_na_err[kNaMaxDevices] = {};   // the lookup's result, once made
+
+// Once per device, with that device current (exc_ctx_create's set-up).
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= kNaMaxDevices) return;
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (g_na_ptr[device] || g_na_err[device] != hipSuccess) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_na_ptr[device] = p;
+    else g_na_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+}
+
+size_t na_align(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+// exc_intersect without any allocation: the plan is built on the device from
+// the query list copied into the scratch array, and the probe, scan and
+// scatter use the scratch for their records, stage and results.  Batches whose
+// records would not fit are refused with EXC_ERR_CAPACITY.
+extern "C" int exc_intersect_noalloc(exc_ctx* c, exc_index* ix, const uint32_t* qterms, const uint64_t* qoff,
+                                     uint32_t nq, uint32_t* cum, uint32_t* res, uint64_t cap, uint64_t* nres)
+{
+    if (!c || !ix || !qoff || (nq && !cum)) return set_error(EXC_ERR_ARG, "exc_intersect_noalloc: bad argument");
+    if (ix->codec == EXC_NEWPFD)
+        return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only (as GC/NewPFD in the reference)");
+    if (c->device < 0 || c->device >= kNaMaxDevices)
+        return set_error(EXC_ERR_ARG, "exc_intersect_noalloc: device %d beyond %d", c->device, kNaMaxDevices);
+    HIP_TRY(hipSetDevice(c->device));
+    noalloc_setup(c->device);  // made by exc_ctx_create already; a no-op then
+    uint8_t* base = static_cast<uint8_t*>(g_na_ptr[c->device]);
+    if (!base)
+        return set_error(EXC_ERR_DEVICE, "scratch address of device %d: %s", c->device,
+                         hipGetErrorString(g_na_err[c->device]));
+    // sizes, checked against the scratch before anything is sent (no storage:
+    // the shortest lengths are summed here and recomputed on the device)
+    const uint64_t nterm = nq ? qoff[nq] - qoff[0] : 0;
+    if (nterm && !qterms) return set_error(EXC_ERR_ARG, "exc_intersect_noalloc: null terms");
+    uint64_t units = 0, short_sum = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (qoff[q + 1] < qoff[q]) return set_error(EXC_ERR_ARG, "query %u: bad offsets", q);
+        const uint64_t tc = qoff[q + 1] - qoff[q];
+        if (tc > kNaMaxTerms)
+            return set_error(EXC_ERR_CAPACITY, "query %u: %llu terms, the allocation-free path plans at most %u", q,
+                             (unsigned long long)tc, kNaMaxTerms);
+        uint32_t n0 = 0;
+        for (uint64_t i = 0; i < tc; ++i) {
+            const uint32_t t = qterms[qoff[q] + i];
+            if (t >= ix->nterms) return set_error(EXC_ERR_ARG, "query %u: term %u out of range", q, t);
+            n0 = (i == 0 || ix->n[t] < n0) ? ix->n[t] : n0;
+        }
+        units += (n0 + kChunk - 1) / kChunk;
+        short_sum += n0;
+    }
+    const uint32_t nblk = (uint32_t)((units + kScanBlockItems - 1) / kScanBlockItems);
+    // layout: counters | terms | offsets | queries | term records | units |
+    // unit counts | unit offsets | block sums | stage | results | cumulative counts
+    size_t o = 0;
+    const size_t o_cnt = o;
+    o += na_align(256);
+    const size_t o_qt = o;
+    o += na_align(nterm * 4);
+    const size_t o_qo = o;
+    o += na_align((nq + 1ull) * 8);
+    const size_t o_q = o;
+    o += na_align((uint64_t)nq * sizeof(QueryRec));
+    const size_t o_tr = o;
+    o += na_align(nterm * sizeof(TermRec));
+    const size_t o_u = o;
+    o += na_align(units * sizeof(UnitRec));
+    const size_t o_uc = o;
+    o += na_align(units * 4);
+    const size_t o_uo = o;
+    o += na_align(units * 4);
+    const size_t o_bs = o;
+    o += na_align((nblk + 1ull) * 4);
+    const size_t o_st = o;
+    o += na_align(short_sum * 4);
+    const size_t o_res = o;
+    o += na_align(short_sum * 4);
+    const size_t o_cum = o;
+    o += na_align((uint64_t)nq * 4);
+    if (o > kNaScratchBytes || units > 0xffffffffull)
+        return set_error(EXC_ERR_CAPACITY,
+                         "batch needs %llu bytes of scratch (%llu shortest-list docIDs, %llu units), the "
+                         "allocation-free path has %llu",
+                         (unsigned long long)o, (unsigned long long)short_sum, (unsigned long long)units,
+                         (unsigned long long)kNaScratchBytes);
+    if (nres) *nres = 0;
+    if (!nq) return EXC_OK;
+    hipStream_t s = c->stream;
+    // counters (the persistent queue's claim word and the scan's ticket)
+    // start at zero: a fill kernel, not a memset node
+    HIP_TRY(launch_zero_words(reinterpret_cast<uint32_t*>(base + o_cnt), 64, s));
+    if (nterm) HIP_TRY(hipMemcpyAsync(base + o_qt, qterms + qoff[0], nterm * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(base + o_qo, qoff, (nq + 1ull) * 8, hipMemcpyHostToDevice, s));
+    NaPlanArgs pa;
+    std::memset(&pa, 0, sizeof pa);
+    pa.terms = ix->dix.terms;
+    pa.qterms = reinterpret_cast<const uint32_t*>(base + o_qt);
+    pa.qoff = reinterpret_cast<const uint64_t*>(base + o_qo);
+    pa.nq = nq;
+    for (int k = 0; k < kRatioClasses; ++k) pa.knobs[k] = knobs_for(c, ix->codec, (uint32_t)k);
+    pa.queries = reinterpret_cast<QueryRec*>(base + o_q);
+    pa.qrecs = reinterpret_cast<TermRec*>(base + o_tr);
+    pa.units = reinterpret_cast<UnitRec*>(base + o_u);
+    HIP_TRY(launch_noalloc_plan(pa, s));
+    ProbeArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.ix = ix->dix;
+    a.queries = pa.queries;
+    a.qrecs = pa.qrecs;
+    a.units = pa.units;  // query order is the launch order here
+    a.nunits = (uint32_t)units;
+    a.persistent = c->persistent;
+    a.stage = reinterpret_cast<uint32_t*>(base + o_st);
+    a.ucount = reinterpret_cast<uint32_t*>(base + o_uc);
+    a.counter = reinterpret_cast<uint32_t*>(base + o_cnt);
+    const uint32_t wpb = probe_waves_per_block();
+    uint32_t grid = std::max<uint32_t>(1, (a.nunits + wpb - 1) / wpb);
+    if (c->persistent)
+        grid = std::max<uint32_t>(
+            1, std::min<uint32_t>((uint32_t)c->cus * (uint32_t)probe_blocks_per_cu(ix->codec, ix->seg), grid));
+    if (a.nunits) {
+        HIP_TRY(launch_unit_prologue(a, s));
+        HIP_TRY(launch_probe(a, grid, s));
+    }
+    ScanArgs sa;
+    std::memset(&sa, 0, sizeof sa);
+    sa.queries = a.queries;
+    sa.units = a.units;
+    sa.ucount = a.ucount;
+    sa.stage = a.stage;
+    sa.nunits = a.nunits;
+    sa.nq = nq;
+    sa.uoff = reinterpret_cast<uint32_t*>(base + o_uo);
+    sa.bsum = reinterpret_cast<uint32_t*>(base + o_bs);
+    sa.result = reinterpret_cast<uint32_t*>(base + o_res);
+    sa.qcum = reinterpret_cast<uint32_t*>(base + o_cum);
+    sa.ticket = a.counter + kScanTicketWord;
+    HIP_TRY(launch_scan(sa, s));
+    HIP_TRY(hipMemcpyAsync(cum, sa.qcum, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const uint64_t total = cum[nq - 1];
+    if (nres) *nres = total;
+    if (total > cap)
+        return set_error(EXC_ERR_CAPACITY, "result buffer holds %llu, need %llu", (unsigned long long)cap,
+                         (unsigned long long)total);
+    if (total) {
+        if (!res) return set_error(EXC_ERR_ARG, "exc_intersect_noalloc: null result buffer");
+        HIP_TRY(hipMemcpyAsync(res, sa.result, total * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return EXC_OK;
 }
 
 // ---------------------------------------------------------------- decompress

@@ -162,6 +162,19 @@ int exc_verify(const exc_batchInfo *bi, const uint32_t *ucntResult, const uint32
 int exc_intersect(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
                   uint32_t nq, uint32_t *ucntResult, uint32_t *result, uint64_t result_cap,
                   uint64_t *n_results);
+/* exc_intersect for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and results as exc_intersect;
+ * nothing is allocated on the device or the host: the batch is planned on the
+ * device from the query list, inside a fixed 64 MiB scratch area that the
+ * library holds on every GPU it runs on until the process ends (about 8 B per
+ * shortest-list docID + 40 B per 512 of them + 110 B per query term; at most
+ * 64 terms per query).  A larger batch returns EXC_ERR_CAPACITY before any
+ * work is queued.  Calls that overlap on one GPU (several host threads or
+ * contexts on the same device) race on the scratch area: serialise them. */
+int exc_intersect_noalloc(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
+                          uint32_t nq, uint32_t *ucntResult, uint32_t *result, uint64_t result_cap,
+                          uint64_t *n_results);
 /* Plan + upload a query list without running it (replay with exc_kernelInvoke). */
 int exc_plan(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint64_t *qoff,
              uint32_t nq, exc_batchInfo *bi);
