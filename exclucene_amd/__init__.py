@@ -17,9 +17,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libexclucene_amd.so")
 
-LRC, SEGLRC, HS_SEGLRC, PARAPFD, NEWPFD, RAW = 0, 1, 2, 3, 4, 5  # NewPFD: decompression only
+LRC, SEGLRC, HS_SEGLRC, PARAPFD, NEWPFD, RAW, RAW_HS = 0, 1, 2, 3, 4, 5, 6  # NewPFD: decompression only
 CODECS = {"LRC": LRC, "SegLRC": SEGLRC, "HS_SegLRC": HS_SEGLRC, "ParaPFD": PARAPFD, "NewPFD": NEWPFD,
-          "uncompressed": RAW}
+          "uncompressed": RAW, "uncompressed_hash_segment": RAW_HS}
 
 OPT_SEARCH, OPT_TPE, OPT_PERSISTENT, OPT_GRAPH, OPT_TIMING, OPT_COOP_MAX = 0, 1, 2, 3, 4, 5
 SEARCH_BINARY, SEARCH_FIT, SEARCH_LEARNED = 0, 1, 2

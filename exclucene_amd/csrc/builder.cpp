@@ -20,6 +20,7 @@
 // the result is identical for any thread count.
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -231,6 +232,47 @@ void hs_table(const uint32_t* L, uint32_t n, uint32_t target, uint32_t fixed_shi
     t[2 + nb] = n;
 }
 
+// ceil(log2 x) as the uncompressed hash-segment generator computes it, in
+// double logarithms (cuda_hashSegment.cu:872-881; for x = 2^29 and 2^31 the
+// quotient rounds above the integer and the result is one more).  x = 0, whose
+// logarithm the reference takes anyway, gives 0 here.
+uint32_t gu_ceil_log2(double x)
+{
+    if (!(x > 0.0)) return 0;
+    const double d = std::log(x) / std::log(2.0);
+    if (d <= 0.0) return 0;
+    const uint32_t i = (uint32_t)d;
+    return (d - (double)i > 0.0) ? i + 1 : i;
+}
+
+// The uncompressed hash-segment table of one list, as RunHashSegInfoGenerator
+// writes it (cuda_hashSegment.cu:815-947): lists of at most `target` docIDs
+// get [bits, 0, n] (one bucket, bits = ceil(log2 max)); longer lists get
+// [shift][start_0 .. start_nb] with shift = bits - ceil(log2(n / target)),
+// nb = (max >> shift) + 1 and start_b the first index whose docID >> shift
+// is >= b.  A fixed shift uses the second form for every list.  An empty list
+// (whose last docID the reference reads anyway) gets [shift, 0, 0].
+void gu_hash_table(const uint32_t* L, uint32_t n, uint32_t target, uint32_t fixed_shift, std::vector<uint32_t>& t)
+{
+    if (n == 0) { t = {fixed_shift, 0u, 0u}; return; }
+    const uint32_t mx = L[n - 1];
+    uint32_t shift = fixed_shift;
+    if (!fixed_shift) {
+        const uint32_t bits = gu_ceil_log2((double)mx);
+        if (n <= target) { t = {bits, 0u, n}; return; }
+        shift = bits - gu_ceil_log2((double)n / target);
+    }
+    const uint32_t nb = (mx >> shift) + 1;
+    t.assign(nb + 2, 0);
+    t[0] = shift;
+    uint32_t cur = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t b = L[i] >> shift;
+        while (cur < b) t[1 + (++cur)] = i;
+    }
+    t[1 + nb] = n;
+}
+
 // Encoded form of one list before it is placed in the index.
 struct ListOut {
     std::vector<uint32_t> words, seg_offset, seg_head, seg_median, hash;
@@ -244,8 +286,9 @@ struct ListOut {
 void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
                  const uint32_t* L, uint32_t n, ListOut& o)
 {
-    if (codec == EXC_RAW) {  // the docIDs as they are (the GU programs' .ind2)
+    if (codec == EXC_RAW || codec == EXC_RAW_HS) {  // the docIDs as they are (the GU programs' .ind2)
         o.words.assign(L, L + n);
+        if (codec == EXC_RAW_HS) gu_hash_table(L, n, hs_target, hs_shift, o.hash);
         return;
     }
     std::vector<uint32_t> vd(n ? n : 1);
@@ -364,8 +407,9 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
                                int nthreads, exc_built** out)
 {
     if (!out || !list_off || (nterms && !docs)) return set_error(EXC_ERR_ARG, "exc_build_index: null argument");
-    if (codec < EXC_LRC || codec > EXC_RAW) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
-    if (codec == EXC_RAW) seg = 128;  // not stored: the unit planner's block size
+    if (codec < EXC_LRC || codec > EXC_RAW_HS) return set_error(EXC_ERR_ARG, "exc_build_index: unknown codec");
+    if (codec == EXC_RAW || codec == EXC_RAW_HS) seg = 128;  // not stored: the unit planner's block size
+    if (hs_shift > 31) return set_error(EXC_ERR_ARG, "exc_build_index: hash shift %u above 31", hs_shift);
     if (codec != EXC_HS_SEGLRC && (seg < 64 || seg > 1024 || (seg & 63)))
         return set_error(EXC_ERR_ARG, "exc_build_index: segment size must be a multiple of 64 in [64,1024]");
     if (hs_target == 0) hs_target = 256;
@@ -411,7 +455,8 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
     if (codec == EXC_SEGLRC || codec == EXC_HS_SEGLRC) { b->seg_info.resize(S); b->base.resize(S); }
     if (codec == EXC_SEGLRC) b->list_info.resize(nterms);
     if (codec == EXC_LRC) b->lrc.resize(nterms);
-    if (codec == EXC_HS_SEGLRC) { b->hash_offset.resize(nterms); b->hash_info.resize(hoff[nterms]); }
+    const bool hashed = codec == EXC_HS_SEGLRC || codec == EXC_RAW_HS;
+    if (hashed) { b->hash_offset.resize(nterms); b->hash_info.resize(hoff[nterms]); }
     parallel_for(nterms, nthreads, [&](uint32_t t, int) {
         ListOut& o = lo[t];
         b->ind1[t].m_urlcount = (uint32_t)(list_off[t + 1] - list_off[t]);
@@ -425,7 +470,7 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
         if (!o.base.empty()) std::copy(o.base.begin(), o.base.end(), b->base.begin() + soff[t]);
         if (codec == EXC_SEGLRC) b->list_info[t] = o.list;
         if (codec == EXC_LRC) b->lrc[t] = o.lrc;
-        if (codec == EXC_HS_SEGLRC) {
+        if (hashed) {
             b->hash_offset[t] = (uint32_t)hoff[t];
             std::copy(o.hash.begin(), o.hash.end(), b->hash_info.begin() + hoff[t]);
         }
@@ -492,8 +537,11 @@ extern "C" int exc_index_save_files(const exc_index_host* h, const char* dir, co
     if (h->seg_info && (rc = write_file(p + ".regression_info", h->seg_info, h->nseg * sizeof(exc_seg_info)))) return rc;
     if (h->list_info && (rc = write_file(p + ".regression_info_list", h->list_info, (size_t)nt * sizeof(exc_seg_info)))) return rc;
     if (h->seg_base && (rc = write_file(p + ".ind2_distance_base", h->seg_base, h->nseg * 4))) return rc;
-    if (h->hash_offset && (rc = write_file(p + ".HashSeg_offset", h->hash_offset, (size_t)nt * 4))) return rc;
-    if (h->hash_info && (rc = write_file(p + ".HashSeg_info", h->hash_info, h->hash_words * 4))) return rc;
+    // the compressed HS programs name the tables <name>.HashSeg_*, the
+    // uncompressed one <name>_HashSeg_* (cuda_hashSegment.cu:158,175, "at_")
+    const std::string hp = p + (h->codec == EXC_RAW_HS ? "_HashSeg" : ".HashSeg");
+    if (h->hash_offset && (rc = write_file(hp + "_offset", h->hash_offset, (size_t)nt * 4))) return rc;
+    if (h->hash_info && (rc = write_file(hp + "_info", h->hash_info, h->hash_words * 4))) return rc;
     return EXC_OK;
 }
 

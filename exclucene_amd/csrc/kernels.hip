@@ -969,6 +969,52 @@ __device__ uint32_t probe_raw(const ProbeArgs& A, const TermRec& R, const LdsWav
     return uni(cnt);
 }
 
+// Uncompressed list R in hash buckets on its top docID bits
+// (GU/cuda_hashSegment_new/cuda_hashSegment_kernel.cu:269-307): a candidate v
+// searches only bucket b = v >> shift, [start_b, start_(b+1)) of R.  The
+// reference takes b unchecked, so a docID past the list's last bucket reads
+// beyond its table; here b is clamped to the last bucket, which is exact
+// (the validated table brackets its docIDs: every element of a bucket below
+// b is below b << shift, so a docID past the last bucket can only be in it).
+// Binary search (EXC_OPT_SEARCH 0, the reference's) or interpolation search
+// inside the bucket; reads stay inside the bucket.
+__device__ uint32_t probe_raw_hs(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
+{
+    constexpr int V = (int)(kChunk / 64);
+    const uint32_t ln = lane_id();
+    const uint32_t* L = A.ix.words + R.woff;
+    const uint32_t* starts = A.ix.hash + R.aux;
+    const uint32_t shift = (uint32_t)R.icpt, lastb = R.nseg - 1;
+    const bool interp = knob_search(knobs) != SEARCH_BINARY;
+    uint32_t fmask = 0;
+    for (int i = 0; i < V; ++i) {
+        const uint32_t e = ln + 64u * (uint32_t)i;
+        if (!__ballot(e < nlive)) break;  // rows fill in order
+        if (e < nlive) {
+            const uint32_t v = W.cand[e];
+            const uint32_t b = min(v >> shift, lastb);
+            const uint32_t lo = starts[b], hi = starts[b + 1];
+            if (lo < hi) {
+                const uint32_t t = interp ? L[lo] : 0u, r = interp ? L[hi - 1] : 0xffffffffu;
+                if (raw_find(L, lo, hi, t, r, v, interp)) fmask |= 1u << i;
+            }
+        }
+    }
+    uint32_t cnt = 0;
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        const bool fi = (fmask >> i) & 1u;
+        const uint64_t m = __ballot(fi);
+        const uint32_t e = ln + 64 * i;
+        const uint32_t vi = e < nlive ? W.cand[e] : kNone;
+        if (fi) W.cand[cnt + below_me(m)] = vi;
+        cnt += popc(m);
+    }
+    wave_sync();
+    return uni(cnt);
+}
+
 // HS_SegLRC: decode elements [k*CH, k*CH+CH) of list R into dst; returns count.
 // Bucket of each element = last bucket start <= its index
 // (HS_SegLRC_Intersection_kernel.cu:274-327).
@@ -1074,10 +1120,13 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
     return uni(cnt);
 }
 
+// uncompressed lists: .ind2 holds the docIDs themselves
+constexpr bool raw_codec(int codec) { return codec == EXC_RAW || codec == EXC_RAW_HS; }
+
 template <int CODEC>
 constexpr int wave_lds_words()
 {
-    return CODEC == EXC_RAW ? (int)kChunk : (int)(kChunk + kBufWords + (CODEC == EXC_PARAPFD ? kChunk : 0));
+    return raw_codec(CODEC) ? (int)kChunk : (int)(kChunk + kBufWords + (CODEC == EXC_PARAPFD ? kChunk : 0));
 }
 
 template <int CODEC, int SEG>
@@ -1094,7 +1143,7 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     uint32_t nlive;
     if constexpr (CODEC == EXC_HS_SEGLRC) {
         nlive = decode_hs_chunk<kChunk>(A.ix, R0, k, W.cand);
-    } else if constexpr (CODEC == EXC_RAW) {
+    } else if constexpr (raw_codec(CODEC)) {
         const uint32_t b0 = k * kChunk;
         nlive = min(kChunk, R0.n - b0);
         const uint32_t* src = A.ix.words + R0.woff + b0;
@@ -1114,6 +1163,8 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
             nlive = probe_hs(A, R, W, nlive, knobs);
         else if constexpr (CODEC == EXC_RAW)
             nlive = probe_raw(A, R, W, nlive, knobs);
+        else if constexpr (CODEC == EXC_RAW_HS)
+            nlive = probe_raw_hs(A, R, W, nlive, knobs);
         else
             nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs, j == 1 ? l1_first : 0u, j == 1 ? l1_count : kUnset);
     }
@@ -1546,7 +1597,7 @@ __global__ __launch_bounds__(256) void k_decompress(DevIndex ix, const DecUnit* 
     const DecUnit D = units[u];
     const uint32_t k = uni(D.k);
     uint32_t* o = out + D.out;
-    if constexpr (CODEC == EXC_RAW) {
+    if constexpr (raw_codec(CODEC)) {
         const uint32_t b0 = k * SEG, n = min((uint32_t)(U * SEG), D.n - b0);
         const uint32_t* src = ix.words + D.woff + b0;
         for (uint32_t e = ln; e < n; e += 64) o[e] = src[e];
@@ -1628,6 +1679,7 @@ int probe_blocks_per_cu(int codec, uint32_t seg)
     case EXC_HS_SEGLRC: return occ_for<EXC_HS_SEGLRC>(seg);
     case EXC_PARAPFD: return occ_for<EXC_PARAPFD>(seg);
     case EXC_RAW: return occ_for<EXC_RAW>(seg);
+    case EXC_RAW_HS: return occ_for<EXC_RAW_HS>(seg);
     }
     return 4;
 }
@@ -1663,6 +1715,7 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s)
     case EXC_HS_SEGLRC: return probe_for_seg<EXC_HS_SEGLRC>(a, grid, s);
     case EXC_PARAPFD: return probe_for_seg<EXC_PARAPFD>(a, grid, s);
     case EXC_RAW: return probe_for_seg<EXC_RAW>(a, grid, s);
+    case EXC_RAW_HS: return probe_for_seg<EXC_RAW_HS>(a, grid, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1670,7 +1723,7 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t grid, hipStream_t s)
 hipError_t launch_unit_prologue(const ProbeArgs& a, hipStream_t s)
 {
     // HS probes find buckets directly; raw lists have no block heads
-    if (!a.nunits || a.ix.codec == EXC_HS_SEGLRC || a.ix.codec == EXC_RAW) return hipSuccess;
+    if (!a.nunits || a.ix.codec == EXC_HS_SEGLRC || raw_codec(a.ix.codec)) return hipSuccess;
     const uint32_t grid = (a.nunits + 255) / 256;
     if (a.ix.seg == 256)
         hipLaunchKernelGGL(k_unit_prologue<256>, dim3(grid), dim3(256), 0, s, a);
@@ -1702,6 +1755,7 @@ hipError_t launch_decompress(const DevIndex& ix, const DecUnit* units, uint32_t 
     case EXC_HS_SEGLRC: return decomp_for_seg<EXC_HS_SEGLRC>(ix, units, n, out, s);
     case EXC_PARAPFD: return decomp_for_seg<EXC_PARAPFD>(ix, units, n, out, s);
     case EXC_RAW: return decomp_for_seg<EXC_RAW>(ix, units, n, out, s);
+    case EXC_RAW_HS: return decomp_for_seg<EXC_RAW_HS>(ix, units, n, out, s);
     case EXC_NEWPFD: return decomp_for_seg<EXC_NEWPFD>(ix, units, n, out, s);
     }
     return hipErrorInvalidValue;

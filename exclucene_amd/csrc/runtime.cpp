@@ -406,8 +406,9 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
 {
     if (!c || !h || !out) return set_error(EXC_ERR_ARG, "exc_index_upload: null argument");
     const int codec = h->codec;
-    if (codec < EXC_LRC || codec > EXC_RAW) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
-    const bool raw = codec == EXC_RAW;
+    if (codec < EXC_LRC || codec > EXC_RAW_HS) return set_error(EXC_ERR_ARG, "unknown codec %d", codec);
+    const bool raw = codec == EXC_RAW || codec == EXC_RAW_HS;
+    const bool raw_hs = codec == EXC_RAW_HS;
     const uint32_t seg = raw ? 128u : h->seg_size;  // raw lists: the unit planner's block size
     const bool hs = codec == EXC_HS_SEGLRC;
     if (hs || raw) {
@@ -423,7 +424,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     if (codec == EXC_LRC && !h->lrc_info) return set_error(EXC_ERR_FORMAT, "LRC index is missing .regression_info");
     if ((codec == EXC_SEGLRC || hs) && (!h->seg_info || !h->seg_base))
         return set_error(EXC_ERR_FORMAT, "index is missing .regression_info/.ind2_distance_base");
-    if (hs && (!h->hash_offset || !h->hash_info))
+    if ((hs || raw_hs) && nt && (!h->hash_offset || !h->hash_info))
         return set_error(EXC_ERR_FORMAT, "HS index is missing .HashSeg_offset/.HashSeg_info");
 
     // ---- validate the layout so that every device read stays inside its array
@@ -454,6 +455,34 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
             R.nseg = (n + seg - 1) / seg;
             R.slope = 1.0f;
             R.range_l = R.range_r = 4.0e9f;
+            if (raw_hs) {
+                // [shift][start_0 .. start_nb]: nb is where the starts reach n
+                // (the last bucket holds the list's maximum, so no earlier
+                // start equals n); starts ascend from 0, and every bucket
+                // brackets its docIDs, so a probe that searches bucket
+                // min(v >> shift, nb - 1) is exact and reads inside the list
+                const uint64_t ho = h->hash_offset[t];
+                if (ho + 3 > h->hash_words) return set_error(EXC_ERR_FORMAT, "term %u: hash table outside _HashSeg_info", t);
+                const uint32_t* ht = h->hash_info + ho;
+                const uint32_t shift = ht[0];
+                if (shift > 31 || ht[1] != 0) return set_error(EXC_ERR_FORMAT, "term %u: bad hash table header", t);
+                uint64_t nb = 1;
+                while (ht[1 + nb] != n) {
+                    if (ht[1 + nb] < ht[nb] || ht[1 + nb] > n || ho + 2 + nb >= h->hash_words)
+                        return set_error(EXC_ERR_FORMAT, "term %u: bucket starts do not reach the list length", t);
+                    ++nb;
+                }
+                if (nb > 0xffffffffull) return set_error(EXC_ERR_FORMAT, "term %u: too many buckets", t);
+                for (uint64_t b = 1; b < nb; ++b) {
+                    const uint32_t s = ht[1 + b];
+                    if ((s > 0 && (L[s - 1] >> shift) >= b) || (s < n && (L[s] >> shift) < b))
+                        return set_error(EXC_ERR_FORMAT, "term %u: bucket %llu does not bracket its docIDs", t,
+                                         (unsigned long long)b);
+                }
+                R.nseg = (uint32_t)nb;
+                R.aux = (uint32_t)(ho + 1);
+                R.icpt = (int32_t)shift;
+            }
             continue;
         }
         uint32_t nseg;
@@ -564,7 +593,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->recs = recs;
     ix->dix.rec = (const SegRec*)up(srec.data(), srec.size() * sizeof(SegRec));
     if (!hs) ix->dix.seg_head = (const uint32_t*)up(h->seg_head, h->nseg * 4);
-    if (hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
+    if (hs || raw_hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
     if (rc) {
         exc_index_free(ix);
         return rc;
@@ -657,7 +686,9 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
     const bool hs = codec == EXC_HS_SEGLRC;
     if ((rc = map_file(p + ".ind1", ind1, false)) || (rc = map_file(p + ".ind2", ind2, false))) return rc;
     if (ind1.size() % sizeof(exc_term_ind1)) return set_error(EXC_ERR_FORMAT, ".ind1 size is not a multiple of 16");
-    if (codec == EXC_RAW) {  // the uncompressed programs' index: .ind1 + the docIDs in .ind2
+    if (codec == EXC_RAW || codec == EXC_RAW_HS) {
+        // the uncompressed programs' index: .ind1 + the docIDs in .ind2, and for
+        // the hash-segment program its tables (cuda_hashSegment.cu:157-186)
         exc_index_host h;
         std::memset(&h, 0, sizeof h);
         h.codec = codec;
@@ -666,6 +697,14 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
         h.ind1 = (const exc_term_ind1*)ind1.data();
         h.ind2 = (const uint32_t*)ind2.data();
         h.ind2_words = ind2.size() / 4;
+        if (codec == EXC_RAW_HS) {
+            if ((rc = map_file(p + "_HashSeg_offset", hoff, false)) || (rc = map_file(p + "_HashSeg_info", hinfo, false)))
+                return rc;
+            if (hoff.size() / 4 != h.nterms) return set_error(EXC_ERR_FORMAT, "_HashSeg_offset length differs from .ind1");
+            h.hash_offset = (const uint32_t*)hoff.data();
+            h.hash_info = (const uint32_t*)hinfo.data();
+            h.hash_words = hinfo.size() / 4;
+        }
         return exc_index_upload(c, &h, out);
     }
     if ((rc = map_file(p + ".segNum", segnum, false)) || (rc = map_file(p + ".segOffset", segoff, false))) return rc;
@@ -1916,7 +1955,7 @@ extern "C" int exc_autotune(exc_ctx* c, exc_index* ix, const uint32_t* qterms, c
     // candidates: cooperative-run limits per codec x search modes
     std::vector<uint32_t> coops;
     if (ix->codec == EXC_PARAPFD) coops = {16, 32, 64};
-    else if (ix->codec == EXC_HS_SEGLRC || ix->codec == EXC_RAW) coops = {0};
+    else if (ix->codec == EXC_HS_SEGLRC || ix->codec == EXC_RAW || ix->codec == EXC_RAW_HS) coops = {0};
     else coops = {0, 2, 4, 8, 16, 32, 64};
     std::vector<uint32_t> searches = {SEARCH_BINARY, SEARCH_FIT};
     if (!ix->models.empty()) searches.push_back(SEARCH_LEARNED);

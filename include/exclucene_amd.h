@@ -42,10 +42,17 @@ enum {
     EXC_NEWPFD = 4,    /* GC/NewPFD    : d-gaps in NewPFor blocks, Simple16
                           exceptions; decompression only, as in the reference
                           (GC/NewPFD/NewPFD_Decompression)                  */
-    EXC_RAW = 5        /* GU (src/gpu/Uncompressed): .ind2 holds the sorted
+    EXC_RAW = 5,       /* GU (src/gpu/Uncompressed): .ind2 holds the sorted
                           docIDs themselves; EXC_OPT_SEARCH 0 = binary search
                           (GU/cuda_binarySearch), else interpolation search
                           (GU/cuda_interpolationSearch)                      */
+    EXC_RAW_HS = 6     /* GU/cuda_hashSegment_new: the sorted docIDs as in
+                          EXC_RAW plus per-list bucket tables on the top docID
+                          bits, [shift][start_0 .. start_nb] in
+                          <name>_HashSeg_info, word offsets in
+                          <name>_HashSeg_offset (cuda_hashSegment.cu:815-947);
+                          a probe searches only its docID's bucket
+                          (cuda_hashSegment_kernel.cu:269-307), bound-checked */
 };
 
 /* ---- reference structs, byte-identical */
@@ -88,8 +95,9 @@ typedef struct {
     const exc_seg_info *seg_info;  /* nseg: SegLRC / HS                           */
     const int32_t *seg_base;       /* nseg: SegLRC / HS (.ind2_distance_base)     */
     const exc_seg_info *list_info; /* nterms: SegLRC list-level fit (optional)    */
-    const uint32_t *hash_offset;   /* nterms: HS                                  */
-    const uint32_t *hash_info;     /* HS tables [shift][nb][start_0..start_nb]    */
+    const uint32_t *hash_offset;   /* nterms: HS, RAW_HS                          */
+    const uint32_t *hash_info;     /* HS tables [shift][nb][start_0..start_nb];
+                                      RAW_HS tables [shift][start_0..start_nb]    */
     uint64_t hash_words;
 } exc_index_host;
 
@@ -295,7 +303,12 @@ int exc_index_export(exc_ctx *ctx, exc_index *ix, exc_built **out);
  *      and their SegLRC / HS_SegLRC / ParaPFD counterparts).
  * docs: concatenated sorted distinct docID lists, list t = docs[list_off[t] .. list_off[t+1]).
  * frac: ParaPFor / NewPFor exception fraction (FRAC); hs_target: expected bucket size (256);
- * hs_shift: 0 = adaptive shift, else a fixed shift for every list. */
+ * hs_shift: 0 = adaptive shift, else a fixed shift for every list.  Adaptive
+ * shifts follow each program's own rule: HS_SegLRC bits(max) - bits(ceil(n /
+ * hs_target)) (HS_SegLRC_Generator.cu:503-594), RAW_HS ceil(log2 max) -
+ * ceil(log2(n / hs_target)) in double logs and a one-bucket table for lists of
+ * at most hs_target docIDs (cuda_hashSegment.cu:872-905).  RAW_HS files:
+ * <name>.ind1, <name>.ind2, <name>_HashSeg_offset, <name>_HashSeg_info. */
 int exc_build_index(int codec, uint32_t seg_size, float frac, uint32_t hs_target,
                     uint32_t hs_shift, const uint32_t *docs, const uint64_t *list_off,
                     uint32_t nterms, int nthreads, exc_built **out);

@@ -67,6 +67,8 @@ def lib():
                                              _vp, _vp, _vp, _vp, _vp, _vp]
         L.orc_hs_table.restype = ctypes.c_uint32
         L.orc_hs_table.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp]
+        L.orc_gu_hashseg_table.restype = ctypes.c_uint32
+        L.orc_gu_hashseg_table.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp]
         L.orc_hs_encode_list.restype = ctypes.c_uint64
         L.orc_hs_encode_list.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_float, _vp, _vp, _vp, _vp]
         L.orc_pfd_encode_list.restype = ctypes.c_uint64
@@ -193,6 +195,19 @@ def ref_lib():
     L.ref_pfor_decode.restype = ctypes.c_uint32
     L.ref_pfor_decode.argtypes = [_vp, ctypes.c_uint32, _vp]
     return L
+
+
+def gu_hashseg_table(L, target=256, shift=0):
+    """The uncompressed hash-segment table of sorted list L:
+    [shift][start_0 .. start_nb] (oracle.c: orc_gu_hashseg_table)."""
+    a = _u32(L)
+    if shift:
+        words = (int(a[-1]) >> shift) + 3 if len(a) else 3
+    else:  # nb <= 2^ceil(log2(n / target)) + 1 (one more where the double log rounds up)
+        words = 4 * len(a) // max(target, 1) + 8
+    out = np.zeros(words + 3, np.uint32)
+    k = lib().orc_gu_hashseg_table(_p(a) if len(a) else None, len(a), target, shift, out.ctypes.data)
+    return out[:k].copy()
 
 
 def lrc_fit(L):

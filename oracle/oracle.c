@@ -342,6 +342,48 @@ uint32_t orc_hs_table(const uint32_t *L, uint32_t n, uint32_t bucket_target,
     return cur + 3;
 }
 
+/* Uncompressed hash-segment table, RunHashSegInfoGenerator of
+ * src/gpu/Uncompressed/src/cuda_hashSegment_new/cuda_hashSegment.cu:815-947:
+ * DOCID_BITNUM = log(max)/log(2) rounded up when it has a fraction (:872-881);
+ * lists of at most UNI_LEN (= target) docIDs get [bits, 0, n] (:884-891);
+ * longer ones nK = log(n / target)/log(2) rounded up the same way, shift =
+ * bits - nK, then the walk that writes each bucket's first index and, after
+ * the last bucket, n (:892-922): [shift][start_0 .. start_nb], nb = (max >>
+ * shift) + 1.  A fixed shift (not in the reference: the C3 layout on the top
+ * docID bits) walks with that shift for every list.  Where the reference takes
+ * log(0) (max = 0, or an empty list, whose "last" docID it reads past the list)
+ * this restatement uses bits = 0 and, for an empty list, [shift, 0, 0]. */
+static uint32_t gu_ceil_log2(double x)
+{
+    if (!(x > 0.0)) return 0;
+    double d = log(x) / log((double)2);
+    if (d <= 0.0) return 0;
+    if (d - (unsigned int)d > 0) return ((unsigned int)d) + 1;
+    return (unsigned int)d;
+}
+
+uint32_t orc_gu_hashseg_table(const uint32_t *L, uint32_t n, uint32_t target, uint32_t fixed_shift,
+                              uint32_t *info)
+{
+    if (n == 0) { info[0] = fixed_shift; info[1] = 0; info[2] = 0; return 3; }
+    uint32_t shift = fixed_shift;
+    if (!fixed_shift) {
+        uint32_t bits = gu_ceil_log2((double)L[n - 1]);
+        if (n <= target) { info[0] = bits; info[1] = 0; info[2] = n; return 3; }
+        shift = bits - gu_ceil_log2(((double)n) / target);
+    }
+    info[0] = shift;
+    uint32_t cur = 0;
+    info[1] = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t b = L[i] >> shift;
+        while (cur < b) { ++cur; info[1 + cur] = i; }
+    }
+    ++cur;
+    info[1 + cur] = n;
+    return cur + 2;
+}
+
 /* per-bucket fit, residual, base: HS_SegLRC_Generator.cu:95-288 and
  * HS_SegLRC_Generator_kernel.cu:64-114; packing: HS_SegLRC_Compression/main.cpp:149-178
  * (an empty bucket has no block; its segOffset equals the next block's). */

@@ -84,6 +84,10 @@ int orc_seglrc_decode_list(const uint32_t *words, uint32_t n, uint32_t seg,
  * fixed_shift == 0 selects the reference's adaptive rule. */
 uint32_t orc_hs_table(const uint32_t *L, uint32_t n, uint32_t bucket_target,
                       uint32_t fixed_shift, uint32_t *info);
+/* uncompressed hash-segment table [shift][start_0 .. start_nb]
+ * (cuda_hashSegment.cu:815-947); returns word count (<= (max >> shift) + 3). */
+uint32_t orc_gu_hashseg_table(const uint32_t *L, uint32_t n, uint32_t target, uint32_t fixed_shift,
+                              uint32_t *info);
 uint64_t orc_hs_encode_list(const uint32_t *L, uint32_t n, const uint32_t *info, float frac,
                             orc_seg_info *fit, int32_t *base, uint32_t *words, uint32_t *seg_off);
 int orc_hs_decode_list(const uint32_t *words, uint32_t n, const uint32_t *info,

@@ -146,3 +146,22 @@ def test_no_gpu_calls_without_device():
         pytest.skip("a GPU may be present")
     with pytest.raises(ea.ExcError):
         ea.Context(0)
+
+
+@pytest.mark.parametrize("shift", [0, 13, 9])
+def test_raw_hash_segment_tables_equal_oracle(shift, tmp_path):
+    """The uncompressed hash-segment index: .ind2 is the docIDs, and every
+    list's [shift][start_0 .. start_nb] table equals the oracle's restatement
+    of the reference's generator; the files carry the reference's names."""
+    lists = _lists(5) + [np.array([3, 1 << 29], np.uint32), np.array([1 << 24], np.uint32)]
+    docs, off = ea.lists_to_flat(lists)
+    b = ea.build_index(docs, off, ea.RAW_HS, hs_shift=shift, nthreads=3)
+    assert np.array_equal(b.ind2, np.concatenate(lists))
+    ends = np.append(b.hash_offset[1:], len(b.hash_info))
+    for t, L in enumerate(lists):
+        assert np.array_equal(b.hash_info[b.hash_offset[t]:ends[t]], oracle.gu_hashseg_table(L, shift=shift)), t
+    b.save(str(tmp_path), "at")
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert "at_HashSeg_info" in names and "at_HashSeg_offset" in names
+    assert np.array_equal(np.fromfile(tmp_path / "at_HashSeg_info", np.uint32), b.hash_info)
+    assert np.array_equal(np.fromfile(tmp_path / "at_HashSeg_offset", np.uint32), b.hash_offset)

@@ -133,6 +133,34 @@ def test_c3_full_size_hash_buckets(ctx, c2, shift):
         dix.close()
 
 
+@pytest.mark.parametrize("shift", [13, 0])
+def test_c3_full_size_uncompressed_hash_buckets(ctx, c2, shift):
+    """C3 on the uncompressed hash-segment layout (the docIDs themselves plus
+    [shift][start_0 .. start_nb] tables: shift 13 = the top 12 bits of 2^25, or
+    the reference generator's own adaptive shift): all 10,000 ratio queries
+    equal to the oracle in both launch modes, binary and interpolation search
+    inside the buckets."""
+    docs, off, _, built, _, _, df = c2
+    queries = W.ratio_queries(df, 10_000, 3)
+    ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
+    ocum = np.asarray(ocum, np.int64)
+    rb = ea.build_index(docs, off, ea.RAW_HS, hs_shift=shift)
+    dix = ctx.upload(rb)
+    try:
+        for persistent in (0, 1):
+            for search in (0, 1):
+                ctx.set_option(ea.OPT_PERSISTENT, persistent)
+                ctx.set_option(ea.OPT_SEARCH, search)
+                cum, res = ctx.intersect(dix, queries)
+                assert np.array_equal(np.asarray(cum, np.int64), ocum), (persistent, search)
+                assert np.array_equal(res, ores), (persistent, search)
+    finally:
+        ctx.set_option(ea.OPT_SEARCH, 1)
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        dix.close()
+        rb.close()
+
+
 def test_c4_full_query_set(ctx, c2):
     """C4's own query set: 10,000 queries of 2-4 terms (seed 44) over the
     full-size SegLRC index, with PLA-256 and then MLP 1-32-1 models on every

@@ -399,3 +399,46 @@ def test_mlp_gradient_planted_slip_is_caught(slip):
     params = _mlp_params(rng)
     err = _fd_max_rel_err(params, L, 256, lambda p: oracle.mlp_loss_grad(p, L, 256, _plant=slip))
     assert err > 1e-2, err
+
+
+# ---------------------------------------------------------------- uncompressed hash segments
+
+def _int_ceil_log2(x):
+    """ceil(log2 x) in exact integer arithmetic (0 for x <= 1)."""
+    return (int(x) - 1).bit_length() if x > 1 else 0
+
+
+def test_gu_hashseg_table_closed_form():
+    """The uncompressed hash-segment tables (cuda_hashSegment.cu:815-947)
+    against their closed form: lists of at most 256 docIDs get
+    [ceil(log2 max), 0, n]; longer lists [shift][start_0 .. start_nb] with
+    shift = ceil(log2 max) - ceil(log2(n / 256)), nb = (max >> shift) + 1 and
+    start_b = the number of docIDs below b << shift.  The reference takes the
+    logarithms in doubles, which round above the integer for max = 2^29 and
+    2^31 only: there its bit count, and so its shift, is one more."""
+    rng = np.random.default_rng(47)
+    lists = [rand_list(rng, n, 1 << 25) for n in (1, 2, 255, 256, 257, 511, 512, 513, 4096, 70000, 1 << 20)]
+    lists += [fixture_list(), np.arange(1000, 1000 + 777, dtype=np.uint32)]
+    lists += [np.array([5, 1 << k], np.uint32) for k in (10, 24, 28, 29, 30)]
+    lists += [np.sort(np.append(rng.choice(1 << 20, 600, replace=False), 1 << 29)).astype(np.uint32)]
+    for L in lists:
+        n, mx = len(L), int(L[-1])
+        bits = _int_ceil_log2(mx) + (1 if mx in (1 << 29, 1 << 31) else 0)
+        h = oracle.gu_hashseg_table(L)
+        if n <= 256:
+            assert list(h) == [bits, 0, n], (n, mx, list(h))
+            continue
+        # ceil(log2(n / 256)) = ceil(log2 n) - 8 for n > 256
+        shift = bits - (_int_ceil_log2(n) - 8)
+        assert h[0] == shift, (n, mx, h[0], shift)
+        nb = (mx >> shift) + 1
+        assert len(h) == nb + 2
+        starts = np.searchsorted(L, np.arange(nb, dtype=np.int64) << shift, side="left")
+        assert np.array_equal(h[1:1 + nb], starts) and h[1 + nb] == n
+    # a fixed shift: the same walk with that shift, for short lists too
+    for L in lists[:4]:
+        h = oracle.gu_hashseg_table(L, shift=13)
+        nb = (int(L[-1]) >> 13) + 1
+        assert h[0] == 13 and len(h) == nb + 2 and h[-1] == len(L)
+        assert np.array_equal(h[1:1 + nb], np.searchsorted(L, np.arange(nb, dtype=np.int64) << 13, side="left"))
+    assert list(oracle.gu_hashseg_table(np.zeros(0, np.uint32))) == [0, 0, 0]
