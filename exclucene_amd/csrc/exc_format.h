@@ -22,7 +22,14 @@
 
 namespace exc {
 
-constexpr uint32_t kPadWords = 64;   // zero words after every device array read as value pairs
+// Zero words after every device array read as value pairs.  Block decoders
+// read field j of a block for every j < the segment size (256 at most)
+// without clamping j to the block's length: the highest word such a read
+// touches is word (255 * 31) / 32 + 1 = 248 of the block's fields, so 256
+// words of padding keep the reads of an array's last block inside it.
+constexpr uint32_t kPadWords = 256;
+constexpr uint32_t kMaxSeg = 256;
+static_assert(((kMaxSeg - 1) * 31) / 32 + 2 <= kPadWords, "unclamped field reads stay inside the padding");
 
 // float -> int32 as the reference's GPU code gets it (CUDA's cvt.rzi.s32.f32)
 // and as gfx950's v_cvt_i32_f32 does it: toward zero, out-of-range values

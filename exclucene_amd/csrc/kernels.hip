@@ -61,6 +61,26 @@ __device__ __forceinline__ uint32_t below_me(uint64_t m)
 }
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
 
+// A wave-uniform float copied into a vector register: a VALU instruction with
+// two scalar operands needs one of them moved first (one scalar operand per
+// instruction on gfx9), so a value used that way across a loop is moved once.
+__device__ __forceinline__ float vgpr_copy(float x)
+{
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
+// x >> 5 as one instruction the compiler cannot look through: it otherwise
+// rewrites ((x >> 5) << 2) + base as ((x >> 3) & ~3) + base, three
+// instructions where a shift and one v_lshl_add suffice.
+__device__ __forceinline__ uint32_t word_of_bit(uint32_t x)
+{
+    uint32_t r;
+    asm("v_lshrrev_b32 %0, 5, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // lb-bit value j of a packed run starting at p (LSB-first, may straddle two
 // words; GI/unpack.h:14-23).  Reads p[w] and p[w+1]; every device array that
 // is read this way carries kPadWords zero words at its end.
@@ -142,19 +162,23 @@ constexpr bool gap_codec(int codec) { return codec == EXC_PARAPFD || codec == EX
 // Raw lb-bit fields of one block for this lane: strided positions
 // (j = lane + 64 t) for the random-access codecs, consecutive (j = lane*V + t)
 // for the d-gap codecs whose prefix sum runs along a lane first.
-template <int CODEC, int SEG>
+// Every lane loads every field position j < SEG, inside the block or not (an
+// array's last block reads into its kPadWords of zero padding), so that all
+// loads of a group are in flight before the first wait and no lane spends
+// instructions clamping its position.  ZERO: fields past the block's length
+// read as 0 (callers that test j < len themselves skip it).
+template <int CODEC, int SEG, bool ZERO = true>
 __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, uint32_t lb, uint32_t len,
                                             uint32_t (&f)[SEG / 64])
 {
+    static_assert(SEG <= (int)kMaxSeg, "field reads past the block stay inside the padding");
     constexpr int V = SEG / 64;
     const uint32_t ln = lane_id();
 #pragma unroll
     for (int t = 0; t < V; ++t) {
         const uint32_t j = gap_codec(CODEC) ? ln * V + t : ln + 64 * t;
-        // every lane loads (from a clamped position inside the block) so that
-        // all loads of a group are in flight before the first wait
-        const uint32_t x = bits_at(data, lb, len ? min(j, len - 1u) : 0u);
-        f[t] = (j < len) ? x : 0u;
+        const uint32_t x = bits_at(data, lb, j);
+        f[t] = (!ZERO || j < len) ? x : 0u;
     }
 }
 
@@ -251,10 +275,13 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
         }
     } else if constexpr (CODEC == EXC_SEGLRC) {
         // GC/SegLRC/SegLRC_Decompression/SegLRC_Decompression_kernel.cu:200-256
+        // (the intercept moved to a vector register once: seg_pred's fma then
+        // reads one scalar operand)
+        const float icv = vgpr_copy(rec.icpt);
 #pragma unroll
         for (int t = 0; t < V; ++t) {
             const uint32_t j = ln + 64 * t;
-            if (j < len) dst[j] = f[t] + seg_pred(j, rec.slope, rec.icpt, rec.base);
+            if (j < len) dst[j] = f[t] + seg_pred(j, rec.slope, icv, rec.base);
         }
     } else {
         // ParaPFD / NewPFD: patch exceptions, segHead, inclusive scan
@@ -683,19 +710,28 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                             rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
                             rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
                             len[g] = seg_len(R, sf + (uint32_t)src, SEG);
-                            load_fields<CODEC, SEG>(A.ix.words + R.woff + rec[g].off + 1, desc_lb(rec[g].desc), len[g],
-                                                    f[g]);
+                            // fields past len are not zeroed: the insertion below tests j < len
+                            load_fields<CODEC, SEG, false>(A.ix.words + R.woff + rec[g].off + 1, desc_lb(rec[g].desc),
+                                                           len[g], f[g]);
                         }
                     }
 #pragma unroll
                     for (int g = 0; g < NBB; ++g) {
                         if ((uint32_t)g < np) {
+                            // the block fit's intercept in a vector register once per block, so
+                            // that each value's fma reads one scalar operand (the slope) only
+                            const float icv = vgpr_copy(rec[g].icpt);
 #pragma unroll
                             for (int t = 0; t < SEG / 64; ++t) {
                                 const uint32_t j = ln + 64 * t;
-                                const uint32_t x = block_value<CODEC, SEG>(R, sf + p0 + g, rec[g], j, f[g][t]) - lo;
+                                uint32_t x;
+                                if constexpr (CODEC == EXC_SEGLRC)
+                                    x = f[g][t] + (uint32_t)f2i_sat(fmaf((float)j, rec[g].slope, icv)) +
+                                        (uint32_t)rec[g].base - lo;
+                                else
+                                    x = block_value<CODEC, SEG>(R, sf + p0 + g, rec[g], j, f[g][t]) - lo;
                                 if (j < len[g] && x <= hi - lo)
-                                    __hip_atomic_fetch_or(bm + (x >> 5), 1u << (x & 31), __ATOMIC_RELAXED,
+                                    __hip_atomic_fetch_or(bm + word_of_bit(x), 1u << (x & 31), __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
                             }
                         }
