@@ -256,6 +256,25 @@ __device__ __forceinline__ ExcPrefetch prefetch_exceptions(const uint32_t* __res
     return x;
 }
 
+// The exceptions of a ParaPFor block whose fields are staged in dst[0..len)
+// ORed in place, lane-parallel (GI/ParaPFor.cpp:107-126: field[pos_e] |=
+// high_e << lb; positions are distinct, so no two lanes write one word).
+// PREFETCH: exception e < 64's position and high part arrived in xp with the
+// fields.  Positions at or past len (none in a valid block) are ignored.
+template <bool PREFETCH>
+__device__ __forceinline__ void pfd_patch_lds(const uint32_t* __restrict__ data, uint32_t desc, uint32_t len,
+                                              ExcPrefetch xp, uint32_t* dst)
+{
+    const uint32_t lb = desc_lb(desc), en = desc_en(desc), eb = desc_eb(desc), hb = desc_hb(desc);
+    const uint32_t* pp = data + words_for(len, lb);
+    const uint32_t* hp = pp + words_for(en, eb);
+    for (uint32_t e = lane_id(); e < en; e += 64) {
+        const uint32_t p = (PREFETCH && e < 64) ? xp.a : bits_at(pp, eb, e);
+        const uint32_t h = (PREFETCH && e < 64) ? xp.b : bits_at(hp, hb, e);
+        if (p < len) dst[p] |= h << lb;
+    }
+}
+
 // Finish decoding block s (fields already loaded) into dst[0..len); xs is
 // the NewPFor exception scratch (2 SEG words), xp the prefetched exceptions.
 template <int CODEC, int SEG, bool PREFETCH = false>
@@ -299,14 +318,7 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
                 const uint32_t* sp = data + words_for(len, lb);
                 npfd_patch(sp, en, lb, len, PREFETCH ? xp.a : sp[ln], xs, dst);
             } else {
-                const uint32_t eb = desc_eb(rec.desc), hb = desc_hb(rec.desc);
-                const uint32_t* pp = data + words_for(len, lb);
-                const uint32_t* hp = pp + words_for(en, eb);
-                for (uint32_t e = ln; e < en; e += 64) {
-                    const uint32_t p = (PREFETCH && e < 64) ? xp.a : bits_at(pp, eb, e);
-                    const uint32_t h = (PREFETCH && e < 64) ? xp.b : bits_at(hp, hb, e);
-                    if (p < len) dst[p] |= h << lb;
-                }
+                pfd_patch_lds<PREFETCH>(data, rec.desc, len, xp, dst);
             }
             wave_sync();
 #pragma unroll
@@ -582,10 +594,16 @@ __device__ __forceinline__ void model_window(const DevIndex& ix, const TermRec& 
     wb = phi <= 0.0f ? 1 : (int64_t)(phi / SEG) + 2;
 }
 
+// Per-wave LDS: the unit's live candidates, then a buffer for decoded
+// segments or a docID-window bitmap.  The ParaPFD distinct-segment path keeps
+// its list of segments in the buffer's last kChunk words and decodes into the
+// first kPfdDecWords, so every compressed codec fits 1,280 words per wave
+// (eight waves per SIMD as far as LDS goes).
+constexpr uint32_t kPfdDecWords = kBufWords - kChunk;
 struct LdsWave {
     uint32_t* cand;   // [kChunk]    live candidates (sorted)
-    uint32_t* buf;    // [kBufWords] decoded segments
-    uint32_t* dlist;  // [kChunk]    distinct segments (ParaPFD long-span path only)
+    uint32_t* buf;    // [kBufWords] decoded segments / window bitmap
+    uint32_t* dlist;  // [kChunk]    = buf + kPfdDecWords: distinct segments (ParaPFD long-span path only)
 };
 
 template <int CODEC, int SEG>
@@ -670,6 +688,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 
     const uint32_t cmax = knob_coop(knobs);
     constexpr uint32_t kWin = kBufWords * 32u;  // docIDs one LDS bitmap covers
+    constexpr uint32_t kWinPfd = (kBufWords - SEG) * 32u;  // ParaPFD: the buffer's tail stages a block
     constexpr int NBB = kBitmapPiece / SEG;       // blocks per load round of the bitmap path
     if (CODEC != EXC_PARAPFD && r <= cmax && ((vmax - vmin) / kWin) < kMaxWindows) {
         // ---- LRC/SegLRC: the run sf .. sf+r-1 decoded straight from
@@ -730,6 +749,96 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                                         (uint32_t)rec[g].base - lo;
                                 else
                                     x = block_value<CODEC, SEG>(R, sf + p0 + g, rec[g], j, f[g][t]) - lo;
+                                if (j < len[g] && x <= hi - lo)
+                                    __hip_atomic_fetch_or(bm + word_of_bit(x), 1u << (x & 31), __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    const uint32_t vi = cand_at(i), x = vi - lo;
+                    if (ln + 64 * i < nlive && vi >= lo && x <= hi - lo) found[i] = (bm[x >> 5] >> (x & 31)) & 1u;
+                }
+                wave_sync();
+            }
+            if (hi == vmax) break;
+            lo = hi + 1;
+        }
+    } else if (CODEC == EXC_PARAPFD && r <= cmax && ((vmax - vmin) / kWinPfd) < kMaxWindows) {
+        // ---- ParaPFD: the run sf .. sf+r-1 decoded in registers straight into
+        // the bitmap of each docID window, as the LRC/SegLRC path does (the
+        // reference walks one block serially per candidate,
+        // GC/ParaPFD/ParaPFD_Intersection_FRAC/ParaPFD_Intersection_kernel.cu:322-455).
+        // Each lane takes PV consecutive fields of a block; lanes e < en load
+        // exception e with the fields.  A block with exceptions is staged in
+        // the wave's distinct-segment area (idle on this path), patched there
+        // lane-parallel and read back; then the block's head replaces its
+        // first gap and a wave scan of the lanes' sums turns the gaps into
+        // docIDs.  No candidate searches: each tests its bit.
+        constexpr int PV = SEG / 64;
+        constexpr int NBP = 4;  // blocks per load round
+        uint32_t* bm = W.buf;                     // kBufWords - SEG words of bitmap
+        uint32_t* stg = W.buf + (kBufWords - SEG);  // then one block's staging
+        const SegRec gr = A.ix.rec[R.segnum + sf + min(ln, r - 1)];
+        const uint32_t hd = ln < r ? gr.head : kNone;
+        for (uint32_t lo = vmin;;) {
+            const uint32_t hi = (vmax - lo < kWinPfd) ? vmax : lo + (kWinPfd - 1);
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const uint32_t vi = cand_at(i);
+                any = any || (ln + 64 * i < nlive && vi >= lo && vi <= hi);
+            }
+            if (__ballot(any)) {
+                const uint32_t c_lo = popc(__ballot(hd <= lo)), c_hi = popc(__ballot(hd <= hi));
+                const uint32_t b0 = c_lo ? c_lo - 1 : 0, b1 = c_hi ? c_hi - 1 : 0;
+                const uint32_t nw = ((hi - lo) >> 5) + 1;
+                for (uint32_t e = ln; e < nw; e += 64) bm[e] = 0;
+                wave_sync();
+                for (uint32_t p0 = b0; p0 <= b1; p0 += NBP) {
+                    const uint32_t np = min((uint32_t)NBP, b1 + 1 - p0);
+                    uint32_t desc[NBP], head[NBP], len[NBP];
+                    ExcPrefetch xp[NBP];
+                    const uint32_t* data[NBP];
+                    uint32_t f[NBP][PV];
+#pragma unroll
+                    for (int g = 0; g < NBP; ++g) {
+                        // past np: a copy of the piece's last block, loaded but not used
+                        const int src = (int)min(p0 + g, b1);
+                        desc[g] = uni(__builtin_amdgcn_readlane(gr.desc, src));
+                        head[g] = uni(__builtin_amdgcn_readlane(gr.head, src));
+                        len[g] = seg_len(R, sf + (uint32_t)src, SEG);
+                        data[g] = A.ix.words + R.woff + uni(__builtin_amdgcn_readlane(gr.off, src)) + 1;
+                        load_fields<CODEC, SEG, false>(data[g], desc_lb(desc[g]), len[g], f[g]);
+                        xp[g] = prefetch_exceptions<CODEC>(data[g], desc[g], len[g]);
+                    }
+#pragma unroll
+                    for (int g = 0; g < NBP; ++g) {
+                        if ((uint32_t)g < np) {
+                            if (desc_en(desc[g])) {
+                                // stage, patch, read back (each lane stages and
+                                // reads its own words; the barrier after the
+                                // staging also orders the previous block's reads
+                                // before this block's patch)
+#pragma unroll
+                                for (int t = 0; t < PV; ++t) stg[ln * PV + t] = f[g][t];
+                                wave_sync();
+                                pfd_patch_lds<true>(data[g], desc[g], len[g], xp[g], stg);
+                                wave_sync();
+#pragma unroll
+                                for (int t = 0; t < PV; ++t) f[g][t] = stg[ln * PV + t];
+                            }
+                            if (ln == 0) f[g][0] = head[g];
+                            uint32_t acc = 0;
+#pragma unroll
+                            for (int t = 0; t < PV; ++t) { acc += f[g][t]; f[g][t] = acc; }
+                            const uint32_t base = wave_incl_scan(acc) - acc - lo;
+#pragma unroll
+                            for (int t = 0; t < PV; ++t) {
+                                const uint32_t j = ln * PV + t, x = f[g][t] + base;
                                 if (j < len[g] && x <= hi - lo)
                                     __hip_atomic_fetch_or(bm + word_of_bit(x), 1u << (x & 31), __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -852,7 +961,10 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 #pragma unroll
         for (int i = 0; i < V; ++i) s[i] = seg_of(v[i], ln + 64 * i < nlive);
         if constexpr (CODEC == EXC_PARAPFD) {
-            // ---- decode each distinct needed segment once, NB per round
+            // ---- decode each distinct needed segment once, ND per round into
+            // the buffer's first kPfdDecWords (its tail holds the list)
+            constexpr int ND = (int)(kPfdDecWords / SEG);
+            static_assert(ND >= 1, "one segment fits the decode area");
             uint32_t nd = 0, di[V];
 #pragma unroll
             for (int i = 0; i < V; ++i) {
@@ -868,13 +980,13 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             }
             nd = uni(nd);
             wave_sync();
-            for (uint32_t r0 = 0; r0 < nd; r0 += NB) {
-                const uint32_t gn = min((uint32_t)NB, nd - r0);
-                decode_blocks<CODEC, SEG, NB, true>(A.ix, R, gn, [&](int g) { return W.dlist[r0 + g]; }, W.buf);
+            for (uint32_t r0 = 0; r0 < nd; r0 += ND) {
+                const uint32_t gn = min((uint32_t)ND, nd - r0);
+                decode_blocks<CODEC, SEG, ND, true>(A.ix, R, gn, [&](int g) { return W.dlist[r0 + g]; }, W.buf);
                 wave_sync();
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
-                    if (s[i] >= 0 && di[i] >= r0 && di[i] < r0 + NB)
+                    if (s[i] >= 0 && di[i] >= r0 && di[i] < r0 + ND)
                         found[i] = lds_contains(W.buf + (di[i] - r0) * SEG, seg_len(R, (uint32_t)s[i], SEG), v[i]);
                 }
                 wave_sync();
@@ -1162,14 +1274,14 @@ constexpr bool raw_codec(int codec) { return codec == EXC_RAW || codec == EXC_RA
 template <int CODEC>
 constexpr int wave_lds_words()
 {
-    return raw_codec(CODEC) ? (int)kChunk : (int)(kChunk + kBufWords + (CODEC == EXC_PARAPFD ? kChunk : 0));
+    return raw_codec(CODEC) ? (int)kChunk : (int)(kChunk + kBufWords);
 }
 
 template <int CODEC, int SEG>
 __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
 {
     constexpr uint32_t U = kChunk / SEG;  // shortest-list segments per unit
-    const LdsWave W{L, L + kChunk, L + kChunk + kBufWords};
+    const LdsWave W{L, L + kChunk, L + kChunk + kPfdDecWords};
     const uint32_t ln = lane_id();
     const UnitRec ur = A.units[u];
     const uint32_t k = uni(ur.k), tc = uni((uint32_t)ur.tc), knobs = uni((uint32_t)ur.knobs);
@@ -1272,7 +1384,7 @@ __device__ uint32_t claim_unit(const ProbeArgs& A, ClaimLds* q)
 // 16-byte claim record takes it to 7 workgroups per CU: eight would need
 // 8 x 20,496 bytes of the 160 KiB.
 template <int CODEC, int SEG, bool PERSIST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CODEC == EXC_PARAPFD ? 1 : (PERSIST ? 7 : 8), PERSIST ? 7 : 8))) void k_probe(ProbeArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERSIST ? 7 : 8, PERSIST ? 7 : 8))) void k_probe(ProbeArgs A)
 {
     __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);

@@ -32,7 +32,9 @@ def _queries(nl, seed):
 
 CASES = [(ea.LRC, 256, 0.0, 0), (ea.LRC, 128, 0.0, 0), (ea.SEGLRC, 128, 0.0, 0), (ea.SEGLRC, 256, 0.0, 0),
          (ea.HS_SEGLRC, 128, 0.0, 0), (ea.HS_SEGLRC, 128, 0.0, 13), (ea.PARAPFD, 128, 0.1, 0),
-         (ea.PARAPFD, 256, 0.1, 0), (ea.PARAPFD, 128, 0.0, 0)]
+         (ea.PARAPFD, 256, 0.1, 0), (ea.PARAPFD, 128, 0.0, 0),
+         # more than 64 exceptions per block: patched in several rounds
+         (ea.PARAPFD, 256, 0.5, 0), (ea.PARAPFD, 128, 0.6, 0)]
 
 
 @pytest.mark.parametrize("codec,seg,frac,shift", CASES)
