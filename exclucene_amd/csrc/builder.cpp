@@ -51,17 +51,10 @@ namespace {
 
 // ---------------------------------------------------------------- ParaPFor
 
-// Smallest b with #(v >= 2^b) <= frac * n, compared in float (GI/ParaPFor.cpp:31-53).
+// Smallest b with #(v >= 2^b) <= frac * n (the shared rule of exc_encode.h).
 uint32_t pfor_best_b(const uint32_t* v, uint32_t n, float frac)
 {
-    uint32_t hist[33] = {0};
-    for (uint32_t i = 0; i < n; ++i) hist[bits_of(v[i])]++;
-    uint32_t above = n - hist[0];  // values needing more than b bits, b = 0
-    for (uint32_t b = 0; b < 32; ++b) {
-        if ((float)above <= frac * (float)n) return b;
-        above -= hist[b + 1];
-    }
-    return 32;
+    return exc::pfor_best_b([v](uint32_t i) { return v[i]; }, n, frac);
 }
 
 void put_bits(uint32_t* out, const uint32_t* v, uint32_t n, uint32_t b)
@@ -77,35 +70,16 @@ void put_bits(uint32_t* out, const uint32_t* v, uint32_t n, uint32_t b)
     }
 }
 
-// One ParaPFor block appended to out; false if the descriptor cannot hold it.
+// One ParaPFor block appended to out, encoded by the code the device builder
+// runs too (exc_encode.h); false if the descriptor cannot hold it.
 bool pfor_block(const uint32_t* v, uint32_t n, float frac, std::vector<uint32_t>& out)
 {
-    uint32_t lb = pfor_best_b(v, n, frac);
-    if (lb > 31) return false;
-    std::vector<uint32_t> low(v, v + n), pos, hi;
-    for (uint32_t i = 0; i < n; ++i) {
-        if (bits_of(v[i]) > lb) {
-            pos.push_back(i);
-            hi.push_back(v[i] >> lb);
-            low[i] = v[i] & ((1u << lb) - 1u);
-        }
-    }
-    uint32_t en = (uint32_t)pos.size();
-    uint32_t eb = en ? bits_of(pos.back()) : 0, hb = 0;
-    for (uint32_t h : hi) hb = std::max(hb, bits_of(h));
-    if (en > 65535 || hb > 31) return false;
-    size_t at = out.size();
-    out.resize(at + 1 + words_for(n, lb) + words_for(en, eb) + words_for(en, hb));
-    uint32_t* o = out.data() + at;
-    o[0] = en ? (lb << 26) | (eb << 21) | (hb << 16) | en : (lb << 26);
-    o += 1;
-    put_bits(o, low.data(), n, lb);
-    o += words_for(n, lb);
-    if (en) {
-        put_bits(o, pos.data(), en, eb);
-        o += words_for(en, eb);
-        put_bits(o, hi.data(), en, hb);
-    }
+    const auto val = [v](uint32_t i) { return v[i]; };
+    const PforPlan p = pfor_plan(val, n, frac);
+    if (!p.words) return false;
+    const size_t at = out.size();
+    out.resize(at + p.words);
+    pfor_write(val, n, p.desc, out.data() + at);
     return true;
 }
 
@@ -175,32 +149,6 @@ bool npfd_block(const uint32_t* v, uint32_t n, float frac, std::vector<uint32_t>
 }
 
 // ---------------------------------------------------------------- fits
-
-exc_lrc_info lrc_fit(const uint32_t* L, uint32_t n)
-{
-    exc_lrc_info r{1.0f, 0, 0.0f};
-    if (n == 0) return r;
-    if (n == 1) { r.iIntercept = (int32_t)L[0]; return r; }
-    double xa = 0, ya = 0;
-    for (uint32_t i = 0; i < n; ++i) { xa += i; ya += L[i]; }
-    xa /= n; ya /= n;
-    double sx = 0, sxy = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        double dx = i - xa;
-        sx += dx * dx;
-        sxy += dx * ((double)L[i] - ya);
-    }
-    double slope = sxy / sx, icpt = ya - slope * xa, rl = 0, rr = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        double d = (L[i] - icpt) / slope - i;
-        if (d > rl) rl = d;
-        else if (d < rr) rr = d;
-    }
-    r.fSlope = (float)slope;
-    r.iIntercept = (int32_t)icpt;
-    r.fSearchRange = (float)(rl - rr);
-    return r;
-}
 
 // residuals against a segment fit, shifted by their minimum; returns the base
 int32_t seg_residuals(const uint32_t* L, uint32_t n, const exc_seg_info& f, uint32_t* vd)
@@ -368,35 +316,47 @@ void parallel_for(uint32_t n, int nthreads, F f)
 
 // A SegLRC index in the reference's arrays from device-resident records
 // (exc_index_export of an index built on the GPU).
-int exc::built_from_seglrc(uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
-                      const uint32_t* head, const uint32_t* words, uint64_t nwords, exc_built** out)
+int exc::built_from_device(int codec, uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg,
+                           const SegRec* rec, const uint32_t* head, const uint32_t* words, uint64_t nwords,
+                           exc_built** out)
 {
     auto* b = new exc_built();
-    b->codec = EXC_SEGLRC;
+    b->codec = codec;
     b->seg = seg;
     b->ind1.resize(nterms);
     b->seg_num.resize(nterms);
-    b->list_info.resize(nterms);
+    if (codec == EXC_SEGLRC) b->list_info.resize(nterms);
+    if (codec == EXC_LRC) b->lrc.resize(nterms);
     for (uint32_t t = 0; t < nterms; ++t) {
         const TermRec& R = terms[t];
         b->ind1[t].m_urlcount = R.n;
         b->ind1[t].m_off = R.woff * 4;
         b->seg_num[t] = R.segnum;
-        exc_seg_info& li = b->list_info[t];
-        li.fSlope = R.slope;
-        std::memcpy(&li.fIntercept, &R.icpt, 4);
-        li.nRangeLeft = (uint32_t)R.range_l;
-        li.nRangeRight = (uint32_t)R.range_r;
+        if (codec == EXC_SEGLRC) {
+            exc_seg_info& li = b->list_info[t];
+            li.fSlope = R.slope;
+            std::memcpy(&li.fIntercept, &R.icpt, 4);
+            li.nRangeLeft = (uint32_t)R.range_l;
+            li.nRangeRight = (uint32_t)R.range_r;
+        } else if (codec == EXC_LRC) {
+            b->lrc[t] = exc_lrc_info{R.slope, R.icpt, R.range_l};
+        }
     }
     b->ind2.assign(words, words + nwords);
     b->seg_offset.resize(nseg);
     b->seg_head.assign(head, head + nseg);
-    b->seg_info.resize(nseg);
-    b->base.resize(nseg);
+    if (codec == EXC_SEGLRC) {
+        b->seg_info.resize(nseg);
+        b->base.resize(nseg);
+    }
+    if (codec == EXC_PARAPFD) b->seg_median.resize(nseg);
     for (uint64_t g = 0; g < nseg; ++g) {
         b->seg_offset[g] = rec[g].off;
-        b->seg_info[g] = exc_seg_info{rec[g].slope, rec[g].icpt, rec[g].rl, rec[g].rr};
-        b->base[g] = rec[g].base;
+        if (codec == EXC_SEGLRC) {
+            b->seg_info[g] = exc_seg_info{rec[g].slope, rec[g].icpt, rec[g].rl, rec[g].rr};
+            b->base[g] = rec[g].base;
+        }
+        if (codec == EXC_PARAPFD) b->seg_median[g] = rec[g].rl;  // the .segMedian row
     }
     *out = b;
     return EXC_OK;

@@ -204,8 +204,9 @@ struct SynthArgs {
     uint64_t doc_base;
     uint64_t item0, item1;
 };
-// SegLRC encoding of segments [g0, g1) and lists [t0, t1) whose docIDs are in
-// docs[] from doc_base on (list t at list_off[t] of the concatenation).
+// Encoding of segments [g0, g1) and lists [t0, t1) whose docIDs are in docs[]
+// from doc_base on (list t at list_off[t] of the concatenation): SegLRC, LRC
+// (residuals against the list fit) or ParaPFD (d-gaps), 128- or 256-docID blocks.
 struct EncodeArgs {
     const uint32_t* docs;
     const uint64_t* list_off;  // nterms + 1
@@ -221,13 +222,21 @@ struct EncodeArgs {
     const uint64_t* goff;      // per segment (+1): global word offset
     uint32_t* words;
     TermRec* terms;
+    int32_t codec;             // EXC_SEGLRC | EXC_LRC | EXC_PARAPFD
+    float frac;                // ParaPFor exception fraction (ParaPFD)
+    uint32_t* err;             // set non-zero when a block cannot be encoded
 };
 hipError_t launch_synth(const SynthArgs& a, bool write, hipStream_t s);
 hipError_t launch_seg_fit(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_seg_pack(const EncodeArgs& a, hipStream_t s);
+// LRC / ParaPFD: list fits (LRC), then per block its plan (descriptor, words,
+// head, median), then after the offset scan its words
+hipError_t launch_lrc_lists(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_blk_fit(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_blk_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
 hipError_t scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
-int built_from_seglrc(uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
+int built_from_device(int codec, uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
                       const uint32_t* head, const uint32_t* words, uint64_t nwords, exc_built** out);
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
 // One whole-list decode work unit (kDecodeUnit docIDs of one list), with the

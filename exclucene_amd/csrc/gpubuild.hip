@@ -1,6 +1,6 @@
 // gpubuild.hip -- index construction on the GPU: seeded synthetic posting
-// lists and the SegLRC encoder, so that an index of tens of GB is generated
-// and encoded where it is served instead of crossing PCIe.
+// lists and the SegLRC, LRC and ParaPFD encoders, so that an index of tens of
+// GB is generated and encoded where it is served instead of crossing PCIe.
 //
 // The encoder follows the host builder block for block: the per-segment fit,
 // base and bit width come from exc_encode.h (the same code, the same IEEE
@@ -251,6 +251,102 @@ __global__ __launch_bounds__(256) void k_list_fit(EncodeArgs a)
     }
 }
 
+// ---------------------------------------------------------------- LRC and ParaPFD
+
+// Thread per list: the LRC list fit and the residuals' minimum folded into the
+// intercept (the host builder's code, exc_encode.h), kept in the term record
+// for the block passes.
+__global__ __launch_bounds__(256) void k_lrc_lists(EncodeArgs a)
+{
+    const uint32_t t = a.t0 + blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.t1) return;
+    const uint32_t n = (uint32_t)(a.list_off[t + 1] - a.list_off[t]);
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base);
+    exc_lrc_info f = lrc_fit(L, n);
+    f.iIntercept = (int32_t)((uint32_t)f.iIntercept + (uint32_t)lrc_min_residual(L, n, f));
+    TermRec& R = a.terms[t];
+    R.slope = f.fSlope;
+    R.icpt = f.iIntercept;
+    R.range_l = R.range_r = f.fSearchRange;
+}
+
+// The values a block of an LRC or ParaPFD list stores: residuals against the
+// list fit (GC/LRC/LRC_Generator/LRC_Generator_kernel.cu:68-88) or d-gaps, the
+// list's first docID standing for its own gap
+// (GC/ParaPFD/ParaPFD_Compression/main.cpp:163-185).
+struct BlockValues {
+    const uint32_t* L;  // the list
+    uint32_t st;        // the block's first index
+    int32_t codec;
+    float slope;
+    int32_t icpt;
+    __device__ uint32_t operator()(uint32_t i) const
+    {
+        const uint32_t p = st + i;
+        if (codec == EXC_LRC) return L[p] - lrc_pred(p, slope, icpt);
+        return p ? L[p] - L[p - 1] : L[0];
+    }
+};
+
+__device__ __forceinline__ BlockValues block_values(const EncodeArgs& a, uint32_t t, uint32_t k, uint32_t& n)
+{
+    const uint64_t nt = a.list_off[t + 1] - a.list_off[t];
+    BlockValues v;
+    v.L = a.docs + (a.list_off[t] - a.doc_base);
+    v.st = k * a.seg;
+    v.codec = a.codec;
+    v.slope = a.codec == EXC_LRC ? a.terms[t].slope : 1.0f;
+    v.icpt = a.codec == EXC_LRC ? a.terms[t].icpt : 0;
+    n = (uint32_t)min((uint64_t)a.seg, nt - v.st);
+    return v;
+}
+
+// Thread per block: descriptor and size by the host builder's ParaPFor rule
+// (exc_encode.h), head and (ParaPFD) median -- the .segMedian row, kept in
+// the record's rl field.
+__global__ __launch_bounds__(256) void k_blk_fit(EncodeArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x + a.g0;
+    if (g64 >= a.g1) return;
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    uint32_t n;
+    const BlockValues val = block_values(a, t, g - a.seg_num[t], n);
+    PforPlan p = pfor_plan(val, n, a.frac);
+    if (!p.words) {
+        atomicOr(a.err, 1u);
+        p.desc = 0;
+        p.words = 1;
+    }
+    SegRec q;
+    q.off = 0;
+    q.desc = p.desc;
+    q.slope = 1.0f;
+    q.icpt = 0.0f;
+    q.rl = a.codec == EXC_PARAPFD ? val.L[val.st + n / 2] : 0u;
+    q.rr = 0;
+    q.base = 0;
+    q.head = val.L[val.st];
+    a.rec[g] = q;
+    a.head[g] = q.head;
+    a.nwords[g] = p.words;
+}
+
+// Thread per block: its words at the block's global offset; the record's
+// offset relative to its list.
+__global__ __launch_bounds__(256) void k_blk_pack(EncodeArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x + a.g0;
+    if (g64 >= a.g1) return;
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    uint32_t n;
+    const BlockValues val = block_values(a, t, g - a.seg_num[t], n);
+    const uint64_t go = a.goff[g];
+    a.rec[g].off = (uint32_t)(go - a.goff[a.seg_num[t]]);
+    pfor_write(val, n, a.rec[g].desc, a.words + go);
+}
+
 // Thread per term: the record fields that do not depend on the docIDs.
 __global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
 {
@@ -265,6 +361,14 @@ __global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
     R.slope = 1.0f;
     R.icpt = 0;
     R.range_l = R.range_r = 4.0e9f;
+    if (a.codec == EXC_LRC) {  // the list fit of k_lrc_lists, as the upload keeps it
+        const TermRec& F = a.terms[t];
+        R.icpt = F.icpt;
+        if (F.slope > 0.0f) {
+            R.slope = F.slope;
+            R.range_l = R.range_r = F.range_l;
+        }
+    }
     R.model = kNoModel;
     R.term = t;
     R.merr = R.mx0 = R.mxs = 0.0f;
@@ -360,6 +464,24 @@ hipError_t launch_seg_pack(const EncodeArgs& a, hipStream_t s)
     if (a.g1 > a.g0)
         hipLaunchKernelGGL(k_seg_pack, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
     if (a.t1 > a.t0) hipLaunchKernelGGL(k_list_fit, dim3(a.t1 - a.t0), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_lrc_lists(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.t1 > a.t0) hipLaunchKernelGGL(k_lrc_lists, dim3((a.t1 - a.t0 + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_blk_fit(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.g1 > a.g0) hipLaunchKernelGGL(k_blk_fit, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_blk_pack(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.g1 > a.g0) hipLaunchKernelGGL(k_blk_pack, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -565,6 +565,8 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
                 if (!(q.slope > 0.0f)) { q.slope = 1.0f; q.rl = q.rr = 0x7fffffffu; }
             }
             q.head = hs ? ht[s] : h->seg_head[g];
+            // ParaPFD keeps the block's .segMedian row in rl (no fit ranges)
+            if (codec == EXC_PARAPFD && h->seg_median) q.rl = h->seg_median[g];
         }
     }
 
@@ -1716,14 +1718,20 @@ int synth_write(exc_ctx* c, const SynthPlan& P, SynthArgs a, uint32_t t0, uint32
     return EXC_OK;
 }
 
-// SegLRC encoding on the device.  docs_for(t0, t1, &ptr) makes the docIDs of
-// lists [t0, t1) available in device memory (ptr[0] = list t0's first); it is
-// called twice per batch (fit, then pack).
+// SegLRC, LRC or ParaPFD encoding on the device, block for block the host
+// builder's bytes.  docs_for(t0, t1, &ptr) makes the docIDs of lists [t0, t1)
+// available in device memory (ptr[0] = list t0's first); it is called twice
+// per batch (fits and block sizes, then packing).
 template <class DocsFor>
-int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::vector<uint64_t>& list_off,
-                        const std::vector<std::pair<uint32_t, uint32_t>>& batches, DocsFor docs_for, exc_index** out)
+int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint32_t nterms,
+                          const std::vector<uint64_t>& list_off,
+                          const std::vector<std::pair<uint32_t, uint32_t>>& batches, DocsFor docs_for, exc_index** out)
 {
     if (seg != 128 && seg != 256) return set_error(EXC_ERR_ARG, "segment size must be 128 or 256");
+    if (codec != EXC_SEGLRC && codec != EXC_LRC && codec != EXC_PARAPFD)
+        return set_error(EXC_ERR_ARG, "the device builder encodes SegLRC, LRC and ParaPFD");
+    if (codec != EXC_PARAPFD) frac = 0.0f;  // the LRC-family probes read blocks without exceptions
+    if (!(frac >= 0.0f && frac <= 1.0f)) return set_error(EXC_ERR_ARG, "FRAC must be in [0, 1]");
     std::vector<uint32_t> seg_num(nterms + 1, 0);
     uint64_t S = 0;
     for (uint32_t t = 0; t < nterms; ++t) {
@@ -1735,7 +1743,7 @@ int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::ve
     TempBufs tmp(c);
     auto* ix = new exc_index();
     ix->ctx = c;
-    ix->codec = EXC_SEGLRC;
+    ix->codec = codec;
     ix->seg = seg;
     ix->nterms = nterms;
     ix->nseg = S;
@@ -1747,14 +1755,16 @@ int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::ve
     uint64_t* d_loff;
     uint32_t *d_snum, *d_nw;
     uint64_t* d_goff;
+    uint32_t* d_err;
     void *p_rec, *p_head, *p_terms;
     if ((rc = tmp.get(nterms + 1, &d_loff)) || (rc = tmp.get(nterms + 1, &d_snum)) || (rc = tmp.get(S + 1, &d_nw)) ||
-        (rc = tmp.get(S + 1, &d_goff)) || (rc = alloc_padded(ix, S * sizeof(SegRec), &p_rec)) ||
+        (rc = tmp.get(S + 1, &d_goff)) || (rc = tmp.get(1, &d_err)) || (rc = alloc_padded(ix, S * sizeof(SegRec), &p_rec)) ||
         (rc = alloc_padded(ix, S * 4, &p_head)) || (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)))
         return fail(rc);
     hipError_t e = hipMemcpyAsync(d_loff, list_off.data(), (nterms + 1ull) * 8, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_snum, seg_num.data(), (nterms + 1ull) * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d_nw + S, 0, 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_err, 0, 4, c->stream);
     if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
     EncodeArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1767,6 +1777,9 @@ int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::ve
     a.nwords = d_nw;
     a.goff = d_goff;
     a.terms = static_cast<TermRec*>(p_terms);
+    a.codec = codec;
+    a.frac = frac;
+    a.err = d_err;
     // pass 1: fits and block sizes
     for (auto [t0, t1] : batches) {
         const uint32_t* docs;
@@ -1775,14 +1788,25 @@ int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::ve
         a.doc_base = list_off[t0];
         a.g0 = seg_num[t0];
         a.g1 = seg_num[t1];
-        if ((e = launch_seg_fit(a, c->stream)) != hipSuccess) break;
+        a.t0 = t0;
+        a.t1 = t1;
+        if (codec == EXC_SEGLRC) {
+            e = launch_seg_fit(a, c->stream);
+        } else {
+            e = codec == EXC_LRC ? launch_lrc_lists(a, c->stream) : hipSuccess;
+            if (e == hipSuccess) e = launch_blk_fit(a, c->stream);
+        }
+        if (e != hipSuccess) break;
     }
     // block offsets, words, term records
     uint64_t W = 0;
+    uint32_t berr = 0;
     if (e == hipSuccess) e = scan_u32_to_u64(d_nw, d_goff, S + 1, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(&W, d_goff + S, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&berr, d_err, 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    if (berr) return fail(set_error(EXC_ERR_FORMAT, "device build: a block cannot be encoded (docIDs must be < 2^31)"));
     void* p_words;
     if ((rc = alloc_padded(ix, W * 4, &p_words))) return fail(rc);
     a.words = static_cast<uint32_t*>(p_words);
@@ -1798,7 +1822,8 @@ int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::ve
         a.g1 = seg_num[t1];
         a.t0 = t0;
         a.t1 = t1;
-        if ((e = launch_seg_pack(a, c->stream)) != hipSuccess) break;
+        if ((e = (codec == EXC_SEGLRC ? launch_seg_pack(a, c->stream) : launch_blk_pack(a, c->stream))) != hipSuccess)
+            break;
     }
     ix->recs.resize(nterms);
     if (e == hipSuccess)
@@ -1808,7 +1833,7 @@ int build_seglrc_device(exc_ctx* c, uint32_t seg, uint32_t nterms, const std::ve
     ix->nwords = W;
     ix->n.resize(nterms);
     for (uint32_t t = 0; t < nterms; ++t) ix->n[t] = ix->recs[t].n;
-    ix->dix.codec = EXC_SEGLRC;
+    ix->dix.codec = codec;
     ix->dix.seg = seg;
     ix->dix.words = a.words;
     ix->dix.terms = a.terms;
@@ -1840,32 +1865,30 @@ extern "C" int exc_synth_lists_device(exc_ctx* c, uint64_t seed, uint32_t univer
     return EXC_OK;
 }
 
-extern "C" int exc_build_index_device(exc_ctx* c, int codec, uint32_t seg, const uint32_t* d_docs,
+extern "C" int exc_build_index_device(exc_ctx* c, int codec, uint32_t seg, float frac, const uint32_t* d_docs,
                                       const uint64_t* list_off, uint32_t nterms, exc_index** out)
 {
     if (!c || !list_off || !out || (list_off[nterms] && !d_docs))
         return set_error(EXC_ERR_ARG, "exc_build_index_device: bad argument");
-    if (codec != EXC_SEGLRC) return set_error(EXC_ERR_ARG, "exc_build_index_device: SegLRC only");
     for (uint32_t t = 0; t < nterms; ++t)
         if (list_off[t + 1] < list_off[t] || list_off[t + 1] - list_off[t] > 0xffffffffull)
             return set_error(EXC_ERR_ARG, "exc_build_index_device: bad list offsets");
     HIP_TRY(hipSetDevice(c->device));
     std::vector<uint64_t> lo(list_off, list_off + nterms + 1);
     std::vector<std::pair<uint32_t, uint32_t>> batches{{0u, nterms}};
-    return build_seglrc_device(c, seg, nterms, lo, batches,
-                               [&](uint32_t t0, uint32_t, const uint32_t** p) {
-                                   *p = d_docs + lo[t0];
-                                   return EXC_OK;
-                               },
-                               out);
+    return build_index_on_device(c, codec, seg, frac, nterms, lo, batches,
+                                 [&](uint32_t t0, uint32_t, const uint32_t** p) {
+                                     *p = d_docs + lo[t0];
+                                     return EXC_OK;
+                                 },
+                                 out);
 }
 
-extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, uint64_t seed, uint32_t universe,
-                                      const uint32_t* df, uint32_t nterms, uint64_t batch_postings,
+extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint64_t seed,
+                                      uint32_t universe, const uint32_t* df, uint32_t nterms, uint64_t batch_postings,
                                       uint64_t* list_off_out, exc_index** out)
 {
     if (!c || (nterms && !df) || !out) return set_error(EXC_ERR_ARG, "exc_synth_index_device: bad argument");
-    if (codec != EXC_SEGLRC) return set_error(EXC_ERR_ARG, "exc_synth_index_device: SegLRC only");
     HIP_TRY(hipSetDevice(c->device));
     TempBufs tmp(c);
     SynthPlan P;
@@ -1884,18 +1907,19 @@ extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, uint6
     }
     uint32_t* d_docs;
     if (int rc = tmp.get(maxb, &d_docs)) return rc;
-    return build_seglrc_device(c, seg, nterms, P.list_off, batches,
-                               [&](uint32_t t0, uint32_t t1, const uint32_t** p) {
-                                   *p = d_docs;
-                                   return synth_write(c, P, a, t0, t1, d_docs);
-                               },
-                               out);
+    return build_index_on_device(c, codec, seg, frac, nterms, P.list_off, batches,
+                                 [&](uint32_t t0, uint32_t t1, const uint32_t** p) {
+                                     *p = d_docs;
+                                     return synth_write(c, P, a, t0, t1, d_docs);
+                                 },
+                                 out);
 }
 
 extern "C" int exc_index_export(exc_ctx* c, exc_index* ix, exc_built** out)
 {
     if (!c || !ix || !out) return set_error(EXC_ERR_ARG, "exc_index_export: bad argument");
-    if (ix->codec != EXC_SEGLRC) return set_error(EXC_ERR_ARG, "exc_index_export: SegLRC only");
+    if (ix->codec != EXC_SEGLRC && ix->codec != EXC_LRC && ix->codec != EXC_PARAPFD)
+        return set_error(EXC_ERR_ARG, "exc_index_export: SegLRC, LRC and ParaPFD indexes");
     HIP_TRY(hipSetDevice(c->device));
     std::vector<SegRec> rec(ix->nseg);
     std::vector<uint32_t> head(ix->nseg), words(ix->nwords);
@@ -1905,7 +1929,7 @@ extern "C" int exc_index_export(exc_ctx* c, exc_index* ix, exc_built** out)
         HIP_TRY(hipMemcpy(head.data(), ix->dix.seg_head, ix->nseg * 4, hipMemcpyDeviceToHost));
     }
     if (ix->nwords) HIP_TRY(hipMemcpy(words.data(), ix->dix.words, ix->nwords * 4, hipMemcpyDeviceToHost));
-    return built_from_seglrc(ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(), head.data(), words.data(),
+    return built_from_device(ix->codec, ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(), head.data(), words.data(),
                              ix->nwords, out);
 }
 

@@ -104,3 +104,54 @@ def test_batched_synth_index_equals_single_pass(ctx):
             many.close()
     finally:
         ctx.device_free(d)
+
+
+@pytest.mark.parametrize("codec,seg,frac", [(ea.LRC, 256, 0.0), (ea.LRC, 128, 0.0), (ea.PARAPFD, 128, 0.1),
+                                            (ea.PARAPFD, 256, 0.1), (ea.PARAPFD, 128, 0.5), (ea.PARAPFD, 128, 0.0)])
+def test_device_lrc_parapfd_encoders_match_host_builder(ctx, codec, seg, frac):
+    """The LRC and ParaPFD encoders on the device: every array the reference's
+    files hold (.ind1, .ind2, .segNum, .segOffset, .segHead, and LRC's
+    .regression_info or ParaPFD's .segMedian) byte-identical to the host
+    builder's, the index answering queries as the oracle does, and the
+    batched synthesize-and-encode path equal to the one-pass build."""
+    d, off, docs = _synth(ctx, 51)
+    try:
+        host = ea.build_index(docs, off, codec, seg=seg, frac=frac)
+        dix = ctx.build_index_device(d, off, seg=seg, codec=codec, frac=frac)
+        try:
+            dev = ctx.export(dix)
+            names = ["ind1", "ind2", "seg_num", "seg_offset", "seg_head"]
+            names += ["lrc_info"] if codec == ea.LRC else ["seg_median"]
+            for name in names:
+                assert getattr(dev, name).tobytes() == getattr(host, name).tobytes(), name
+            lists = [docs[int(off[t]):int(off[t + 1])] for t in range(len(DF))]
+            rng = np.random.default_rng(5)
+            qs = [[int(x) for x in rng.choice(len(DF), int(rng.integers(1, 5)), replace=False)] for _ in range(200)]
+            ocum, ores = oracle.eval_queries(oracle.build_index(lists, codec, seg=seg, frac=frac), qs)
+            cum, res = ctx.intersect(dix, qs)
+            assert np.array_equal(cum, ocum) and np.array_equal(res, ores)
+            out = ctx.decompress(dix, list(range(len(DF))))
+            assert np.array_equal(out, docs)
+        finally:
+            dix.close()
+        many, off2 = ctx.synth_index_device(51, U, DF, seg=seg, batch_postings=1 << 20, codec=codec, frac=frac)
+        try:
+            assert np.array_equal(off, off2)
+            b = ctx.export(many)
+            for name in names:
+                assert getattr(b, name).tobytes() == getattr(host, name).tobytes(), name
+        finally:
+            many.close()
+    finally:
+        ctx.device_free(d)
+
+
+def test_device_encoder_refuses_other_codecs(ctx):
+    d, off, docs = _synth(ctx, 61, df=np.array([10, 100], np.uint32))
+    try:
+        for codec in (ea.HS_SEGLRC, ea.NEWPFD, ea.RAW):
+            with pytest.raises(ea.ExcError) as e:
+                ctx.build_index_device(d, off, seg=128, codec=codec)
+            assert e.value.code == -1
+    finally:
+        ctx.device_free(d)

@@ -199,19 +199,22 @@ def test_c2_full_size_decode(ctx, c2, codec, seg, frac):
     assert np.array_equal(out, docs)
 
 
-def test_c2_full_size_device_builder(ctx, c2):
-    """The device encoder over the full-size lists: the index's segment-level
-    arrays byte-identical to the host builder's, and the C2 batch answered on
-    the device-built index exactly as the oracle answers it."""
+@pytest.mark.parametrize("codec,seg,frac", [(ea.SEGLRC, 128, 0.0), (ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1)])
+def test_c2_full_size_device_builder(ctx, c2, codec, seg, frac):
+    """The device encoders over the full-size lists: the index's arrays in the
+    reference layout byte-identical to the host builder's, and the C2 batch
+    answered on the device-built index exactly as the oracle answers it."""
     docs, off, queries, built, ocum, ores, _ = c2
+    host = built if codec == ea.SEGLRC else ea.build_index(docs, off, codec, seg=seg, frac=frac)
+    names = {ea.SEGLRC: ("seg_info", "seg_base"), ea.LRC: ("lrc_info",), ea.PARAPFD: ("seg_median",)}[codec]
     d = ctx.device_alloc(len(docs) * 4)
     try:
         ctx.to_device(d, docs)
-        dix = ctx.build_index_device(d, off, seg=128)
+        dix = ctx.build_index_device(d, off, seg=seg, codec=codec, frac=frac)
         try:
             dev = ctx.export(dix)
-            for name in ("ind1", "ind2", "seg_num", "seg_offset", "seg_head", "seg_info", "seg_base"):
-                assert np.array_equal(getattr(dev, name), getattr(built, name)), name
+            for name in ("ind1", "ind2", "seg_num", "seg_offset", "seg_head") + names:
+                assert getattr(dev, name).tobytes() == getattr(host, name).tobytes(), name
             cum, res = ctx.intersect(dix, queries)
             assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
         finally:
