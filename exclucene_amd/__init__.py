@@ -153,10 +153,10 @@ def lib():
         L.exc_index_model_info.argtypes = [_vp, _vp, _u32, ctypes.POINTER(ModelHost)]
         L.exc_mlp_gradient.argtypes = [_vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float)]
         L.exc_synth_lists_device.argtypes = [_vp, _u64, _u32, _vp, _u32, _vp, _vp, _u64]
-        L.exc_build_index_device.argtypes = [_vp, ctypes.c_int, _u32, ctypes.c_float, _vp, _vp, _u32,
+        L.exc_build_index_device.argtypes = [_vp, ctypes.c_int, _u32, ctypes.c_float, _u32, _u32, _vp, _vp, _u32,
                                              ctypes.POINTER(_vp)]
-        L.exc_synth_index_device.argtypes = [_vp, ctypes.c_int, _u32, ctypes.c_float, _u64, _u32, _vp, _u32, _u64,
-                                             _vp, ctypes.POINTER(_vp)]
+        L.exc_synth_index_device.argtypes = [_vp, ctypes.c_int, _u32, ctypes.c_float, _u32, _u32, _u64, _u32, _vp,
+                                             _u32, _u64, _vp, ctypes.POINTER(_vp)]
         L.exc_index_export.argtypes = [_vp, _vp, ctypes.POINTER(_vp)]
         L.exc_device_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
         L.exc_device_free.argtypes = [_vp, _vp]
@@ -486,22 +486,23 @@ class Context:
                                             ctypes.c_void_p(d_out) if d_out else None, cap))
         return off
 
-    def build_index_device(self, d_docs, list_off, seg=128, codec=SEGLRC, frac=0.0):
-        """Encode sorted lists already in device memory (SegLRC, LRC or ParaPFD)."""
+    def build_index_device(self, d_docs, list_off, seg=128, codec=SEGLRC, frac=0.0, hs_target=256, hs_shift=0):
+        """Encode sorted lists already in device memory (SegLRC, LRC, ParaPFD or HS_SegLRC)."""
         off = np.ascontiguousarray(list_off, np.uint64)
         h = _vp()
-        _check(lib().exc_build_index_device(self._h, codec, seg, frac, ctypes.c_void_p(d_docs), _p(off),
-                                            len(off) - 1, ctypes.byref(h)))
+        _check(lib().exc_build_index_device(self._h, codec, seg, frac, hs_target, hs_shift, ctypes.c_void_p(d_docs),
+                                            _p(off), len(off) - 1, ctypes.byref(h)))
         return DeviceIndex(self, h, lengths=np.diff(off).astype(np.uint32))
 
-    def synth_index_device(self, seed, universe, df, seg=128, batch_postings=1 << 30, codec=SEGLRC, frac=0.0):
-        """Synthesize and encode an index (SegLRC, LRC or ParaPFD) on the device
-        in batches; returns (DeviceIndex, list offsets)."""
+    def synth_index_device(self, seed, universe, df, seg=128, batch_postings=1 << 30, codec=SEGLRC, frac=0.0,
+                           hs_target=256, hs_shift=0):
+        """Synthesize and encode an index (SegLRC, LRC, ParaPFD or HS_SegLRC)
+        on the device in batches; returns (DeviceIndex, list offsets)."""
         df = np.ascontiguousarray(df, np.uint32)
         off = np.zeros(len(df) + 1, np.uint64)
         h = _vp()
-        _check(lib().exc_synth_index_device(self._h, codec, seg, frac, seed, universe, _p(df), len(df),
-                                            batch_postings, _p(off), ctypes.byref(h)))
+        _check(lib().exc_synth_index_device(self._h, codec, seg, frac, hs_target, hs_shift, seed, universe, _p(df),
+                                            len(df), batch_postings, _p(off), ctypes.byref(h)))
         return DeviceIndex(self, h, lengths=np.diff(off).astype(np.uint32)), off
 
     def export(self, index):

@@ -318,8 +318,9 @@ void parallel_for(uint32_t n, int nthreads, F f)
 // (exc_index_export of an index built on the GPU).
 int exc::built_from_device(int codec, uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg,
                            const SegRec* rec, const uint32_t* head, const uint32_t* words, uint64_t nwords,
-                           exc_built** out)
+                           const uint32_t* hash, uint64_t hash_words, exc_built** out)
 {
+    const bool hs = codec == EXC_HS_SEGLRC;
     auto* b = new exc_built();
     b->codec = codec;
     b->seg = seg;
@@ -344,15 +345,20 @@ int exc::built_from_device(int codec, uint32_t seg, uint32_t nterms, const TermR
     }
     b->ind2.assign(words, words + nwords);
     b->seg_offset.resize(nseg);
-    b->seg_head.assign(head, head + nseg);
-    if (codec == EXC_SEGLRC) {
+    if (!hs) b->seg_head.assign(head, head + nseg);
+    if (codec == EXC_SEGLRC || hs) {
         b->seg_info.resize(nseg);
         b->base.resize(nseg);
+    }
+    if (hs) {  // the tables, and each list's table offset (its record points past the header)
+        b->hash_info.assign(hash, hash + hash_words);
+        b->hash_offset.resize(nterms);
+        for (uint32_t t = 0; t < nterms; ++t) b->hash_offset[t] = terms[t].aux - 2;
     }
     if (codec == EXC_PARAPFD) b->seg_median.resize(nseg);
     for (uint64_t g = 0; g < nseg; ++g) {
         b->seg_offset[g] = rec[g].off;
-        if (codec == EXC_SEGLRC) {
+        if (codec == EXC_SEGLRC || hs) {
             b->seg_info[g] = exc_seg_info{rec[g].slope, rec[g].icpt, rec[g].rl, rec[g].rr};
             b->base[g] = rec[g].base;
         }

@@ -222,9 +222,16 @@ struct EncodeArgs {
     const uint64_t* goff;      // per segment (+1): global word offset
     uint32_t* words;
     TermRec* terms;
-    int32_t codec;             // EXC_SEGLRC | EXC_LRC | EXC_PARAPFD
+    int32_t codec;             // EXC_SEGLRC | EXC_LRC | EXC_PARAPFD | EXC_HS_SEGLRC
     float frac;                // ParaPFor exception fraction (ParaPFD)
     uint32_t* err;             // set non-zero when a block cannot be encoded
+    // HS_SegLRC: per list its shift and bucket count (first pass), the word
+    // offset of its [shift][nb][start_0 .. start_nb] table, and the tables
+    uint32_t* hs_shift;
+    uint32_t* hs_nb;
+    const uint64_t* hs_off;
+    uint32_t* hash;
+    uint32_t hs_target, hs_fixed;
 };
 hipError_t launch_synth(const SynthArgs& a, bool write, hipStream_t s);
 hipError_t launch_seg_fit(const EncodeArgs& a, hipStream_t s);
@@ -234,10 +241,16 @@ hipError_t launch_seg_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_lrc_lists(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_blk_fit(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_blk_pack(const EncodeArgs& a, hipStream_t s);
+// HS_SegLRC: per list its shift and bucket count, then per bucket its table
+// entry, fit, base and size, then after the offset scan its words
+hipError_t launch_hs_lists(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_hs_fit(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_hs_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
 hipError_t scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 int built_from_device(int codec, uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
-                      const uint32_t* head, const uint32_t* words, uint64_t nwords, exc_built** out);
+                      const uint32_t* head, const uint32_t* words, uint64_t nwords, const uint32_t* hash,
+                      uint64_t hash_words, exc_built** out);
 hipError_t launch_scan(const ScanArgs& a, hipStream_t s);
 // One whole-list decode work unit (kDecodeUnit docIDs of one list), with the
 // list fields the decoder needs so that its first load brings all of them.

@@ -347,6 +347,111 @@ __global__ __launch_bounds__(256) void k_blk_pack(EncodeArgs a)
     pfor_write(val, n, a.rec[g].desc, a.words + go);
 }
 
+// ---------------------------------------------------------------- HS_SegLRC
+
+// Thread per list: the bucket shift and count of its hash table, by the host
+// builder's rule (GC/HS_SegLRC/HS_SegLRC_Generator/HS_SegLRC_Generator.cu:503-594):
+// a fixed shift, or bits(max) - bits(ceil(n / target)), and one bucket
+// (shift = bits(max)) for lists of at most `target` docIDs.
+__global__ __launch_bounds__(256) void k_hs_lists(EncodeArgs a)
+{
+    const uint32_t t = a.t0 + blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.t1) return;
+    const uint32_t n = (uint32_t)(a.list_off[t + 1] - a.list_off[t]);
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base);
+    const uint32_t maxb = n ? bits_of(L[n - 1]) : 0;
+    uint32_t shift = a.hs_fixed;
+    if (!shift) shift = n <= a.hs_target ? maxb : maxb - bits_of((n + a.hs_target - 1) / a.hs_target);
+    a.hs_shift[t] = shift;
+    a.hs_nb[t] = n ? (L[n - 1] >> shift) + 1 : 1;
+}
+
+// first index of L[0 .. n) whose docID is >= v
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* L, uint32_t n, uint64_t v)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if ((uint64_t)L[m] < v) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+
+// bucket g of list t: [start, end) of the list
+__device__ __forceinline__ void hs_bucket(const EncodeArgs& a, uint32_t t, uint32_t b, const uint32_t* L, uint32_t n,
+                                          uint32_t& start, uint32_t& end)
+{
+    const uint32_t shift = a.hs_shift[t], nb = a.hs_nb[t];
+    start = b ? lower_bound_u32(L, n, (uint64_t)b << shift) : 0u;
+    end = b + 1 < nb ? lower_bound_u32(L, n, (uint64_t)(b + 1) << shift) : n;
+}
+
+// Thread per bucket: its table entry (bucket 0 writes the header, the last
+// bucket the closing n), the SegLRC fit and base of its docIDs (exc_encode.h,
+// the host builder's code) and its block size; an empty bucket has no block.
+__global__ __launch_bounds__(256) void k_hs_fit(EncodeArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x + a.g0;
+    if (g64 >= a.g1) return;
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    const uint32_t b = g - a.seg_num[t];
+    const uint32_t n = (uint32_t)(a.list_off[t + 1] - a.list_off[t]);
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base);
+    uint32_t start, end;
+    hs_bucket(a, t, b, L, n, start, end);
+    uint32_t* tab = a.hash + a.hs_off[t];
+    if (b == 0) {
+        tab[0] = a.hs_shift[t];
+        tab[1] = a.hs_nb[t];
+    }
+    tab[2 + b] = start;
+    if (b + 1 == a.hs_nb[t]) tab[3 + b] = n;
+    const uint32_t len = end - start;
+    const exc_seg_info f = seg_fit(L + start, len);
+    int32_t base;
+    uint32_t lb;
+    seg_base_bits(L + start, len, f, base, lb);
+    SegRec q;
+    q.off = 0;
+    q.desc = lb << 26;
+    q.slope = f.fSlope;
+    q.icpt = f.fIntercept;
+    q.rl = f.nRangeLeft;
+    q.rr = f.nRangeRight;
+    q.base = base;
+    q.head = start;
+    a.rec[g] = q;
+    a.nwords[g] = len ? 1 + words_for(len, lb) : 0u;
+}
+
+// Thread per bucket: descriptor and packed residuals (none for an empty bucket,
+// whose offset is the next block's, as in the reference's files).
+__global__ __launch_bounds__(256) void k_hs_pack(EncodeArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x + a.g0;
+    if (g64 >= a.g1) return;
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    const uint32_t b = g - a.seg_num[t];
+    const uint32_t n = (uint32_t)(a.list_off[t + 1] - a.list_off[t]);
+    const uint32_t* L = a.docs + (a.list_off[t] - a.doc_base);
+    const SegRec q = a.rec[g];
+    const uint64_t go = a.goff[g];
+    a.rec[g].off = (uint32_t)(go - a.goff[a.seg_num[t]]);
+    const uint32_t* tab = a.hash + a.hs_off[t];
+    const uint32_t start = tab[2 + b], len = tab[3 + b] - start;
+    (void)n;
+    if (!len) return;
+    const exc_seg_info f{q.slope, q.icpt, 0, 0};
+    const uint32_t lb = desc_lb(q.desc);
+    uint32_t* o = a.words + go;
+    o[0] = q.desc;
+    BitWriter w(o + 1);
+    for (uint32_t j = 0; j < len; ++j) w.put((uint32_t)(seg_residual(L + start, j, f) - q.base), lb);
+    w.flush();
+}
+
 // Thread per term: the record fields that do not depend on the docIDs.
 __global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
 {
@@ -361,6 +466,10 @@ __global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
     R.slope = 1.0f;
     R.icpt = 0;
     R.range_l = R.range_r = 4.0e9f;
+    if (a.codec == EXC_HS_SEGLRC) {  // buckets: count, table position, shift
+        R.aux = (uint32_t)(a.hs_off[t] + 2);
+        R.icpt = (int32_t)a.hs_shift[t];
+    }
     if (a.codec == EXC_LRC) {  // the list fit of k_lrc_lists, as the upload keeps it
         const TermRec& F = a.terms[t];
         R.icpt = F.icpt;
@@ -482,6 +591,24 @@ hipError_t launch_blk_fit(const EncodeArgs& a, hipStream_t s)
 hipError_t launch_blk_pack(const EncodeArgs& a, hipStream_t s)
 {
     if (a.g1 > a.g0) hipLaunchKernelGGL(k_blk_pack, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hs_lists(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.t1 > a.t0) hipLaunchKernelGGL(k_hs_lists, dim3((a.t1 - a.t0 + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hs_fit(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.g1 > a.g0) hipLaunchKernelGGL(k_hs_fit, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hs_pack(const EncodeArgs& a, hipStream_t s)
+{
+    if (a.g1 > a.g0) hipLaunchKernelGGL(k_hs_pack, dim3((uint32_t)((a.g1 - a.g0 + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -118,6 +118,7 @@ struct exc_index {
     uint32_t nterms = 0;
     uint64_t nseg = 0;
     uint64_t nwords = 0;           // compressed words (.ind2)
+    uint64_t hash_words = 0;       // .HashSeg_info words (HS codecs)
     std::vector<uint32_t> n;       // list lengths (host copy for planning)
     std::vector<TermRec> recs;     // host copy of the device term records
     TermRec* d_terms = nullptr;
@@ -577,6 +578,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->nterms = nt;
     ix->nseg = h->nseg;
     ix->nwords = W;
+    ix->hash_words = (hs || raw_hs) ? h->hash_words : 0;
     ix->n.resize(nt);
     for (uint32_t t = 0; t < nt; ++t) ix->n[t] = recs[t].n;
     HIP_TRY(hipSetDevice(c->device));
@@ -1844,6 +1846,136 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
     return EXC_OK;
 }
 
+// HS_SegLRC encoding on the device, bucket for bucket the host builder's bytes
+// (hash tables, fits, bases, packed residuals): the lists' shifts and bucket
+// counts first (they depend on each list's maximum), then the buckets, then
+// after the word-offset scan the packing.  docs_for as for the other codecs,
+// called three times per batch.
+template <class DocsFor>
+int build_hs_on_device(exc_ctx* c, uint32_t hs_target, uint32_t hs_shift, uint32_t nterms,
+                       const std::vector<uint64_t>& list_off, const std::vector<std::pair<uint32_t, uint32_t>>& batches,
+                       DocsFor docs_for, exc_index** out)
+{
+    if (hs_target == 0) hs_target = 256;
+    if (hs_shift > 31) return set_error(EXC_ERR_ARG, "hash shift %u above 31", hs_shift);
+    TempBufs tmp(c);
+    uint64_t* d_loff;
+    uint32_t *d_shift, *d_nb;
+    int rc;
+    if ((rc = tmp.get(nterms + 1, &d_loff)) || (rc = tmp.get(std::max<uint32_t>(nterms, 1), &d_shift)) ||
+        (rc = tmp.get(std::max<uint32_t>(nterms, 1), &d_nb)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(d_loff, list_off.data(), (nterms + 1ull) * 8, hipMemcpyHostToDevice, c->stream));
+    EncodeArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.list_off = d_loff;
+    a.nterms = nterms;
+    a.seg = 128;
+    a.codec = EXC_HS_SEGLRC;
+    a.hs_shift = d_shift;
+    a.hs_nb = d_nb;
+    a.hs_target = hs_target;
+    a.hs_fixed = hs_shift;
+    // pass 1: shifts and bucket counts
+    for (auto [t0, t1] : batches) {
+        const uint32_t* docs;
+        if ((rc = docs_for(t0, t1, &docs))) return rc;
+        a.docs = docs;
+        a.doc_base = list_off[t0];
+        a.t0 = t0;
+        a.t1 = t1;
+        HIP_TRY(launch_hs_lists(a, c->stream));
+    }
+    std::vector<uint32_t> nb(nterms);
+    if (nterms) HIP_TRY(hipMemcpyAsync(nb.data(), d_nb, nterms * 4ull, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> seg_num(nterms + 1, 0);
+    std::vector<uint64_t> hoff(nterms + 1, 0);
+    for (uint32_t t = 0; t < nterms; ++t) {
+        seg_num[t + 1] = seg_num[t] + nb[t];
+        hoff[t + 1] = hoff[t] + nb[t] + 3ull;
+        if ((uint64_t)seg_num[t] + nb[t] > 0xfffffffeull || hoff[t + 1] > 0xffffffffull)
+            return set_error(EXC_ERR_CAPACITY, "more than 2^32 buckets or table words");
+    }
+    const uint64_t S = seg_num[nterms], H = hoff[nterms];
+    auto* ix = new exc_index();
+    ix->ctx = c;
+    ix->codec = EXC_HS_SEGLRC;
+    ix->seg = 128;
+    ix->nterms = nterms;
+    ix->nseg = S;
+    auto fail = [&](int r) {
+        exc_index_free(ix);
+        return r;
+    };
+    uint32_t *d_snum, *d_nw;
+    uint64_t *d_goff, *d_hoff;
+    void *p_rec, *p_hash, *p_terms;
+    if ((rc = tmp.get(nterms + 1, &d_snum)) || (rc = tmp.get(S + 1, &d_nw)) || (rc = tmp.get(S + 1, &d_goff)) ||
+        (rc = tmp.get(nterms + 1, &d_hoff)) || (rc = alloc_padded(ix, S * sizeof(SegRec), &p_rec)) ||
+        (rc = alloc_padded(ix, H * 4, &p_hash)) || (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)))
+        return fail(rc);
+    hipError_t e = hipMemcpyAsync(d_snum, seg_num.data(), (nterms + 1ull) * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_hoff, hoff.data(), (nterms + 1ull) * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_nw + S, 0, 4, c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    a.seg_num = d_snum;
+    a.rec = static_cast<SegRec*>(p_rec);
+    a.nwords = d_nw;
+    a.goff = d_goff;
+    a.terms = static_cast<TermRec*>(p_terms);
+    a.hs_off = d_hoff;
+    a.hash = static_cast<uint32_t*>(p_hash);
+    // pass 2: tables, fits, bases, block sizes
+    for (auto [t0, t1] : batches) {
+        const uint32_t* docs;
+        if ((rc = docs_for(t0, t1, &docs))) return fail(rc);
+        a.docs = docs;
+        a.doc_base = list_off[t0];
+        a.g0 = seg_num[t0];
+        a.g1 = seg_num[t1];
+        if ((e = launch_hs_fit(a, c->stream)) != hipSuccess) break;
+    }
+    uint64_t W = 0;
+    if (e == hipSuccess) e = scan_u32_to_u64(d_nw, d_goff, S + 1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&W, d_goff + S, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    void* p_words;
+    if ((rc = alloc_padded(ix, W * 4, &p_words))) return fail(rc);
+    a.words = static_cast<uint32_t*>(p_words);
+    if ((e = launch_term_recs(a, c->stream)) != hipSuccess)
+        return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    // pass 3: packed residuals
+    for (auto [t0, t1] : batches) {
+        const uint32_t* docs;
+        if ((rc = docs_for(t0, t1, &docs))) return fail(rc);
+        a.docs = docs;
+        a.doc_base = list_off[t0];
+        a.g0 = seg_num[t0];
+        a.g1 = seg_num[t1];
+        if ((e = launch_hs_pack(a, c->stream)) != hipSuccess) break;
+    }
+    ix->recs.resize(nterms);
+    if (e == hipSuccess && nterms)
+        e = hipMemcpyAsync(ix->recs.data(), p_terms, (size_t)nterms * sizeof(TermRec), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    ix->nwords = W;
+    ix->hash_words = H;
+    ix->n.resize(nterms);
+    for (uint32_t t = 0; t < nterms; ++t) ix->n[t] = ix->recs[t].n;
+    ix->dix.codec = EXC_HS_SEGLRC;
+    ix->dix.seg = 128;
+    ix->dix.words = a.words;
+    ix->dix.terms = a.terms;
+    ix->d_terms = a.terms;
+    ix->dix.rec = a.rec;
+    ix->dix.hash = a.hash;
+    *out = ix;
+    return EXC_OK;
+}
+
 }  // namespace
 
 extern "C" int exc_synth_lists_device(exc_ctx* c, uint64_t seed, uint32_t universe, const uint32_t* df, uint32_t nterms,
@@ -1865,8 +1997,9 @@ extern "C" int exc_synth_lists_device(exc_ctx* c, uint64_t seed, uint32_t univer
     return EXC_OK;
 }
 
-extern "C" int exc_build_index_device(exc_ctx* c, int codec, uint32_t seg, float frac, const uint32_t* d_docs,
-                                      const uint64_t* list_off, uint32_t nterms, exc_index** out)
+extern "C" int exc_build_index_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint32_t hs_target,
+                                      uint32_t hs_shift, const uint32_t* d_docs, const uint64_t* list_off,
+                                      uint32_t nterms, exc_index** out)
 {
     if (!c || !list_off || !out || (list_off[nterms] && !d_docs))
         return set_error(EXC_ERR_ARG, "exc_build_index_device: bad argument");
@@ -1876,17 +2009,18 @@ extern "C" int exc_build_index_device(exc_ctx* c, int codec, uint32_t seg, float
     HIP_TRY(hipSetDevice(c->device));
     std::vector<uint64_t> lo(list_off, list_off + nterms + 1);
     std::vector<std::pair<uint32_t, uint32_t>> batches{{0u, nterms}};
-    return build_index_on_device(c, codec, seg, frac, nterms, lo, batches,
-                                 [&](uint32_t t0, uint32_t, const uint32_t** p) {
-                                     *p = d_docs + lo[t0];
-                                     return EXC_OK;
-                                 },
-                                 out);
+    const auto docs_for = [&](uint32_t t0, uint32_t, const uint32_t** p) {
+        *p = d_docs + lo[t0];
+        return EXC_OK;
+    };
+    if (codec == EXC_HS_SEGLRC) return build_hs_on_device(c, hs_target, hs_shift, nterms, lo, batches, docs_for, out);
+    return build_index_on_device(c, codec, seg, frac, nterms, lo, batches, docs_for, out);
 }
 
-extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint64_t seed,
-                                      uint32_t universe, const uint32_t* df, uint32_t nterms, uint64_t batch_postings,
-                                      uint64_t* list_off_out, exc_index** out)
+extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint32_t hs_target,
+                                      uint32_t hs_shift, uint64_t seed, uint32_t universe, const uint32_t* df,
+                                      uint32_t nterms, uint64_t batch_postings, uint64_t* list_off_out,
+                                      exc_index** out)
 {
     if (!c || (nterms && !df) || !out) return set_error(EXC_ERR_ARG, "exc_synth_index_device: bad argument");
     HIP_TRY(hipSetDevice(c->device));
@@ -1907,30 +2041,33 @@ extern "C" int exc_synth_index_device(exc_ctx* c, int codec, uint32_t seg, float
     }
     uint32_t* d_docs;
     if (int rc = tmp.get(maxb, &d_docs)) return rc;
-    return build_index_on_device(c, codec, seg, frac, nterms, P.list_off, batches,
-                                 [&](uint32_t t0, uint32_t t1, const uint32_t** p) {
-                                     *p = d_docs;
-                                     return synth_write(c, P, a, t0, t1, d_docs);
-                                 },
-                                 out);
+    const auto docs_for = [&](uint32_t t0, uint32_t t1, const uint32_t** p) {
+        *p = d_docs;
+        return synth_write(c, P, a, t0, t1, d_docs);
+    };
+    if (codec == EXC_HS_SEGLRC)
+        return build_hs_on_device(c, hs_target, hs_shift, nterms, P.list_off, batches, docs_for, out);
+    return build_index_on_device(c, codec, seg, frac, nterms, P.list_off, batches, docs_for, out);
 }
 
 extern "C" int exc_index_export(exc_ctx* c, exc_index* ix, exc_built** out)
 {
     if (!c || !ix || !out) return set_error(EXC_ERR_ARG, "exc_index_export: bad argument");
-    if (ix->codec != EXC_SEGLRC && ix->codec != EXC_LRC && ix->codec != EXC_PARAPFD)
-        return set_error(EXC_ERR_ARG, "exc_index_export: SegLRC, LRC and ParaPFD indexes");
+    if (ix->codec != EXC_SEGLRC && ix->codec != EXC_LRC && ix->codec != EXC_PARAPFD && ix->codec != EXC_HS_SEGLRC)
+        return set_error(EXC_ERR_ARG, "exc_index_export: SegLRC, LRC, ParaPFD and HS_SegLRC indexes");
     HIP_TRY(hipSetDevice(c->device));
+    const bool hs = ix->codec == EXC_HS_SEGLRC;
     std::vector<SegRec> rec(ix->nseg);
-    std::vector<uint32_t> head(ix->nseg), words(ix->nwords);
+    std::vector<uint32_t> head(hs ? 0 : ix->nseg), words(ix->nwords), hash(hs ? ix->hash_words : 0);
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (ix->nseg) {
         HIP_TRY(hipMemcpy(rec.data(), ix->dix.rec, ix->nseg * sizeof(SegRec), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(head.data(), ix->dix.seg_head, ix->nseg * 4, hipMemcpyDeviceToHost));
+        if (!hs) HIP_TRY(hipMemcpy(head.data(), ix->dix.seg_head, ix->nseg * 4, hipMemcpyDeviceToHost));
     }
     if (ix->nwords) HIP_TRY(hipMemcpy(words.data(), ix->dix.words, ix->nwords * 4, hipMemcpyDeviceToHost));
-    return built_from_device(ix->codec, ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(), head.data(), words.data(),
-                             ix->nwords, out);
+    if (!hash.empty()) HIP_TRY(hipMemcpy(hash.data(), ix->dix.hash, hash.size() * 4, hipMemcpyDeviceToHost));
+    return built_from_device(ix->codec, ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(),
+                             hs ? nullptr : head.data(), words.data(), ix->nwords, hash.data(), hash.size(), out);
 }
 
 // ---------------------------------------------------------------- autotuner

@@ -278,29 +278,33 @@ void exc_server_destroy(exc_server *sv);
 /* ---- index construction on the GPU (replaces the reference's per-list
  *      generator loops with synchronous copies, GC/SegLRC/SegLRC_Generator/
  *      SegLRC_Generator.cu:94-331, GC/LRC/LRC_Generator/LRC_Generator.cu:144-188,
- *      and the CPU packing of GC/LRC/LRC_Compression/main.cpp:163-199 and
- *      GC/ParaPFD/ParaPFD_Compression/main.cpp:151-189, for indexes built where
- *      they are served).
+ *      GC/HS_SegLRC/HS_SegLRC_Generator/HS_SegLRC_Generator.cu:95-288,503-594,
+ *      and the CPU packing of GC/LRC/LRC_Compression/main.cpp:163-199,
+ *      GC/ParaPFD/ParaPFD_Compression/main.cpp:151-189 and
+ *      GC/HS_SegLRC/HS_SegLRC_Compression/main.cpp:149-178, for indexes built
+ *      where they are served).
  * exc_synth_lists_device: seeded synthetic lists, term t a Bernoulli(df[t] /
  *   universe) sample of [0, universe) (so its length is df[t] on average);
  *   list_off[0..nterms] receives the offsets; with d_docs (device memory of cap
  *   docIDs) the lists are written there, with d_docs NULL only counted.
- * exc_build_index_device: an index of codec EXC_SEGLRC, EXC_LRC or
- *   EXC_PARAPFD (frac: the ParaPFor exception fraction; LRC and SegLRC use 0)
- *   built from sorted lists already in device memory (list t = d_docs[list_off[t]
- *   .. list_off[t+1]), strictly increasing, < 2^31); block-level arrays
- *   byte-identical to exc_build_index (the LRC list fit too).
+ * exc_build_index_device: an index of codec EXC_SEGLRC, EXC_LRC, EXC_PARAPFD
+ *   or EXC_HS_SEGLRC (frac: the ParaPFor exception fraction, ParaPFD only;
+ *   hs_target and hs_shift: the HS bucket rule as for exc_build_index) built
+ *   from sorted lists already in device memory (list t = d_docs[list_off[t] ..
+ *   list_off[t+1]), strictly increasing, < 2^31); block-level arrays and HS
+ *   tables byte-identical to exc_build_index (the LRC list fit too; SegLRC's
+ *   list-level fit sums in another order).
  * exc_synth_index_device: both, in batches of about batch_postings docIDs, so
  *   that the raw lists never exist whole; list_off_out (nterms + 1) optional.
- * exc_index_export: a SegLRC, LRC or ParaPFD index's arrays back on the host
+ * exc_index_export: a SegLRC, LRC, ParaPFD or HS_SegLRC index's arrays back on the host
  *   in the reference layout (for exc_index_save_files or comparison). */
 int exc_synth_lists_device(exc_ctx *ctx, uint64_t seed, uint32_t universe, const uint32_t *df, uint32_t nterms,
                            uint64_t *list_off, uint32_t *d_docs, uint64_t cap);
-int exc_build_index_device(exc_ctx *ctx, int codec, uint32_t seg, float frac, const uint32_t *d_docs,
-                           const uint64_t *list_off, uint32_t nterms, exc_index **out);
-int exc_synth_index_device(exc_ctx *ctx, int codec, uint32_t seg, float frac, uint64_t seed, uint32_t universe,
-                           const uint32_t *df, uint32_t nterms, uint64_t batch_postings, uint64_t *list_off_out,
-                           exc_index **out);
+int exc_build_index_device(exc_ctx *ctx, int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
+                           const uint32_t *d_docs, const uint64_t *list_off, uint32_t nterms, exc_index **out);
+int exc_synth_index_device(exc_ctx *ctx, int codec, uint32_t seg, float frac, uint32_t hs_target, uint32_t hs_shift,
+                           uint64_t seed, uint32_t universe, const uint32_t *df, uint32_t nterms,
+                           uint64_t batch_postings, uint64_t *list_off_out, exc_index **out);
 int exc_index_export(exc_ctx *ctx, exc_index *ix, exc_built **out);
 
 /* ---- index builder (replaces the *_Generator and *_Compression programs,

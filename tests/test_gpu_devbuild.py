@@ -1,6 +1,7 @@
 """GPU tests of index construction on the device: seeded synthetic lists, the
-SegLRC encoder (segment-level bytes identical to the host builder's), the
-batched synthesize-and-encode path, and exact queries on a device-built index."""
+SegLRC, LRC, ParaPFD and HS_SegLRC encoders (the reference layout's arrays
+byte-identical to the host builder's), the batched synthesize-and-encode path,
+and exact queries on device-built indexes."""
 import numpy as np
 import pytest
 
@@ -146,10 +147,44 @@ def test_device_lrc_parapfd_encoders_match_host_builder(ctx, codec, seg, frac):
         ctx.device_free(d)
 
 
+@pytest.mark.parametrize("shift", [0, 13, 9])
+def test_device_hs_encoder_matches_host_builder(ctx, shift):
+    """The HS_SegLRC encoder on the device: the hash tables
+    ([shift][nb][start_0 .. start_nb], adaptive or a fixed shift), per-bucket
+    fits, bases, offsets and packed residuals byte-identical to the host
+    builder's, the index answering queries as the oracle does."""
+    d, off, docs = _synth(ctx, 71)
+    try:
+        host = ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=shift)
+        dix = ctx.build_index_device(d, off, codec=ea.HS_SEGLRC, hs_shift=shift)
+        try:
+            dev = ctx.export(dix)
+            for name in ("ind1", "ind2", "seg_num", "seg_offset", "seg_info", "seg_base", "hash_offset", "hash_info"):
+                assert getattr(dev, name).tobytes() == getattr(host, name).tobytes(), name
+            lists = [docs[int(off[t]):int(off[t + 1])] for t in range(len(DF))]
+            rng = np.random.default_rng(6)
+            qs = [[int(x) for x in rng.choice(len(DF), int(rng.integers(1, 5)), replace=False)] for _ in range(200)]
+            ocum, ores = oracle.eval_queries(oracle.build_index(lists, oracle.HS, seg=128, hs_shift=shift), qs)
+            cum, res = ctx.intersect(dix, qs)
+            assert np.array_equal(cum, ocum) and np.array_equal(res, ores)
+            assert np.array_equal(ctx.decompress(dix, list(range(len(DF)))), docs)
+        finally:
+            dix.close()
+        many, off2 = ctx.synth_index_device(71, U, DF, batch_postings=1 << 20, codec=ea.HS_SEGLRC, hs_shift=shift)
+        try:
+            b = ctx.export(many)
+            for name in ("ind2", "seg_offset", "seg_info", "hash_info"):
+                assert getattr(b, name).tobytes() == getattr(host, name).tobytes(), name
+        finally:
+            many.close()
+    finally:
+        ctx.device_free(d)
+
+
 def test_device_encoder_refuses_other_codecs(ctx):
     d, off, docs = _synth(ctx, 61, df=np.array([10, 100], np.uint32))
     try:
-        for codec in (ea.HS_SEGLRC, ea.NEWPFD, ea.RAW):
+        for codec in (ea.NEWPFD, ea.RAW, ea.RAW_HS):
             with pytest.raises(ea.ExcError) as e:
                 ctx.build_index_device(d, off, seg=128, codec=codec)
             assert e.value.code == -1

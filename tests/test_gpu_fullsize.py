@@ -199,21 +199,25 @@ def test_c2_full_size_decode(ctx, c2, codec, seg, frac):
     assert np.array_equal(out, docs)
 
 
-@pytest.mark.parametrize("codec,seg,frac", [(ea.SEGLRC, 128, 0.0), (ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1)])
+@pytest.mark.parametrize("codec,seg,frac", [(ea.SEGLRC, 128, 0.0), (ea.LRC, 256, 0.0), (ea.PARAPFD, 128, 0.1),
+                                            (ea.HS_SEGLRC, 128, 0.0)])
 def test_c2_full_size_device_builder(ctx, c2, codec, seg, frac):
-    """The device encoders over the full-size lists: the index's arrays in the
-    reference layout byte-identical to the host builder's, and the C2 batch
-    answered on the device-built index exactly as the oracle answers it."""
+    """The device encoders over the full-size lists (HS_SegLRC with the
+    adaptive bucket width): the index's arrays in the reference layout
+    byte-identical to the host builder's, and the C2 batch answered on the
+    device-built index exactly as the oracle answers it."""
     docs, off, queries, built, ocum, ores, _ = c2
     host = built if codec == ea.SEGLRC else ea.build_index(docs, off, codec, seg=seg, frac=frac)
-    names = {ea.SEGLRC: ("seg_info", "seg_base"), ea.LRC: ("lrc_info",), ea.PARAPFD: ("seg_median",)}[codec]
+    names = {ea.SEGLRC: ("seg_info", "seg_base", "seg_head"), ea.LRC: ("lrc_info", "seg_head"),
+             ea.PARAPFD: ("seg_median", "seg_head"),
+             ea.HS_SEGLRC: ("seg_info", "seg_base", "hash_offset", "hash_info")}[codec]
     d = ctx.device_alloc(len(docs) * 4)
     try:
         ctx.to_device(d, docs)
         dix = ctx.build_index_device(d, off, seg=seg, codec=codec, frac=frac)
         try:
             dev = ctx.export(dix)
-            for name in ("ind1", "ind2", "seg_num", "seg_offset", "seg_head") + names:
+            for name in ("ind1", "ind2", "seg_num", "seg_offset") + names:
                 assert getattr(dev, name).tobytes() == getattr(host, name).tobytes(), name
             cum, res = ctx.intersect(dix, queries)
             assert np.array_equal(np.asarray(cum, np.int64), ocum) and np.array_equal(res, ores)
