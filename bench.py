@@ -417,14 +417,54 @@ def ratio_class(df, q):
     return k
 
 
+def routed_leg(ea, ctx, docs, off, hs_keep, seglrc128, qs, sample, rcum, rres, steps):
+    """The C3 batch through the library router: candidates are the hash-bucket
+    widths of hs_keep and SegLRC at segments of 128 and 256 docIDs; the router
+    tunes every candidate, routes each ratio class to its fastest one, keeps
+    the routes only if the routed sample beats every candidate alone (its own
+    measured runs), then the 10,000-query batch replays as one captured graph
+    of all sub-batches."""
+    s256 = ea.build_index(docs, off, ea.SEGLRC, seg=256)
+    d256 = ctx.upload(s256)
+    names = ["hs_" + ("adaptive" if sh == 0 else "shift%d" % sh) for sh in hs_keep] + ["seglrc128", "seglrc256"]
+    cands = [v[0] for v in hs_keep.values()] + [seglrc128, d256]
+    router = ea.Router(ctx, cands)
+    try:
+        t = router.autotune(sample, reps=2)
+        router.set_option(ea.OPT_GRAPH, 1)
+        router.plan(qs)
+        router.invoke()
+        router.invoke()
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ms.append(router.invoke())
+        dt = time.perf_counter() - t0
+        cum, res = router.fetch()
+        row = {"candidates": names, "route": {str(k): names[r] for k, r in enumerate(t["route"])},
+               "sample_single_ms": dict(zip(names, t["single_ms"])), "sample_routed_ms": t["routed_ms"],
+               "persistent": t["persistent"], "graph_step_ms": float(np.median(ms)),
+               "qps": len(qs) / (float(np.median(ms)) * 1e-3), "wall_qps": len(qs) * steps / dt,
+               "matches_seglrc": bool(np.array_equal(cum, rcum) and np.array_equal(res, rres))}
+    finally:
+        router.close()
+        d256.close()
+        s256.close()
+    log("C3 routed: %s" % row)
+    return row
+
+
 def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
     """C3: the C2 lists in hash buckets on the top docID bits (shift 13 = the top
     12 bits of 2^25; also 12, 14 and the adaptive ~256-per-bucket width), 10,000
     2-term queries with long:short ratios 10^U(1,4).  Per bucket width: search
     knobs autotuned per ratio class, the whole batch as one launch (persistent
     work queue or one wave per unit, as the autotuner chose) replayed from a
-    hipGraph, and every class's sub-batch timed; the bucket
-    width is then chosen per ratio class and the batch run routed that way."""
+    hipGraph, and every class's sub-batch timed.  Then the library's router
+    (exc_router_*) takes the four widths and two SegLRC segment sizes as
+    candidates, chooses a candidate per ratio class on a 4,000-query sample,
+    and the whole 10,000-query batch runs routed: every candidate's sub-batch
+    on a stream of its own, one captured hipGraph, timed as one run."""
     qs = W.ratio_queries(df, 10_000, 3)
     sample = W.ratio_queries(df, 4000, 33)
     rcum, rres = ctx.intersect(ref_dix, qs)  # SegLRC result of the same queries
@@ -454,21 +494,14 @@ def c3_leg(ea, W, ctx, docs, off, df, ref_dix, steps, shifts=(13, 12, 14, 0)):
         out["per_width"]["adaptive" if shift == 0 else "shift%d" % shift] = row
         keep[shift] = (dix, hs, row)
         log("C3 shift %d: %s" % (shift, row))
-    # bucket width per ratio class, then the batch routed class by class
-    choice, routed_ms = {}, 0.0
-    for k in classes:
-        best = min(shifts, key=lambda sh: keep[sh][2]["class_probe_ms"][str(k)])
-        choice[str(k)] = "adaptive" if best == 0 else "shift%d" % best
-        routed_ms += keep[best][2]["class_probe_ms"][str(k)]
-    out["width_per_class"] = choice
-    out["routed_probe_ms"] = routed_ms
-    out["routed_qps"] = len(qs) / (routed_ms * 1e-3)
+    ctx.set_tuning(None)
     out["fixed_shift13_qps"] = keep[13][2]["graph_qps"]
     out["plain_single_query_shift13"] = plain_leg(ea, ctx, keep[13][0], qs, rcum, rres)
     ctx.set_option(ea.OPT_SEARCH, 1)
+    out["routed"] = routed_leg(ea, ctx, docs, off, keep, ref_dix, qs, sample, rcum, rres, steps)
     for dix, hs, _ in keep.values():
         dix.close()
-    ctx.set_tuning(None)
+        hs.close()
     # the uncompressed hash-segment layout (GU/cuda_hashSegment_new): the docIDs
     # themselves plus [shift][start_0 .. start_nb] tables, the same queries
     out["uncompressed_hash_segment"] = {}

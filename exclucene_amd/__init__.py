@@ -36,6 +36,8 @@ EXPORTS = [
     "exc_device_alloc", "exc_device_free", "exc_memcpy",
     "exc_synth_lists_device", "exc_build_index_device", "exc_synth_index_device", "exc_index_export",
     "exc_intersect_noalloc",
+    "exc_router_create", "exc_router_autotune", "exc_router_set_routes", "exc_router_set_option",
+    "exc_router_plan", "exc_router_invoke", "exc_router_fetch", "exc_router_intersect", "exc_router_destroy",
 ]
 MODEL_PLA, MODEL_MLP = 1, 2
 SERVER_COUNTS = 1
@@ -68,6 +70,22 @@ class TuneTable(ctypes.Structure):
         return {"queries": list(self.queries), "coop_max": list(self.coop_max), "search": list(self.search),
                 "ms": [float(x) for x in self.ms], "ms_default": [float(x) for x in self.ms_default],
                 "persistent": int(self.persistent), "batch_ms": [float(x) for x in self.batch_ms]}
+
+
+ROUTER_MAX_CANDIDATES = 8
+
+
+class RouteTable(ctypes.Structure):
+    _fields_ = [("ncands", _u32), ("route", _u32 * RATIO_CLASSES),
+                ("class_ms", (ctypes.c_float * RATIO_CLASSES) * ROUTER_MAX_CANDIDATES),
+                ("single_ms", ctypes.c_float * ROUTER_MAX_CANDIDATES), ("routed_ms", ctypes.c_float),
+                ("persistent", _u32)]
+
+    def as_dict(self):
+        n = int(self.ncands)
+        return {"route": list(self.route), "class_ms": [[float(x) for x in self.class_ms[i]] for i in range(n)],
+                "single_ms": [float(x) for x in self.single_ms[:n]], "routed_ms": float(self.routed_ms),
+                "persistent": int(self.persistent)}
 
 
 class ServerInfo(ctypes.Structure):
@@ -164,6 +182,16 @@ def lib():
         L.exc_decompress_device.argtypes = [_vp, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]
         L.exc_autotune.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(TuneTable)]
         L.exc_ctx_set_tuning.argtypes = [_vp, ctypes.POINTER(TuneTable)]
+        L.exc_router_create.argtypes = [_vp, _vp, _u32, ctypes.POINTER(_vp)]
+        L.exc_router_autotune.argtypes = [_vp, _vp, _vp, _u32, _u32, ctypes.POINTER(RouteTable)]
+        L.exc_router_set_routes.argtypes = [_vp, _vp]
+        L.exc_router_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
+        L.exc_router_plan.argtypes = [_vp, _vp, _vp, _u32]
+        L.exc_router_invoke.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
+        L.exc_router_fetch.argtypes = [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+        L.exc_router_intersect.argtypes = [_vp, _vp, _vp, _u32, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+        L.exc_router_destroy.argtypes = [_vp]
+        L.exc_router_destroy.restype = None
         L.exc_server_destroy.restype = None
         L.exc_server_create.argtypes = [_vp, _vp, _u32, _u32, _u64, _u32, ctypes.POINTER(_vp)]
         L.exc_server_submit.argtypes = [_vp, _vp, _u32, ctypes.POINTER(_u64)]
@@ -317,6 +345,19 @@ class DeviceIndex:
             self._h = None
 
 
+def _result_cap(ucnt, qterms, qoff):
+    """Results a batch can hold at most: the summed shortest-list lengths."""
+    if ucnt is None:
+        return 1 << 26
+    cap = 0
+    for q in range(len(qoff) - 1):
+        ts = qterms[int(qoff[q]):int(qoff[q + 1])]
+        ts = ts[ts < len(ucnt)]  # out-of-range terms are reported by the library
+        if len(ts):
+            cap += int(ucnt[ts].min())
+    return cap
+
+
 class Context:
     """One HIP stream and its batch buffers (one per host thread)."""
 
@@ -408,16 +449,7 @@ class Context:
     def _intersect_with(self, fn, index, queries):
         qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
         nq = len(qoff) - 1
-        ucnt = index.ucnt
-        cap = 0
-        if ucnt is not None:
-            for q in range(nq):
-                ts = qterms[int(qoff[q]):int(qoff[q + 1])]
-                ts = ts[ts < len(ucnt)]  # out-of-range terms are reported by the library
-                if len(ts):
-                    cap += int(ucnt[ts].min())
-        else:
-            cap = 1 << 26
+        cap = _result_cap(index.ucnt, qterms, qoff)
         cum = np.zeros(max(nq, 1), np.uint32)
         res = np.zeros(max(cap, 1), np.uint32)
         n = _u64()
@@ -542,6 +574,74 @@ class Context:
         if self._h:
             lib().exc_ctx_destroy(self._h)
             self._h = None
+
+
+class Router:
+    """One batch routed by ratio class over several indexes of the same lists
+    (exc_router_*): each candidate's sub-batch on a stream of its own, the
+    whole run timed as one."""
+
+    def __init__(self, ctx, indexes):
+        self.ctx = ctx
+        self.indexes = list(indexes)
+        arr = (_vp * len(self.indexes))(*[ix.handle.value for ix in self.indexes])
+        h = _vp()
+        _check(lib().exc_router_create(ctx.handle, arr, len(self.indexes), ctypes.byref(h)))
+        self._h = h
+        self._cap = 0
+        self._nq = 0
+        ucnts = [ix.ucnt for ix in self.indexes if ix.ucnt is not None]
+        self._ucnt = ucnts[0] if ucnts else None
+
+    def autotune(self, queries, reps=3):
+        qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
+        t = RouteTable()
+        _check(lib().exc_router_autotune(self._h, _p(qterms), _p(qoff), len(qoff) - 1, reps, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_routes(self, routes):
+        r = np.ascontiguousarray(routes, np.uint32)
+        assert len(r) == RATIO_CLASSES
+        _check(lib().exc_router_set_routes(self._h, _p(r)))
+
+    def set_option(self, key, value):
+        _check(lib().exc_router_set_option(self._h, key, int(value)))
+
+    def plan(self, queries):
+        qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
+        _check(lib().exc_router_plan(self._h, _p(qterms), _p(qoff), len(qoff) - 1))
+        self._nq = len(qoff) - 1
+        self._cap = _result_cap(self._ucnt, qterms, qoff)
+
+    def invoke(self):
+        """Run the planned batch; returns the run's device time in ms."""
+        ms = ctypes.c_float()
+        _check(lib().exc_router_invoke(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def fetch(self):
+        cum = np.zeros(max(self._nq, 1), np.uint32)
+        res = np.zeros(max(self._cap, 1), np.uint32)
+        n = _u64()
+        _check(lib().exc_router_fetch(self._h, _p(cum), _p(res), self._cap, ctypes.byref(n)))
+        return cum[:self._nq], res[:int(n.value)]
+
+    def intersect(self, queries):
+        """Exact results of every query, in query order: (cum_counts, results)."""
+        self.plan(queries)
+        self.invoke()
+        return self.fetch()
+
+    def close(self):
+        if self._h:
+            lib().exc_router_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

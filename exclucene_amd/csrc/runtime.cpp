@@ -2204,6 +2204,379 @@ extern "C" int exc_ctx_set_tuning(exc_ctx* c, const exc_tune_table* t)
     return EXC_OK;
 }
 
+// ---------------------------------------------------------------- router
+
+// One context per candidate: its own stream and batch buffers, so the
+// candidates' sub-batches are queued side by side and run concurrently.  The
+// router's own stream forks them (each candidate stream waits on a marker of
+// it) and joins them (it waits on a marker of each), which also makes the
+// whole run one capturable unit.
+struct exc_router {
+    exc_ctx* parent = nullptr;
+    std::vector<exc_index*> cands;
+    std::vector<exc_ctx*> kids;
+    uint32_t route[kRatioClasses] = {};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around the run, on the router stream
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    int graph = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint64_t> graph_model_gen;  // each candidate's model generation the graph captured
+    // plan of the batch
+    uint32_t nq = 0;
+    int state = 0;  // 0 none, 2 planned and on the device, 3 run
+    std::vector<uint8_t> active;  // candidate has queries in the batch
+    std::vector<uint32_t> qcand, qidx;  // candidate of each query, its number in that sub-batch
+    float ms = 0;
+};
+
+namespace {
+
+void router_drop_graph(exc_router* r)
+{
+    if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+    r->gexec = nullptr;
+}
+
+int router_plan(exc_router* r, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq)
+{
+    const size_t n = r->cands.size();
+    const exc_index* ix0 = r->cands[0];
+    std::vector<std::vector<uint32_t>> st(n);
+    std::vector<std::vector<uint64_t>> so(n, std::vector<uint64_t>{0});
+    r->state = 0;
+    r->qcand.resize(nq);
+    r->qidx.resize(nq);
+    std::vector<uint32_t> tmp;
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (qoff[q + 1] < qoff[q]) return set_error(EXC_ERR_ARG, "query %u: bad offsets", q);
+        tmp.assign(qterms + qoff[q], qterms + qoff[q + 1]);
+        for (uint32_t t : tmp)
+            if (t >= ix0->nterms) return set_error(EXC_ERR_ARG, "query %u: term %u out of range", q, t);
+        std::stable_sort(tmp.begin(), tmp.end(), [&](uint32_t a, uint32_t b) { return ix0->n[a] < ix0->n[b]; });
+        const uint32_t i = r->route[ratio_class(ix0, tmp.data(), (uint32_t)tmp.size())];
+        r->qcand[q] = i;
+        r->qidx[q] = (uint32_t)(so[i].size() - 1);
+        st[i].insert(st[i].end(), qterms + qoff[q], qterms + qoff[q + 1]);
+        so[i].push_back(st[i].size());
+    }
+    router_drop_graph(r);
+    for (size_t i = 0; i < n; ++i) {
+        r->active[i] = so[i].size() > 1;
+        if (!r->active[i]) continue;
+        int rc = plan_queries(r->kids[i], r->cands[i], st[i].data(), so[i].data(), (uint32_t)(so[i].size() - 1));
+        if (!rc) rc = upload_plan(r->kids[i]);
+        if (rc) return rc;
+    }
+    r->nq = nq;
+    r->state = 2;
+    return EXC_OK;
+}
+
+// Queue every active candidate's sub-batch between the fork and the join.
+int router_enqueue(exc_router* r, hipEvent_t fork)
+{
+    for (size_t i = 0; i < r->kids.size(); ++i) {
+        if (!r->active[i]) continue;
+        exc_ctx* k = r->kids[i];
+        HIP_TRY(hipStreamWaitEvent(k->stream, fork, 0));
+        if (int rc = enqueue_invoke(k, false)) return rc;
+        HIP_TRY(hipEventRecord(r->join[i], k->stream));
+        HIP_TRY(hipStreamWaitEvent(r->stream, r->join[i], 0));
+    }
+    return EXC_OK;
+}
+
+int router_run(exc_router* r)
+{
+    if (r->state < 2) return set_error(EXC_ERR_STATE, "exc_router_invoke before exc_router_plan");
+    HIP_TRY(hipSetDevice(r->parent->device));
+    if (std::find(r->active.begin(), r->active.end(), 1) == r->active.end()) {  // an empty batch
+        r->ms = 0;
+        r->state = 3;
+        return EXC_OK;
+    }
+    for (size_t i = 0; i < r->kids.size(); ++i) {
+        if (!r->active[i]) continue;
+        if (int rc = refresh_knobs(r->kids[i])) return rc;
+        if (r->gexec && r->graph_model_gen[i] != r->cands[i]->model_gen) router_drop_graph(r);
+    }
+    if (r->graph) {
+        if (!r->gexec) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipEventRecord(r->fork, r->stream);
+            int rc = e == hipSuccess ? router_enqueue(r, r->fork)
+                                     : set_error(EXC_ERR_DEVICE, "hipEventRecord: %s", hipGetErrorString(e));
+            e = hipStreamEndCapture(r->stream, &g);  // ends the capture on every joined stream
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&r->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return set_error(EXC_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            for (size_t i = 0; i < r->cands.size(); ++i) r->graph_model_gen[i] = r->cands[i]->model_gen;
+        }
+        HIP_TRY(hipEventRecord(r->ev[0], r->stream));
+        HIP_TRY(hipGraphLaunch(r->gexec, r->stream));
+        HIP_TRY(hipEventRecord(r->ev[1], r->stream));
+    } else {
+        HIP_TRY(hipEventRecord(r->ev[0], r->stream));
+        if (int rc = router_enqueue(r, r->ev[0])) return rc;
+        HIP_TRY(hipEventRecord(r->ev[1], r->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    r->ms = 0;
+    (void)hipEventElapsedTime(&r->ms, r->ev[0], r->ev[1]);
+    for (size_t i = 0; i < r->kids.size(); ++i)
+        if (r->active[i]) r->kids[i]->state = 3;
+    r->state = 3;
+    return EXC_OK;
+}
+
+// Best whole-run time over reps runs of the planned batch (after one warm run).
+int router_time(exc_router* r, uint32_t reps, float* best)
+{
+    int rc = router_run(r);
+    *best = 1e30f;
+    for (uint32_t k = 0; k < reps && !rc; ++k)
+        if (!(rc = router_run(r))) *best = std::min(*best, r->ms);
+    return rc;
+}
+
+void router_set_persistent(exc_router* r, const std::vector<int>& p)
+{
+    for (size_t i = 0; i < r->kids.size(); ++i)
+        if (r->kids[i]->persistent != p[i]) {
+            r->kids[i]->persistent = p[i];
+            drop_graph(r->kids[i]);
+        }
+    router_drop_graph(r);
+}
+
+}  // namespace
+
+extern "C" int exc_router_create(exc_ctx* c, exc_index* const* cands, uint32_t n, exc_router** out)
+{
+    if (!c || !cands || !out || !n) return set_error(EXC_ERR_ARG, "exc_router_create: bad argument");
+    if (n > EXC_ROUTER_MAX_CANDIDATES)
+        return set_error(EXC_ERR_ARG, "exc_router_create: at most %d candidates", EXC_ROUTER_MAX_CANDIDATES);
+    for (uint32_t i = 0; i < n; ++i) {
+        const exc_index* ix = cands[i];
+        if (!ix) return set_error(EXC_ERR_ARG, "exc_router_create: candidate %u is null", i);
+        if (ix->codec == EXC_NEWPFD) return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only");
+        if (ix->ctx->device != c->device)
+            return set_error(EXC_ERR_ARG, "candidate %u lives on device %d, the context on %d", i, ix->ctx->device,
+                             c->device);
+        if (ix->nterms != cands[0]->nterms || ix->n != cands[0]->n)
+            return set_error(EXC_ERR_ARG, "candidate %u holds other lists than candidate 0", i);
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    auto* r = new exc_router();
+    r->parent = c;
+    r->cands.assign(cands, cands + n);
+    r->active.assign(n, 0);
+    r->join.assign(n, nullptr);
+    r->graph_model_gen.assign(n, 0);
+    int rc = EXC_OK;
+    for (uint32_t i = 0; i < n && !rc; ++i) {
+        exc_ctx* k = nullptr;
+        if (!(rc = exc_ctx_create(c->device, c->budget, &k))) {
+            k->search = c->search;
+            k->tpe = c->tpe;
+            k->coop_max = c->coop_max;
+            k->persistent = c->persistent;
+            k->timing = 0;
+            r->kids.push_back(k);
+        }
+    }
+    hipError_t e = rc ? hipSuccess : hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    for (int j = 0; j < 2 && e == hipSuccess; ++j) e = hipEventCreate(&r->ev[j]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->fork, hipEventDisableTiming);
+    for (uint32_t i = 0; i < n && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&r->join[i], hipEventDisableTiming);
+    if (!rc && e != hipSuccess) rc = set_error(EXC_ERR_DEVICE, "exc_router_create: %s", hipGetErrorString(e));
+    if (rc) {
+        exc_router_destroy(r);
+        return rc;
+    }
+    *out = r;
+    return EXC_OK;
+}
+
+extern "C" void exc_router_destroy(exc_router* r)
+{
+    if (!r) return;
+    (void)hipSetDevice(r->parent->device);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    router_drop_graph(r);
+    for (exc_ctx* k : r->kids) exc_ctx_destroy(k);
+    for (hipEvent_t e : r->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (r->fork) (void)hipEventDestroy(r->fork);
+    for (hipEvent_t e : r->join)
+        if (e) (void)hipEventDestroy(e);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}
+
+extern "C" int exc_router_set_routes(exc_router* r, const uint32_t* route)
+{
+    if (!r || !route) return set_error(EXC_ERR_ARG, "exc_router_set_routes: null argument");
+    for (int k = 0; k < kRatioClasses; ++k)
+        if (route[k] >= r->cands.size())
+            return set_error(EXC_ERR_ARG, "class %d: candidate %u of %zu", k, route[k], r->cands.size());
+    std::memcpy(r->route, route, sizeof r->route);
+    r->state = 0;  // the plan in hand was routed the old way
+    router_drop_graph(r);
+    return EXC_OK;
+}
+
+extern "C" int exc_router_set_option(exc_router* r, int key, int64_t v)
+{
+    if (!r) return set_error(EXC_ERR_ARG, "exc_router_set_option: null router");
+    if (key == EXC_OPT_GRAPH) {
+        r->graph = v != 0;
+    } else if (key == EXC_OPT_TIMING) {
+        // the router times every run itself; the candidates' own events stay off
+    } else {
+        for (exc_ctx* k : r->kids)
+            if (int rc = exc_ctx_set_option(k, key, v)) return rc;
+    }
+    router_drop_graph(r);
+    return EXC_OK;
+}
+
+extern "C" int exc_router_plan(exc_router* r, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq)
+{
+    if (!r || !qoff || (nq && !qterms && qoff[nq])) return set_error(EXC_ERR_ARG, "exc_router_plan: null argument");
+    return router_plan(r, qterms, qoff, nq);
+}
+
+extern "C" int exc_router_invoke(exc_router* r, float* ms)
+{
+    if (!r) return set_error(EXC_ERR_ARG, "exc_router_invoke: null router");
+    int rc = router_run(r);
+    if (!rc && ms) *ms = r->ms;
+    return rc;
+}
+
+extern "C" int exc_router_fetch(exc_router* r, uint32_t* cum, uint32_t* res, uint64_t cap, uint64_t* nres)
+{
+    if (!r || (r->nq && !cum)) return set_error(EXC_ERR_ARG, "exc_router_fetch: null argument");
+    if (r->state < 3) return set_error(EXC_ERR_STATE, "exc_router_fetch before exc_router_invoke");
+    HIP_TRY(hipSetDevice(r->parent->device));
+    const size_t n = r->kids.size();
+    std::vector<std::vector<uint32_t>> kc(n), kr(n);
+    uint64_t total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (!r->active[i]) continue;
+        exc_ctx* k = r->kids[i];
+        kc[i].resize(k->nq);
+        HIP_TRY(hipMemcpy(kc[i].data(), k->d_qcum.p, (size_t)k->nq * 4, hipMemcpyDeviceToHost));
+        kr[i].resize(kc[i].back());
+        if (!kr[i].empty()) HIP_TRY(hipMemcpy(kr[i].data(), k->d_result.p, kr[i].size() * 4, hipMemcpyDeviceToHost));
+        total += kr[i].size();
+    }
+    if (nres) *nres = total;
+    if (total > cap)
+        return set_error(EXC_ERR_CAPACITY, "result buffer holds %llu, need %llu", (unsigned long long)cap,
+                         (unsigned long long)total);
+    if (total > 0xffffffffull) return set_error(EXC_ERR_CAPACITY, "more than 2^32 results in one batch");
+    uint64_t o = 0;
+    for (uint32_t q = 0; q < r->nq; ++q) {
+        const uint32_t i = r->qcand[q], j = r->qidx[q];
+        const uint32_t b = j ? kc[i][j - 1] : 0u, e = kc[i][j];
+        if (e > b) std::memcpy(res + o, kr[i].data() + b, (size_t)(e - b) * 4);
+        o += e - b;
+        cum[q] = (uint32_t)o;
+    }
+    return EXC_OK;
+}
+
+extern "C" int exc_router_intersect(exc_router* r, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq,
+                                    uint32_t* cum, uint32_t* res, uint64_t cap, uint64_t* nres)
+{
+    if (int rc = exc_router_plan(r, qterms, qoff, nq)) return rc;
+    if (int rc = router_run(r)) return rc;
+    return exc_router_fetch(r, cum, res, cap, nres);
+}
+
+extern "C" int exc_router_autotune(exc_router* r, const uint32_t* qterms, const uint64_t* qoff, uint32_t nq,
+                                   uint32_t reps, exc_route_table* out)
+{
+    if (!r || !qoff || (nq && !qterms) || !out) return set_error(EXC_ERR_ARG, "exc_router_autotune: bad argument");
+    const uint32_t n = (uint32_t)r->cands.size();
+    reps = std::max<uint32_t>(1, std::min<uint32_t>(reps, 50));
+    exc_route_table T;
+    std::memset(&T, 0, sizeof T);
+    T.ncands = n;
+    const bool had_graph = r->graph;
+    r->graph = 0;
+    router_drop_graph(r);
+    // every candidate's knobs and launch mode on the sample
+    std::vector<exc_tune_table> tt(n);
+    std::vector<int> own(n);
+    int rc = EXC_OK;
+    for (uint32_t i = 0; i < n && !rc; ++i) {
+        rc = exc_autotune(r->kids[i], r->cands[i], qterms, qoff, nq, reps, &tt[i]);
+        own[i] = r->kids[i]->persistent;
+        r->kids[i]->timing = 0;
+        for (int k = 0; k < kRatioClasses; ++k) T.class_ms[i][k] = tt[i].ms[k];
+    }
+    uint32_t route[kRatioClasses] = {};
+    if (!rc) {
+        // each class to its fastest candidate; single-term queries probe
+        // nothing and go where most of the sample goes
+        std::vector<uint32_t> load(n, 0);
+        for (int k = 0; k + 1 < kRatioClasses; ++k) {
+            if (!tt[0].queries[k]) continue;
+            for (uint32_t i = 1; i < n; ++i)
+                if (T.class_ms[i][k] < T.class_ms[route[k]][k]) route[k] = i;
+            load[route[k]] += tt[0].queries[k];
+        }
+        route[kRatioClasses - 1] = (uint32_t)(std::max_element(load.begin(), load.end()) - load.begin());
+    }
+    // the whole sample on every candidate alone, then routed, timed alike
+    std::vector<uint32_t> solo(kRatioClasses);
+    for (uint32_t i = 0; i < n && !rc; ++i) {
+        std::fill(solo.begin(), solo.end(), i);
+        std::memcpy(r->route, solo.data(), sizeof r->route);
+        router_set_persistent(r, own);
+        if (!(rc = router_plan(r, qterms, qoff, nq))) rc = router_time(r, reps, &T.single_ms[i]);
+    }
+    std::memcpy(r->route, route, sizeof r->route);
+    T.routed_ms = 1e30f;
+    for (int p = 0; p < 2 && !rc; ++p) {
+        router_set_persistent(r, std::vector<int>(n, p));
+        float ms = 0;
+        if (!(rc = router_plan(r, qterms, qoff, nq)) && !(rc = router_time(r, reps, &ms)) && ms < T.routed_ms) {
+            T.routed_ms = ms;
+            T.persistent = (uint32_t)p;
+        }
+    }
+    if (!rc) {
+        const uint32_t best = (uint32_t)(std::min_element(T.single_ms, T.single_ms + n) - T.single_ms);
+        if (T.single_ms[best] <= T.routed_ms) {
+            for (auto& x : route) x = best;
+            router_set_persistent(r, own);
+            T.persistent = (uint32_t)own[best];
+        } else {
+            router_set_persistent(r, std::vector<int>(n, (int)T.persistent));
+        }
+        std::memcpy(r->route, route, sizeof r->route);
+        std::memcpy(T.route, route, sizeof T.route);
+    }
+    r->graph = had_graph;
+    r->state = 0;  // the last plan was the sample
+    router_drop_graph(r);
+    if (rc) return rc;
+    *out = T;
+    return EXC_OK;
+}
+
 // ---------------------------------------------------------------- serving
 
 // Continuous-batching request loop.  The compressed index stays resident; the

@@ -213,6 +213,52 @@ int exc_autotune(exc_ctx *ctx, exc_index *ix, const uint32_t *qterms, const uint
 /* Install a table (NULL: back to the plain options). */
 int exc_ctx_set_tuning(exc_ctx *ctx, const exc_tune_table *table);
 
+/* ---- routing one batch over candidate layouts of the same lists.  No
+ *      reference counterpart: the reference fixes the hash-bucket width
+ *      (UNI_LEN, src/gpu/Uncompressed/src/cuda_hashSegment_new/cuda_hashSegment.cu:42)
+ *      and the segment size (SegmentSize, GC/LRC/LRC_Compression/main.cpp:40)
+ *      at compile time, one program per choice.
+ * The candidates are indexes of the same lists (same term count and list
+ * lengths) under different layouts: segment sizes, bucket widths, codecs.  A
+ * router sends every query to the candidate chosen for its ratio class (the
+ * classes of exc_autotune).  Each candidate's sub-batch runs on a stream of its
+ * own; all are queued before any is waited for, and the routed batch is timed
+ * as one run, from a marker before the first kernel to one after the last.
+ * With EXC_OPT_GRAPH the whole routed run is one captured hipGraph.  Results
+ * come back in the caller's query order, as exc_intersect's.
+ * exc_router_autotune tunes every candidate's knobs on the sample
+ * (exc_autotune), routes each class to the candidate with the fastest class
+ * time, then times the whole sample routed (both launch modes) and the whole
+ * sample on each candidate alone, the same way; when one candidate alone is
+ * faster than the routed run, every class goes to it.  A new router routes
+ * every class to candidate 0. */
+#define EXC_ROUTER_MAX_CANDIDATES 8
+typedef struct exc_router exc_router;
+typedef struct {
+    uint32_t ncands;
+    uint32_t route[EXC_RATIO_CLASSES];                            /* candidate of each ratio class     */
+    float class_ms[EXC_ROUTER_MAX_CANDIDATES][EXC_RATIO_CLASSES]; /* class probe time, knobs tuned      */
+    float single_ms[EXC_ROUTER_MAX_CANDIDATES];                   /* whole sample on one candidate      */
+    float routed_ms;                                              /* whole sample routed by class_ms    */
+    uint32_t persistent;                                          /* launch mode of the routed kernels  */
+} exc_route_table;
+int exc_router_create(exc_ctx *ctx, exc_index *const *cands, uint32_t ncands, exc_router **out);
+int exc_router_autotune(exc_router *r, const uint32_t *qterms, const uint64_t *qoff, uint32_t nq,
+                        uint32_t reps, exc_route_table *out);
+/* route[c] = candidate of ratio class c, for all EXC_RATIO_CLASSES classes */
+int exc_router_set_routes(exc_router *r, const uint32_t *route);
+/* EXC_OPT_GRAPH is the router's own; every other option goes to each candidate's context */
+int exc_router_set_option(exc_router *r, int key, int64_t value);
+/* plan + upload (the stages of exc_plan), run the planned batch (ms = the
+ * run's device time, may be NULL), fetch its results */
+int exc_router_plan(exc_router *r, const uint32_t *qterms, const uint64_t *qoff, uint32_t nq);
+int exc_router_invoke(exc_router *r, float *ms);
+int exc_router_fetch(exc_router *r, uint32_t *ucntResult, uint32_t *result, uint64_t result_cap,
+                     uint64_t *n_results);
+int exc_router_intersect(exc_router *r, const uint32_t *qterms, const uint64_t *qoff, uint32_t nq,
+                         uint32_t *ucntResult, uint32_t *result, uint64_t result_cap, uint64_t *n_results);
+void exc_router_destroy(exc_router *r);
+
 /* ---- decompression programs (GC/<codec>/<codec>_Decompression):
  * decode whole lists into out (concatenated in the given order). */
 int exc_decompress(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, uint32_t nterms,
