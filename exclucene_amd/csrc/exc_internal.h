@@ -16,9 +16,11 @@ namespace exc {
 struct alignas(16) TermRec {
     uint64_t woff;    // word offset of the list's first block in ind2
     uint32_t n;       // list length (ind1.m_urlcount)
-    uint32_t segnum;  // first segment / bucket (.segNum)
-    uint32_t nseg;    // segments, or buckets (HS)
-    uint32_t aux;     // HS: index of start_0 in hash_info
+    uint32_t segnum;  // first segment record (.segNum; HS: first record of a non-empty bucket)
+    uint32_t nseg;    // segments (HS: non-empty buckets, one record each; RAW_HS: buckets)
+    uint32_t aux;     // HS: word offset of the list's bucket-occupancy table in
+                      // DevIndex::hash, or kDenseBuckets (every bucket holds docIDs);
+                      // RAW_HS: index of start_0 in .HashSeg_info
     float slope;      // LRC fSlope | SegLRC list-level slope
     int32_t icpt;     // LRC iIntercept | SegLRC list-level intercept bits | HS shift
     float range_l;    // LRC fSearchRange | SegLRC list-level nRangeLeft
@@ -33,6 +35,18 @@ struct alignas(16) TermRec {
 static_assert(sizeof(TermRec) == 64, "TermRec is one 64-byte record");
 
 constexpr uint32_t kNoModel = 0xffffffffu;
+
+// HS_SegLRC on the device keeps a record only for buckets that hold docIDs: a
+// fixed shift leaves most buckets of a short list empty, and a 32-byte record
+// per empty bucket is most of such an index.  A list with empty buckets has an
+// occupancy table [nb, 0, then per 32 buckets {bits, records before them}]
+// (8-byte aligned pairs), so bucket b's record is
+// segnum + prefix[b / 32] + popcount(bits[b / 32] & ((1 << b % 32) - 1)),
+// present iff bit b % 32 is set; a list whose buckets all hold docIDs has no
+// table (aux = kDenseBuckets) and record b is bucket b.  DevIndex::seg_head
+// holds each record's first element index, densely, for the chunk decoder.
+constexpr uint32_t kDenseBuckets = 0xffffffffu;
+constexpr uint32_t hs_occ_words(uint32_t nb) { return 2u + 2u * ((nb + 31u) / 32u); }
 
 // Learned position model of one list (docID -> position), fp32; inference in
 // exc_model.h.  err = max |prediction - true position| over the list's own
@@ -69,8 +83,8 @@ struct DevIndex {
     const SegRec* rec;
     const uint32_t* words;
     const TermRec* terms;
-    const uint32_t* seg_head;  // .segHead (dense, for the head searches)
-    const uint32_t* hash;      // .HashSeg_info
+    const uint32_t* seg_head;  // .segHead (dense, for the head searches); HS: first element of each record
+    const uint32_t* hash;      // RAW_HS: .HashSeg_info; HS: the bucket-occupancy tables
     const Model* models;
     const float* mparams;
     int32_t codec;
@@ -247,6 +261,29 @@ hipError_t launch_hs_lists(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_hs_fit(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_hs_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
+// HS_SegLRC: the per-bucket records and tables of the encoder compacted to the
+// device layout (records of non-empty buckets, occupancy tables; kDenseBuckets)
+struct HsCompactArgs {
+    const uint32_t* seg_num;  // nterms + 1: first bucket of each list, all buckets counted
+    const uint64_t* hs_off;   // per list: word offset of its [shift][nb][start_0 .. start_nb] table
+    const uint32_t* hash;     // those tables
+    const SegRec* rec;        // per bucket
+    uint32_t nterms;
+    uint64_t nbuckets;
+    uint32_t* flag;           // per bucket (+1): 1 if it holds docIDs
+    const uint64_t* cidx;     // per bucket (+1): records before it (exclusive scan of flag)
+    uint32_t* count;          // per list: records
+    const uint64_t* occ_off;  // per list: word offset of its occupancy table, ~0 for a dense list
+    const uint64_t* grp_off;  // per list (+1): occupancy groups of 32 buckets before it
+    uint64_t ngroups;
+    SegRec* crec;
+    uint32_t* chead;
+    uint32_t* occ;
+    TermRec* terms;
+};
+hipError_t launch_hs_compact_flags(const HsCompactArgs& a, hipStream_t s);  // flag, then count after the scan
+hipError_t launch_hs_compact_counts(const HsCompactArgs& a, hipStream_t s);
+hipError_t launch_hs_compact(const HsCompactArgs& a, hipStream_t s);        // records, heads, tables, TermRecs
 hipError_t scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint64_t n, hipStream_t s);
 int built_from_device(int codec, uint32_t seg, uint32_t nterms, const TermRec* terms, uint64_t nseg, const SegRec* rec,
                       const uint32_t* head, const uint32_t* words, uint64_t nwords, const uint32_t* hash,

@@ -485,6 +485,81 @@ __global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
     a.terms[t] = R;
 }
 
+// ---------------------------------------------------------------- HS_SegLRC compaction
+
+// Thread per bucket (and one past the last): 1 if the bucket holds docIDs.
+__global__ __launch_bounds__(256) void k_hsc_flags(HsCompactArgs a)
+{
+    const uint64_t g64 = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g64 > a.nbuckets) return;
+    if (g64 == a.nbuckets) {
+        a.flag[g64] = 0;
+        return;
+    }
+    const uint32_t g = (uint32_t)g64;
+    const uint32_t t = term_of_segment(a.seg_num, a.nterms, g);
+    const uint32_t b = g - a.seg_num[t];
+    const uint32_t* tab = a.hash + a.hs_off[t];
+    a.flag[g] = tab[3 + b] > tab[2 + b] ? 1u : 0u;
+}
+
+// Thread per list: its records (non-empty buckets).
+__global__ __launch_bounds__(256) void k_hsc_counts(HsCompactArgs a)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.nterms) return;
+    a.count[t] = (uint32_t)(a.cidx[a.seg_num[t + 1]] - a.cidx[a.seg_num[t]]);
+}
+
+// Thread per bucket: a non-empty bucket's record and first element to its place.
+__global__ __launch_bounds__(256) void k_hsc_recs(HsCompactArgs a)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= a.nbuckets || !a.flag[g]) return;
+    const SegRec q = a.rec[g];
+    const uint64_t c = a.cidx[g];
+    a.crec[c] = q;
+    a.chead[c] = q.head;
+}
+
+// Thread per occupancy group (32 buckets of a list with empty buckets): its
+// bit word and the records before it; group 0 also writes the header.
+__global__ __launch_bounds__(256) void k_hsc_occ(HsCompactArgs a)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.ngroups) return;
+    uint32_t lo = 0, hi = a.nterms;  // last list whose groups start at or before j
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (a.grp_off[m] <= j) lo = m; else hi = m;
+    }
+    const uint32_t t = lo;
+    const uint32_t grp = (uint32_t)(j - a.grp_off[t]);
+    const uint32_t nb = a.hash[a.hs_off[t] + 1];
+    const uint64_t g0 = (uint64_t)a.seg_num[t] + 32ull * grp;
+    uint32_t bits = 0;
+    for (uint32_t i = 0; i < 32 && 32 * grp + i < nb; ++i) bits |= a.flag[g0 + i] << i;
+    uint32_t* o = a.occ + a.occ_off[t];
+    if (grp == 0) {
+        o[0] = nb;
+        o[1] = 0;
+    }
+    o[2 + 2 * grp] = bits;
+    o[3 + 2 * grp] = (uint32_t)(a.cidx[g0] - a.cidx[a.seg_num[t]]);
+}
+
+// Thread per list: the record fields of the compact layout.
+__global__ __launch_bounds__(256) void k_hsc_terms(HsCompactArgs a)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.nterms) return;
+    TermRec R = a.terms[t];
+    R.segnum = (uint32_t)a.cidx[a.seg_num[t]];
+    R.nseg = a.count[t];
+    R.aux = a.occ_off[t] == ~0ull ? kDenseBuckets : (uint32_t)a.occ_off[t];
+    a.terms[t] = R;
+}
+
 // ---------------------------------------------------------------- scan (u64)
 
 constexpr uint32_t kScanItems = 4096;  // 256 threads x 16
@@ -616,6 +691,27 @@ hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s)
 {
     if (!a.nterms) return hipSuccess;
     hipLaunchKernelGGL(k_term_recs, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hs_compact_flags(const HsCompactArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hsc_flags, dim3((uint32_t)((a.nbuckets + 256) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hs_compact_counts(const HsCompactArgs& a, hipStream_t s)
+{
+    if (a.nterms) hipLaunchKernelGGL(k_hsc_counts, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hs_compact(const HsCompactArgs& a, hipStream_t s)
+{
+    if (a.nbuckets)
+        hipLaunchKernelGGL(k_hsc_recs, dim3((uint32_t)((a.nbuckets + 255) / 256)), dim3(256), 0, s, a);
+    if (a.ngroups) hipLaunchKernelGGL(k_hsc_occ, dim3((uint32_t)((a.ngroups + 255) / 256)), dim3(256), 0, s, a);
+    if (a.nterms) hipLaunchKernelGGL(k_hsc_terms, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

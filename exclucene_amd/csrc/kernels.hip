@@ -1164,8 +1164,8 @@ __device__ uint32_t probe_raw_hs(const ProbeArgs& A, const TermRec& R, const Lds
 }
 
 // HS_SegLRC: decode elements [k*CH, k*CH+CH) of list R into dst; returns count.
-// Bucket of each element = last bucket start <= its index
-// (HS_SegLRC_Intersection_kernel.cu:274-327).
+// Record of each element = last record whose first element is <= its index
+// (HS_SegLRC_Intersection_kernel.cu:274-327, over the non-empty buckets only).
 template <int CH>
 __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32_t k, uint32_t* dst)
 {
@@ -1173,10 +1173,10 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
     const uint32_t ln = lane_id();
     const uint32_t e0 = k * CH;
     const uint32_t clen = uni(min((uint32_t)CH, R.n - e0));
-    const uint32_t* starts = ix.hash + R.aux;
-    const uint32_t nb = R.nseg;
-    const uint32_t b0 = uni(wave_upper_bound(starts, nb + 1, e0) - 1);
-    const uint32_t hreg = (b0 + 1 + ln <= nb) ? starts[b0 + 1 + ln] : kNone;
+    const uint32_t* heads = ix.seg_head + R.segnum;
+    const uint32_t nr = R.nseg;
+    const uint32_t r0 = uni(wave_upper_bound(heads, nr, e0) - 1);
+    const uint32_t hreg = (r0 + 1 + ln < nr) ? heads[r0 + 1 + ln] : kNone;
     const uint32_t h63 = uni(__builtin_amdgcn_readlane(hreg, 63));
 #pragma unroll
     for (int i = 0; i < V; ++i) {
@@ -1184,12 +1184,12 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
         const uint32_t e = e0 + j;
         const uint32_t pos = lane_count_le(hreg, h63, e);
         if (j < clen) {
-            uint32_t b = b0 + pos;
+            uint32_t b = r0 + pos;
             if (pos == 64) {
-                uint32_t lo = b0 + 65, hi = nb + 1;
+                uint32_t lo = r0 + 65, hi = nr;
                 while (lo < hi) {
                     const uint32_t m = (lo + hi) >> 1;
-                    if (starts[m] <= e) lo = m + 1; else hi = m;
+                    if (heads[m] <= e) lo = m + 1; else hi = m;
                 }
                 b = lo - 1;
             }
@@ -1201,6 +1201,54 @@ __device__ uint32_t decode_hs_chunk(const DevIndex& ix, const TermRec& R, uint32
     return clen;
 }
 
+// Record of the bucket holding docID v in an HS_SegLRC list, and the number
+// of elements in it; false when v is past the last bucket or its bucket is
+// empty.  A dense list's record is its bucket; otherwise the occupancy pair of
+// the bucket's group of 32 gives presence and rank.
+struct HsList {
+    const SegRec* rec;
+    const uint32_t* heads;
+    const uint2* occ;  // null for a dense list
+    uint32_t shift, nb, nr, n;
+};
+
+__device__ __forceinline__ HsList hs_list(const DevIndex& ix, const TermRec& R)
+{
+    HsList h;
+    h.rec = ix.rec + R.segnum;
+    h.heads = ix.seg_head + R.segnum;
+    h.shift = (uint32_t)R.icpt;
+    h.nr = R.nseg;
+    h.n = R.n;
+    h.occ = nullptr;
+    h.nb = h.nr;
+    const uint32_t aux = uni(R.aux);
+    if (aux != kDenseBuckets) {
+        h.occ = reinterpret_cast<const uint2*>(ix.hash + aux + 2);
+        h.nb = uni(ix.hash[aux]);
+    }
+    return h;
+}
+
+__device__ __forceinline__ bool hs_record(const HsList& h, uint32_t v, uint32_t& r)
+{
+    const uint32_t b = v >> h.shift;
+    if (b >= h.nb) return false;
+    r = b;
+    if (h.occ) {
+        const uint2 p = h.occ[b >> 5];
+        const uint32_t bit = b & 31u;
+        if (!((p.x >> bit) & 1u)) return false;
+        r = p.y + popc(p.x & ((1u << bit) - 1u));
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint32_t hs_len(const HsList& h, uint32_t r, uint32_t head)
+{
+    return (r + 1 < h.nr ? h.heads[r + 1] : h.n) - head;
+}
+
 // HS_SegLRC probe: O(1) bucket, fit window, exact fallback
 // (HS_SegLRC_Intersection_kernel.cu:334-401 plus the fallback it lacks).
 __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
@@ -1208,10 +1256,10 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
     const uint32_t search = knob_search(knobs);
     constexpr int V = kV;
     const uint32_t ln = lane_id();
-    const uint32_t shift = (uint32_t)R.icpt, nb = R.nseg;
-    const uint32_t* starts = A.ix.hash + R.aux;
+    const HsList H = hs_list(A.ix, R);
+    const uint32_t nb = H.nb;
     uint32_t fmask = 0;
-    if (R.n > 512u * nb) {
+    if ((uint64_t)R.n > 512ull * nb) {
         // large buckets (fixed shifts on long lists): their fits are too loose
         // for a one-load window; two candidates per lane in flight, each
         // predicting its position and walking / binary-searching in HBM
@@ -1224,12 +1272,12 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
             for (int x = 0; x < 2; ++x) {
                 const uint32_t e = ln + 64 * (i + x);
                 vv[x] = e < nlive ? W.cand[e] : kNone;
-                const uint32_t b = vv[x] >> shift;
-                ok[x] = e < nlive && b < nb;
+                uint32_t r = 0;
+                ok[x] = e < nlive && hs_record(H, vv[x], r);
                 len[x] = 0;
                 if (ok[x]) {
-                    rec[x] = A.ix.rec[R.segnum + b];
-                    len[x] = starts[b + 1] - rec[x].head;
+                    rec[x] = H.rec[r];
+                    len[x] = hs_len(H, r, rec[x].head);
                 }
             }
 #pragma unroll
@@ -1243,12 +1291,11 @@ __device__ uint32_t probe_hs(const ProbeArgs& A, const TermRec& R, const LdsWave
         const uint32_t e = ln + 64u * (uint32_t)i;
         if (!__ballot(e < nlive)) break;  // rows fill in order
         const uint32_t vi = e < nlive ? W.cand[e] : kNone;
-        const uint32_t b = vi >> shift;
-        const bool inb = e < nlive && b < nb;
+        uint32_t r = 0;
+        const bool inb = e < nlive && hs_record(H, vi, r);
         if (!__ballot(inb)) continue;
-        const uint32_t bb = inb ? b : 0u;
-        const SegRec rc = A.ix.rec[R.segnum + bb];
-        const uint32_t len = starts[bb + 1] - rc.head;
+        const SegRec rc = H.rec[r];
+        const uint32_t len = inb ? hs_len(H, r, rc.head) : 0u;
         const bool f = win_search(W.buf, A.ix.words + R.woff + rc.off + 1, len, rc, vi, inb && len > 0, search);
         if (f) fmask |= 1u << i;
     }

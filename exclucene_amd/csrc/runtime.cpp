@@ -249,6 +249,52 @@ uint32_t block_need_words(const uint32_t* blk, uint32_t len)
     return 1 + words_for(len, lb) + (en ? words_for(en, eb) + words_for(en, hb) : 0);
 }
 
+// The reference's per-bucket HS_SegLRC arrays from the device's compact
+// layout (records of non-empty buckets, occupancy tables): every bucket's
+// record, an empty one as the host builder writes it (fit {1, 0, 0, 0}, base
+// 0, the offset and first element of the next non-empty bucket, or the
+// list's block words and length after the last), the tables
+// [shift][nb][start_0 .. start_nb] and terms that point at them.
+void hs_expand(const std::vector<TermRec>& terms, const std::vector<SegRec>& crec, const std::vector<uint32_t>& occ,
+               const std::vector<uint32_t>& words, std::vector<TermRec>& fterms, std::vector<SegRec>& frec,
+               std::vector<uint32_t>& table)
+{
+    fterms = terms;
+    frec.clear();
+    table.clear();
+    for (size_t t = 0; t < terms.size(); ++t) {
+        const TermRec& R = terms[t];
+        const bool dense = R.aux == kDenseBuckets;
+        const uint32_t nb = dense ? R.nseg : occ[R.aux];
+        const uint64_t first = frec.size();
+        auto present = [&](uint32_t b) { return dense || ((occ[R.aux + 2 + 2 * (b / 32)] >> (b % 32)) & 1u); };
+        uint32_t r = 0;
+        for (uint32_t b = 0; b < nb; ++b) frec.push_back(present(b) ? crec[R.segnum + r++] : SegRec{});
+        uint32_t next_off = 0, next_head = R.n;
+        if (R.nseg) {  // after the last block: the list's block words
+            const SegRec& q = crec[R.segnum + R.nseg - 1];
+            next_off = q.off + block_need_words(words.data() + R.woff + q.off, R.n - q.head);
+        }
+        for (uint32_t b = nb; b-- > 0;) {
+            SegRec& q = frec[first + b];
+            if (present(b)) {
+                next_off = q.off;
+                next_head = q.head;
+            } else {
+                q = SegRec{next_off, 0u, 1.0f, 0.0f, 0u, 0u, 0, next_head};
+            }
+        }
+        TermRec& F = fterms[t];
+        F.segnum = (uint32_t)first;
+        F.nseg = nb;
+        F.aux = (uint32_t)table.size() + 2;
+        table.push_back((uint32_t)R.icpt);
+        table.push_back(nb);
+        for (uint32_t b = 0; b < nb; ++b) table.push_back(frec[first + b].head);
+        table.push_back(R.n);
+    }
+}
+
 // Words of a NewPFor block of len values whose words are blk[0 .. avail), or 0
 // if it is malformed: its Simple16 stream must end inside avail, its
 // exception positions must lie inside the block and its descriptor must have
@@ -542,22 +588,62 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
         if (codec == EXC_LRC && !(R.slope > 0.0f)) { R.slope = 1.0f; R.range_l = R.range_r = 4.0e9f; }
     }
 
-    // one 32-byte record per segment / bucket: offset, descriptor, fit, base, head
-    std::vector<SegRec> srec(raw ? 0 : h->nseg);
-    for (uint32_t t = 0; t < nt && !raw; ++t) {  // raw lists have no block records
+    // one 32-byte record per segment / non-empty bucket: offset, descriptor,
+    // fit, base, head; HS lists with empty buckets get an occupancy table
+    std::vector<SegRec> srec(raw || hs ? 0 : h->nseg);
+    std::vector<uint32_t> chead, occ;
+    for (uint32_t t = 0; t < nt && hs; ++t) {
+        TermRec& R = recs[t];
+        const uint32_t* ht = h->hash_info + h->hash_offset[t] + 2;
+        const uint32_t nb = R.nseg;
+        const uint64_t first = srec.size();
+        std::vector<uint32_t> bits((nb + 31) / 32, 0u);
+        for (uint32_t b = 0; b < nb; ++b) {
+            if (ht[b + 1] == ht[b]) continue;
+            bits[b / 32] |= 1u << (b % 32);
+            const uint64_t g = (uint64_t)R.segnum + b;
+            SegRec q;
+            q.off = h->seg_offset[g];
+            q.desc = h->ind2[R.woff + q.off];
+            q.slope = h->seg_info[g].fSlope;
+            q.icpt = h->seg_info[g].fIntercept;
+            q.rl = h->seg_info[g].nRangeLeft;
+            q.rr = h->seg_info[g].nRangeRight;
+            q.base = h->seg_base[g];
+            if (!(q.slope > 0.0f)) { q.slope = 1.0f; q.rl = q.rr = 0x7fffffffu; }
+            q.head = ht[b];
+            srec.push_back(q);
+            chead.push_back(ht[b]);
+        }
+        const uint64_t nr = srec.size() - first;
+        if (srec.size() > 0xffffffffull || occ.size() + hs_occ_words(nb) > 0xfffffffeull)
+            return set_error(EXC_ERR_CAPACITY, "HS index: more than 2^32 records or table words");
+        R.segnum = (uint32_t)first;
+        R.nseg = (uint32_t)nr;
+        R.aux = kDenseBuckets;
+        if (nr == nb) continue;
+        R.aux = (uint32_t)occ.size();
+        occ.push_back(nb);
+        occ.push_back(0);
+        uint32_t before = 0;
+        for (uint32_t w : bits) {
+            occ.push_back(w);
+            occ.push_back(before);
+            before += (uint32_t)__builtin_popcount(w);
+        }
+    }
+    for (uint32_t t = 0; t < nt && !raw && !hs; ++t) {  // raw lists have no block records
         const TermRec& R = recs[t];
-        const uint32_t* ht = hs ? h->hash_info + h->hash_offset[t] + 2 : nullptr;
         for (uint32_t s = 0; s < R.nseg; ++s) {
             const uint64_t g = (uint64_t)R.segnum + s;
             SegRec& q = srec[g];
             q.off = h->seg_offset[g];
-            const bool empty = hs ? ht[s + 1] == ht[s] : false;
-            q.desc = empty ? 0u : h->ind2[R.woff + q.off];
+            q.desc = h->ind2[R.woff + q.off];
             q.slope = 1.0f;
             q.icpt = 0.0f;
             q.rl = q.rr = 0;
             q.base = 0;
-            if (codec == EXC_SEGLRC || hs) {
+            if (codec == EXC_SEGLRC) {
                 q.slope = h->seg_info[g].fSlope;
                 q.icpt = h->seg_info[g].fIntercept;
                 q.rl = h->seg_info[g].nRangeLeft;
@@ -565,7 +651,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
                 q.base = h->seg_base[g];
                 if (!(q.slope > 0.0f)) { q.slope = 1.0f; q.rl = q.rr = 0x7fffffffu; }
             }
-            q.head = hs ? ht[s] : h->seg_head[g];
+            q.head = h->seg_head[g];
             // ParaPFD keeps the block's .segMedian row in rl (no fit ranges)
             if (codec == EXC_PARAPFD && h->seg_median) q.rl = h->seg_median[g];
         }
@@ -576,9 +662,9 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->codec = codec;
     ix->seg = hs ? 128 : seg;
     ix->nterms = nt;
-    ix->nseg = h->nseg;
+    ix->nseg = hs ? srec.size() : h->nseg;
     ix->nwords = W;
-    ix->hash_words = (hs || raw_hs) ? h->hash_words : 0;
+    ix->hash_words = hs ? occ.size() : raw_hs ? h->hash_words : 0;
     ix->n.resize(nt);
     for (uint32_t t = 0; t < nt; ++t) ix->n[t] = recs[t].n;
     HIP_TRY(hipSetDevice(c->device));
@@ -596,8 +682,13 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->d_terms = const_cast<TermRec*>(ix->dix.terms);
     ix->recs = recs;
     ix->dix.rec = (const SegRec*)up(srec.data(), srec.size() * sizeof(SegRec));
-    if (!hs) ix->dix.seg_head = (const uint32_t*)up(h->seg_head, h->nseg * 4);
-    if (hs || raw_hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
+    if (hs) {
+        ix->dix.seg_head = (const uint32_t*)up(chead.data(), chead.size() * 4);
+        ix->dix.hash = (const uint32_t*)up(occ.data(), occ.size() * 4);
+    } else {
+        ix->dix.seg_head = (const uint32_t*)up(h->seg_head, h->nseg * 4);
+        if (raw_hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
+    }
     if (rc) {
         exc_index_free(ix);
         return rc;
@@ -1908,24 +1999,27 @@ int build_hs_on_device(exc_ctx* c, uint32_t hs_target, uint32_t hs_shift, uint32
         exc_index_free(ix);
         return r;
     };
-    uint32_t *d_snum, *d_nw;
+    uint32_t *d_snum, *d_nw, *p_hash;
     uint64_t *d_goff, *d_hoff;
-    void *p_rec, *p_hash, *p_terms;
+    SegRec* p_rec;
+    void* p_terms;
+    // every bucket's record and the tables are temporaries: the index keeps
+    // the compact layout made from them below
     if ((rc = tmp.get(nterms + 1, &d_snum)) || (rc = tmp.get(S + 1, &d_nw)) || (rc = tmp.get(S + 1, &d_goff)) ||
-        (rc = tmp.get(nterms + 1, &d_hoff)) || (rc = alloc_padded(ix, S * sizeof(SegRec), &p_rec)) ||
-        (rc = alloc_padded(ix, H * 4, &p_hash)) || (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)))
+        (rc = tmp.get(nterms + 1, &d_hoff)) || (rc = tmp.get(S, &p_rec)) || (rc = tmp.get(H, &p_hash)) ||
+        (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)))
         return fail(rc);
     hipError_t e = hipMemcpyAsync(d_snum, seg_num.data(), (nterms + 1ull) * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_hoff, hoff.data(), (nterms + 1ull) * 8, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d_nw + S, 0, 4, c->stream);
     if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
     a.seg_num = d_snum;
-    a.rec = static_cast<SegRec*>(p_rec);
+    a.rec = p_rec;
     a.nwords = d_nw;
     a.goff = d_goff;
     a.terms = static_cast<TermRec*>(p_terms);
     a.hs_off = d_hoff;
-    a.hash = static_cast<uint32_t*>(p_hash);
+    a.hash = p_hash;
     // pass 2: tables, fits, bases, block sizes
     for (auto [t0, t1] : batches) {
         const uint32_t* docs;
@@ -1956,13 +2050,68 @@ int build_hs_on_device(exc_ctx* c, uint32_t hs_target, uint32_t hs_shift, uint32
         a.g1 = seg_num[t1];
         if ((e = launch_hs_pack(a, c->stream)) != hipSuccess) break;
     }
+    // compaction: records of the non-empty buckets, occupancy tables of the
+    // lists that have empty buckets
+    HsCompactArgs k;
+    std::memset(&k, 0, sizeof k);
+    k.seg_num = d_snum;
+    k.hs_off = d_hoff;
+    k.hash = p_hash;
+    k.rec = p_rec;
+    k.nterms = nterms;
+    k.nbuckets = S;
+    uint32_t *d_flag, *d_count;
+    uint64_t *d_cidx, *d_occoff, *d_grpoff;
+    if ((rc = tmp.get(S + 1, &d_flag)) || (rc = tmp.get(S + 1, &d_cidx)) ||
+        (rc = tmp.get(std::max<uint32_t>(nterms, 1), &d_count)) || (rc = tmp.get(std::max<uint32_t>(nterms, 1), &d_occoff)) ||
+        (rc = tmp.get(nterms + 1, &d_grpoff)))
+        return fail(rc);
+    k.flag = d_flag;
+    k.cidx = d_cidx;
+    k.count = d_count;
+    std::vector<uint32_t> count(nterms);
+    uint64_t C = 0;
+    if (e == hipSuccess) e = launch_hs_compact_flags(k, c->stream);
+    if (e == hipSuccess) e = scan_u32_to_u64(d_flag, d_cidx, S + 1, c->stream);
+    if (e == hipSuccess) e = launch_hs_compact_counts(k, c->stream);
+    if (e == hipSuccess && nterms) e = hipMemcpyAsync(count.data(), d_count, nterms * 4ull, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&C, d_cidx + S, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
+    std::vector<uint64_t> occ_off(std::max<uint32_t>(nterms, 1), ~0ull), grp_off(nterms + 1, 0);
+    uint64_t O = 0;
+    for (uint32_t t = 0; t < nterms; ++t) {
+        const bool dense = count[t] == nb[t];
+        grp_off[t + 1] = grp_off[t] + (dense ? 0u : (nb[t] + 31) / 32);
+        if (!dense) {
+            occ_off[t] = O;
+            O += hs_occ_words(nb[t]);
+        }
+    }
+    if (C > 0xffffffffull || O > 0xfffffffeull)
+        return fail(set_error(EXC_ERR_CAPACITY, "more than 2^32 records or table words"));
+    void *p_crec, *p_chead, *p_occ;
+    if ((rc = alloc_padded(ix, C * sizeof(SegRec), &p_crec)) || (rc = alloc_padded(ix, C * 4, &p_chead)) ||
+        (rc = alloc_padded(ix, O * 4, &p_occ)))
+        return fail(rc);
+    k.occ_off = d_occoff;
+    k.grp_off = d_grpoff;
+    k.ngroups = grp_off[nterms];
+    k.crec = static_cast<SegRec*>(p_crec);
+    k.chead = static_cast<uint32_t*>(p_chead);
+    k.occ = static_cast<uint32_t*>(p_occ);
+    k.terms = a.terms;
+    e = hipMemcpyAsync(d_occoff, occ_off.data(), occ_off.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_grpoff, grp_off.data(), grp_off.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_hs_compact(k, c->stream);
     ix->recs.resize(nterms);
     if (e == hipSuccess && nterms)
         e = hipMemcpyAsync(ix->recs.data(), p_terms, (size_t)nterms * sizeof(TermRec), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(set_error(EXC_ERR_DEVICE, "device build: %s", hipGetErrorString(e)));
     ix->nwords = W;
-    ix->hash_words = H;
+    ix->nseg = C;
+    ix->hash_words = O;
     ix->n.resize(nterms);
     for (uint32_t t = 0; t < nterms; ++t) ix->n[t] = ix->recs[t].n;
     ix->dix.codec = EXC_HS_SEGLRC;
@@ -1970,8 +2119,9 @@ int build_hs_on_device(exc_ctx* c, uint32_t hs_target, uint32_t hs_shift, uint32
     ix->dix.words = a.words;
     ix->dix.terms = a.terms;
     ix->d_terms = a.terms;
-    ix->dix.rec = a.rec;
-    ix->dix.hash = a.hash;
+    ix->dix.rec = k.crec;
+    ix->dix.seg_head = k.chead;
+    ix->dix.hash = k.occ;
     *out = ix;
     return EXC_OK;
 }
@@ -2058,16 +2208,23 @@ extern "C" int exc_index_export(exc_ctx* c, exc_index* ix, exc_built** out)
     HIP_TRY(hipSetDevice(c->device));
     const bool hs = ix->codec == EXC_HS_SEGLRC;
     std::vector<SegRec> rec(ix->nseg);
-    std::vector<uint32_t> head(hs ? 0 : ix->nseg), words(ix->nwords), hash(hs ? ix->hash_words : 0);
+    std::vector<uint32_t> head(hs ? 0 : ix->nseg), words(ix->nwords), occ(hs ? ix->hash_words : 0);
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (ix->nseg) {
         HIP_TRY(hipMemcpy(rec.data(), ix->dix.rec, ix->nseg * sizeof(SegRec), hipMemcpyDeviceToHost));
         if (!hs) HIP_TRY(hipMemcpy(head.data(), ix->dix.seg_head, ix->nseg * 4, hipMemcpyDeviceToHost));
     }
     if (ix->nwords) HIP_TRY(hipMemcpy(words.data(), ix->dix.words, ix->nwords * 4, hipMemcpyDeviceToHost));
-    if (!hash.empty()) HIP_TRY(hipMemcpy(hash.data(), ix->dix.hash, hash.size() * 4, hipMemcpyDeviceToHost));
-    return built_from_device(ix->codec, ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(),
-                             hs ? nullptr : head.data(), words.data(), ix->nwords, hash.data(), hash.size(), out);
+    if (!occ.empty()) HIP_TRY(hipMemcpy(occ.data(), ix->dix.hash, occ.size() * 4, hipMemcpyDeviceToHost));
+    if (!hs)
+        return built_from_device(ix->codec, ix->seg, ix->nterms, ix->recs.data(), ix->nseg, rec.data(), head.data(),
+                                 words.data(), ix->nwords, nullptr, 0, out);
+    std::vector<TermRec> fterms;
+    std::vector<SegRec> frec;
+    std::vector<uint32_t> table;
+    hs_expand(ix->recs, rec, occ, words, fterms, frec, table);
+    return built_from_device(ix->codec, ix->seg, ix->nterms, fterms.data(), frec.size(), frec.data(), nullptr,
+                             words.data(), ix->nwords, table.data(), table.size(), out);
 }
 
 // ---------------------------------------------------------------- autotuner

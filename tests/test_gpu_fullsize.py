@@ -109,6 +109,10 @@ def test_c3_full_size_hash_buckets(ctx, c2, shift):
     ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
     ocum = np.asarray(ocum, np.int64)
     dix = ctx.upload(built)
+    if shift == 13:
+        # records only for buckets that hold docIDs: 4.47 GB on the device,
+        # against 15.5 GB with a record for every bucket
+        assert dix.device_bytes() < 5.0e9, dix.device_bytes()
     try:
         for persistent in (0, 1):
             ctx.set_option(ea.OPT_PERSISTENT, persistent)
@@ -131,6 +135,37 @@ def test_c3_full_size_hash_buckets(ctx, c2, shift):
         ctx.set_option(ea.OPT_SEARCH, 1)
         ctx.set_option(ea.OPT_PERSISTENT, 1)
         dix.close()
+
+
+def test_c3_full_size_routed(ctx, c2):
+    """C3 with the bucket width chosen per ratio class: the router over the
+    full-size lists in buckets of shift 13, of the adaptive width and in
+    SegLRC-128 segments, routes tuned on a 2,000-query sample, all 10,000
+    ratio queries run as one captured graph of the routed sub-batches (two
+    replays), every docID equal to the oracle's."""
+    docs, off, _, built, _, _, df = c2
+    queries = W.ratio_queries(df, 10_000, 3)
+    ocum, ores = oracle.eval_queries(_oracle_index(built), queries, nthreads=_threads())
+    ocum = np.asarray(ocum, np.int64)
+    hs = [ea.build_index(docs, off, ea.HS_SEGLRC, seg=128, hs_shift=sh) for sh in (13, 0)]
+    dixs = [ctx.upload(b) for b in hs] + [ctx.upload(built)]
+    router = ea.Router(ctx, dixs)
+    try:
+        t = router.autotune(W.ratio_queries(df, 2000, 33), reps=1)
+        print("routes", t["route"], "routed_ms", t["routed_ms"], "single_ms", t["single_ms"], flush=True)
+        router.set_option(ea.OPT_GRAPH, 1)
+        router.plan(queries)
+        for rep in range(2):
+            router.invoke()
+            cum, res = router.fetch()
+            assert np.array_equal(np.asarray(cum, np.int64), ocum), rep
+            assert np.array_equal(res, ores), rep
+    finally:
+        router.close()
+        for d in dixs:
+            d.close()
+        for b in hs:
+            b.close()
 
 
 @pytest.mark.parametrize("shift", [13, 0])
