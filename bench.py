@@ -857,13 +857,17 @@ def run_c2(args, ea, W, ctx):
 
     if not args.no_raw:
         # the C2 lists uncompressed (the reference's src/gpu/Uncompressed programs):
-        # binary search (mqSearch) and interpolation search, one wave per unit
+        # binary search (mqSearch), the lists' regression window
+        # (cuda_runRegressionAlgorithm), their PLA models' window and
+        # interpolation search, one wave per unit
         rb = ea.build_index(docs, off, ea.RAW)
         rix = ctx.upload(rb)
         ctx.set_tuning(None)
         extra["uncompressed"] = {"device_bytes": int(rix.device_bytes())}
+        extra["uncompressed"]["models"] = ctx.train_models(rix, kind=ea.MODEL_PLA, min_len=65536)
         ralg, _ = algorithmic_bytes(rb, docs, off, queries, threads=CPU_SHARE)
-        for mode, name in ((ea.SEARCH_BINARY, "binary"), (1, "interpolation")):
+        for mode, name in ((ea.SEARCH_BINARY, "binary"), (ea.SEARCH_FIT, "regression"),
+                           (ea.SEARCH_LEARNED, "learned"), (ea.SEARCH_INTERP, "interpolation")):
             ctx.set_option(ea.OPT_SEARCH, mode)
             ctx.set_option(ea.OPT_PERSISTENT, 0)
             rdt, rprobe, _ = time_batch(ctx, rix, queries, reps_other, 1)

@@ -22,7 +22,7 @@ CODECS = {"LRC": LRC, "SegLRC": SEGLRC, "HS_SegLRC": HS_SEGLRC, "ParaPFD": PARAP
           "uncompressed": RAW, "uncompressed_hash_segment": RAW_HS}
 
 OPT_SEARCH, OPT_TPE, OPT_PERSISTENT, OPT_GRAPH, OPT_TIMING, OPT_COOP_MAX = 0, 1, 2, 3, 4, 5
-SEARCH_BINARY, SEARCH_FIT, SEARCH_LEARNED = 0, 1, 2
+SEARCH_BINARY, SEARCH_FIT, SEARCH_LEARNED, SEARCH_INTERP = 0, 1, 2, 3
 
 EXPORTS = [
     "exc_ctx_create", "exc_ctx_destroy", "exc_last_error", "exc_ctx_set_option", "exc_ctx_last_times",
@@ -54,7 +54,7 @@ class IndexHost(ctypes.Structure):
                 ("seg_num", _vp), ("seg_offset", _vp), ("nseg", _u64),
                 ("seg_head", _vp), ("seg_median", _vp), ("lrc_info", _vp), ("seg_info", _vp),
                 ("seg_base", _vp), ("list_info", _vp), ("hash_offset", _vp), ("hash_info", _vp),
-                ("hash_words", _u64)]
+                ("hash_words", _u64), ("gu_regression", _vp)]
 
 
 RATIO_CLASSES = 6
@@ -117,6 +117,9 @@ class CheckSum(ctypes.Structure):
 IND1 = np.dtype([("m_urlcount", "<u4"), ("pad", "<u4"), ("m_off", "<u8")])
 LRC_INFO = np.dtype([("fSlope", "<f4"), ("iIntercept", "<i4"), ("fSearchRange", "<f4")])
 SEG_INFO = np.dtype([("fSlope", "<f4"), ("fIntercept", "<f4"), ("nRangeLeft", "<u4"), ("nRangeRight", "<u4")])
+GU_REG_INFO = np.dtype([("fSlope", "<f4"), ("fIntercept", "<f4"), ("fRSquare", "<f4"), ("fMultiple", "<f4"),
+                        ("fInterceptHigh", "<f4"), ("fInterceptLow", "<f4"), ("nRangeLeft", "<u4"),
+                        ("nRangeRight", "<u4"), ("nMagnification", "<u4")])
 
 _lib = None
 
@@ -255,6 +258,7 @@ class BuiltIndex:
         self.list_info = _arr(v.list_info, nt, SEG_INFO)
         self.hash_offset = _arr(v.hash_offset, nt, np.uint32)
         self.hash_info = _arr(v.hash_info, v.hash_words, np.uint32)
+        self.gu_regression = _arr(v.gu_regression, nt, GU_REG_INFO)
 
     @property
     def ucnt(self):

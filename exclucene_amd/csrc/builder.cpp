@@ -35,6 +35,8 @@
 
 using namespace exc;
 
+static_assert(sizeof(exc_gu_regression_info) == 36, "at_regression_info rows are 36 bytes");
+
 struct exc_built {
     int32_t codec = 0;
     uint32_t seg = 256;
@@ -45,6 +47,7 @@ struct exc_built {
     std::vector<exc_seg_info> seg_info, list_info;
     std::vector<int32_t> base;
     std::vector<uint32_t> hash_offset, hash_info;
+    std::vector<exc_gu_regression_info> gu_reg;
 };
 
 namespace {
@@ -221,6 +224,51 @@ void gu_hash_table(const uint32_t* L, uint32_t n, uint32_t target, uint32_t fixe
     t[1 + nb] = n;
 }
 
+// The regression row of one uncompressed list, as getRegressionInfo and
+// RunGetRegressionInfo compute it (GU/cuda_runRegressionAlgorithm/
+// cuda_linearRegression.cu:792-873, 900-925; x magnification 1): a
+// one-element list gets slope 1 and its docID as intercept, every other
+// list (the empty one included, whose line is NaN) the least-squares line
+// with means accumulated term by term in double, slope and intercept stored
+// as float, and the ranges from each element's float position on the line.
+exc_gu_regression_info gu_regression(const uint32_t* L, uint32_t n)
+{
+    exc_gu_regression_info r;
+    std::memset(&r, 0, sizeof r);
+    if (n == 1) {
+        r.fSlope = 1.0f;
+        r.fIntercept = (float)L[0];
+        return r;
+    }
+    const uint32_t mag = 1;
+    double xa = 0, ya = 0, sx = 0, sxy = 0, sy = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        xa += (double)(i * mag) / (double)n;
+        ya += (double)L[i] / (double)n;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const double y = (double)L[i], dx = (double)i - xa;
+        sx += dx * dx;
+        sxy += dx * (y - ya);
+        sy += (y - ya) * (y - ya);
+    }
+    r.fSlope = (float)(sxy / sx);
+    r.fIntercept = (float)(ya - (double)r.fSlope * xa);
+    r.fRSquare = n != 2 ? (float)(sxy / std::sqrt(sx * sy)) : 1.0f;
+    r.fMultiple = std::sqrt(1.0f + r.fSlope * r.fSlope);
+    r.nMagnification = mag;
+    float left = 0.0f, right = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float x = ((float)L[i] - r.fIntercept) / r.fSlope / (float)mag;
+        const float d = x - (float)i;
+        if (d > left) left = d;
+        else if (d < right) right = d;
+    }
+    r.nRangeLeft = (uint32_t)left + 1;
+    r.nRangeRight = (uint32_t)(0.0f - right) + 1;
+    return r;
+}
+
 // Encoded form of one list before it is placed in the index.
 struct ListOut {
     std::vector<uint32_t> words, seg_offset, seg_head, seg_median, hash;
@@ -228,6 +276,7 @@ struct ListOut {
     std::vector<int32_t> base;
     exc_lrc_info lrc{1.0f, 0, 0.0f};
     exc_seg_info list{1.0f, 0.0f, 0, 0};
+    exc_gu_regression_info greg{};
     bool ok = true;
 };
 
@@ -236,6 +285,7 @@ void encode_list(int codec, uint32_t seg, float frac, uint32_t hs_target, uint32
 {
     if (codec == EXC_RAW || codec == EXC_RAW_HS) {  // the docIDs as they are (the GU programs' .ind2)
         o.words.assign(L, L + n);
+        if (codec == EXC_RAW) o.greg = gu_regression(L, n);
         if (codec == EXC_RAW_HS) gu_hash_table(L, n, hs_target, hs_shift, o.hash);
         return;
     }
@@ -421,6 +471,7 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
     if (codec == EXC_SEGLRC || codec == EXC_HS_SEGLRC) { b->seg_info.resize(S); b->base.resize(S); }
     if (codec == EXC_SEGLRC) b->list_info.resize(nterms);
     if (codec == EXC_LRC) b->lrc.resize(nterms);
+    if (codec == EXC_RAW) b->gu_reg.resize(nterms);
     const bool hashed = codec == EXC_HS_SEGLRC || codec == EXC_RAW_HS;
     if (hashed) { b->hash_offset.resize(nterms); b->hash_info.resize(hoff[nterms]); }
     parallel_for(nterms, nthreads, [&](uint32_t t, int) {
@@ -436,6 +487,7 @@ extern "C" int exc_build_index(int codec, uint32_t seg, float frac, uint32_t hs_
         if (!o.base.empty()) std::copy(o.base.begin(), o.base.end(), b->base.begin() + soff[t]);
         if (codec == EXC_SEGLRC) b->list_info[t] = o.list;
         if (codec == EXC_LRC) b->lrc[t] = o.lrc;
+        if (codec == EXC_RAW) b->gu_reg[t] = o.greg;
         if (hashed) {
             b->hash_offset[t] = (uint32_t)hoff[t];
             std::copy(o.hash.begin(), o.hash.end(), b->hash_info.begin() + hoff[t]);
@@ -471,6 +523,7 @@ extern "C" int exc_built_view(const exc_built* b, exc_index_host* v)
     v->hash_offset = ptr_or_null(b->hash_offset);
     v->hash_info = ptr_or_null(b->hash_info);
     v->hash_words = b->hash_info.size();
+    v->gu_regression = ptr_or_null(b->gu_reg);
     return EXC_OK;
 }
 
@@ -508,6 +561,10 @@ extern "C" int exc_index_save_files(const exc_index_host* h, const char* dir, co
     const std::string hp = p + (h->codec == EXC_RAW_HS ? "_HashSeg" : ".HashSeg");
     if (h->hash_offset && (rc = write_file(hp + "_offset", h->hash_offset, (size_t)nt * 4))) return rc;
     if (h->hash_info && (rc = write_file(hp + "_info", h->hash_info, h->hash_words * 4))) return rc;
+    // the regression program's at_regression_info (cuda_linearRegression.cu:888)
+    if (h->gu_regression &&
+        (rc = write_file(p + "_regression_info", h->gu_regression, (size_t)nt * sizeof(exc_gu_regression_info))))
+        return rc;
     return EXC_OK;
 }
 

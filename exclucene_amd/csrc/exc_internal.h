@@ -21,10 +21,10 @@ struct alignas(16) TermRec {
     uint32_t aux;     // HS: word offset of the list's bucket-occupancy table in
                       // DevIndex::hash, or kDenseBuckets (every bucket holds docIDs);
                       // RAW_HS: index of start_0 in .HashSeg_info
-    float slope;      // LRC fSlope | SegLRC list-level slope
-    int32_t icpt;     // LRC iIntercept | SegLRC list-level intercept bits | HS shift
-    float range_l;    // LRC fSearchRange | SegLRC list-level nRangeLeft
-    float range_r;    // SegLRC list-level nRangeRight
+    float slope;      // LRC fSlope | SegLRC list-level / RAW regression slope
+    int32_t icpt;     // LRC iIntercept | SegLRC list-level / RAW regression intercept bits | HS shift
+    float range_l;    // LRC fSearchRange | SegLRC list-level / RAW regression nRangeLeft
+    float range_r;    // SegLRC list-level / RAW regression nRangeRight
     uint32_t model;   // learned model slot, or kNoModel
     uint32_t term;    // the term's own index
     float merr;       // learned model: recorded max error (positions)
@@ -102,7 +102,7 @@ struct QueryRec {
 };
 static_assert(sizeof(QueryRec) == 32, "QueryRec is 32 bytes");
 
-enum { SEARCH_BINARY = 0, SEARCH_FIT = 1, SEARCH_LEARNED = 2 };
+enum { SEARCH_BINARY = 0, SEARCH_FIT = 1, SEARCH_LEARNED = 2, SEARCH_INTERP = 3 };
 
 // Per-query search knobs: bits 0-7 = longest run of a longer list's segments a
 // wave decodes cooperatively (0 = one lane per candidate, 64 = always

@@ -1083,25 +1083,120 @@ __device__ __forceinline__ bool raw_find(const uint32_t* __restrict__ L, uint32_
     return false;
 }
 
+// Positions [lo, hi) where docID v can sit in uncompressed list R: from the
+// list's regression line, x = (v - fIntercept) / fSlope with every element i
+// inside x - nRangeLeft < i < x + nRangeRight
+// (GU/cuda_runRegressionAlgorithm/cuda_linearRegression_kernel.cu:262-311,
+// which takes the window once per block of the shortest list), or from its
+// learned model, prediction +- recorded max error + 2.  Either is a guess
+// the caller verifies: float rounding may move an edge.
+__device__ __forceinline__ void raw_window(const DevIndex& ix, const TermRec& R, uint32_t v, bool learned,
+                                           int64_t& lo, int64_t& hi)
+{
+    float x, l, r;
+    if (learned) {
+        const float* prm = ix.mparams + (uint64_t)R.model * model_stride(R.mkind);
+        x = R.mkind == kModelPla ? pla_pred(prm, R.mx0, R.mxs, v) : mlp_pred(prm, R.mx0, R.mxs, R.n, v);
+        l = r = R.merr + 2.0f;
+    } else {
+        x = ((float)v - __int_as_float(R.icpt)) / R.slope;
+        l = R.range_l;
+        r = R.range_r;
+    }
+    // NaN and out-of-range guesses clamp to [-1, 4.3e9]: an empty window at worst
+    lo = (int64_t)floorf(fminf(fmaxf(x - l, -1.0f), 4.3e9f));
+    hi = (int64_t)ceilf(fminf(fmaxf(x + r, -1.0f), 4.3e9f)) + 1;
+}
+
+// x clamped into [lo, hi]
+__device__ __forceinline__ uint32_t clamp_pos(int64_t x, uint32_t lo, uint32_t hi)
+{
+    return x < (int64_t)lo ? lo : x > (int64_t)hi ? hi : (uint32_t)x;
+}
+
+__device__ __forceinline__ bool bsearch_u32(const uint32_t* __restrict__ L, uint32_t a, uint32_t b, uint32_t v)
+{
+    int32_t lo = (int32_t)a, hi = (int32_t)b - 1;
+    while (lo <= hi) {
+        const int32_t m = (lo + hi) >> 1;
+        const uint32_t q = L[m];
+        if (q == v) return true;
+        if (q < v) lo = m + 1; else hi = m - 1;
+    }
+    return false;
+}
+
+// Is v in L[a .. b)?  Binary search of the part of [a, b) inside the window
+// [wl, wh) first; when that leaves v's place open (v below the part's first
+// element or above its last), the rest of [a, b) on that side.  Exact for any
+// window; reads stay in L[a .. b).
+__device__ __forceinline__ bool raw_find_window(const uint32_t* __restrict__ L, uint32_t a, uint32_t b, int64_t wl,
+                                                int64_t wh, uint32_t v)
+{
+    const uint32_t lo = clamp_pos(wl, a, b);
+    const uint32_t hi = clamp_pos(wh, lo, b);
+    if (lo >= hi) return bsearch_u32(L, a, b, v);
+    uint32_t x = lo, y = hi;  // lower bound of v in [lo, hi)
+    while (x < y) {
+        const uint32_t m = (x + y) >> 1;
+        if (L[m] < v) x = m + 1; else y = m;
+    }
+    if (x < hi && L[x] == v) return true;
+    if (x > lo && x < hi) return false;  // L[x - 1] < v < L[x]
+    if (x == lo) return lo > a && bsearch_u32(L, a, lo, v);
+    return hi < b && bsearch_u32(L, hi, b, v);
+}
+
 // Uncompressed list R filters the unit's candidates: the wave bounds the
-// part of R that can hold [vmin, vmax], then each lane searches it for its
-// candidates, row by row; order-preserving compaction as for the codecs.
+// part [a, b) of R that can hold [vmin, vmax] (inside the regression or
+// learned window of vmin .. vmax when the list's elements around it show
+// that the window holds them all), then each lane searches it for its
+// candidates, row by row: binary search, interpolation search, or binary
+// search of the candidate's own window first; order-preserving compaction as
+// for the codecs.
 __device__ uint32_t probe_raw(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
 {
     constexpr int V = (int)(kChunk / 64);
     const uint32_t ln = lane_id();
     const uint32_t* L = A.ix.words + R.woff;
+    const uint32_t n = R.n;
     const uint32_t vmin = uni(W.cand[0]), vmax = uni(W.cand[nlive - 1]);
-    const uint32_t a = wave_lower_bound(L, 0, R.n, vmin);
-    const uint32_t b = a < R.n ? wave_lower_bound(L, a, R.n, vmax + 1u) : a;
+    const uint32_t search = knob_search(knobs);
+    const bool learned = search == SEARCH_LEARNED && R.model != kNoModel;
+    const bool windowed = learned || (search != SEARCH_BINARY && search != SEARCH_INTERP && R.range_l < 1.0e9f);
+    uint32_t lo = 0, hi = n;
+    if (windowed) {
+        int64_t l0, h0, l1, h1;
+        raw_window(A.ix, R, vmin, learned, l0, h0);
+        raw_window(A.ix, R, vmax, learned, l1, h1);
+        const uint32_t wl = uni(clamp_pos(l0, 0, n));
+        const uint32_t wh = uni(clamp_pos(h1, wl, n));
+        if ((wl == 0 || L[wl - 1] < vmin) && (wh == n || L[wh] > vmax)) {
+            lo = wl;
+            hi = wh;
+        }
+    }
+    const uint32_t a = wave_lower_bound(L, lo, hi, vmin);
+    const uint32_t b = a < hi ? wave_lower_bound(L, a, hi, vmax + 1u) : a;
     if (a >= b) return 0;
     const uint32_t t = uni(L[a]), r = uni(L[b - 1]);
-    const bool interp = knob_search(knobs) != SEARCH_BINARY;
+    const bool interp = search == SEARCH_INTERP;
+    const bool lane_window = windowed && b - a > 64;
     bool found[V];
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         const uint32_t e = ln + 64 * i;
-        found[i] = e < nlive && raw_find(L, a, b, t, r, W.cand[e], interp);
+        found[i] = false;
+        if (e < nlive) {
+            const uint32_t v = W.cand[e];
+            if (lane_window) {
+                int64_t wl, wh;
+                raw_window(A.ix, R, v, learned, wl, wh);
+                found[i] = v >= t && v <= r && raw_find_window(L, a, b, wl, wh, v);
+            } else {
+                found[i] = raw_find(L, a, b, t, r, v, interp);
+            }
+        }
     }
     uint32_t cnt = 0;
     wave_sync();

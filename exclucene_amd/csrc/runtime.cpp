@@ -413,7 +413,7 @@ extern "C" int exc_ctx_set_option(exc_ctx* c, int key, int64_t v)
     if (!c) return set_error(EXC_ERR_ARG, "exc_ctx_set_option: null ctx");
     switch (key) {
     case EXC_OPT_SEARCH:
-        if (v < 0 || v > 2) return set_error(EXC_ERR_ARG, "search mode must be 0, 1 or 2");
+        if (v < 0 || v > 3) return set_error(EXC_ERR_ARG, "search mode must be 0, 1, 2 or 3");
         c->search = (int)v;
         ++c->opt_gen;
         c->tuned = false;  // an explicit knob replaces the tuned table
@@ -502,6 +502,15 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
             R.nseg = (n + seg - 1) / seg;
             R.slope = 1.0f;
             R.range_l = R.range_r = 4.0e9f;
+            if (codec == EXC_RAW && h->gu_regression && n > 1 && h->gu_regression[t].fSlope > 0.0f) {
+                // the list's regression line and ranges: x = (v - icpt) / slope,
+                // every element i with x - nRangeLeft < i < x + nRangeRight
+                const exc_gu_regression_info& g = h->gu_regression[t];
+                R.slope = g.fSlope;
+                std::memcpy(&R.icpt, &g.fIntercept, 4);
+                R.range_l = (float)g.nRangeLeft;
+                R.range_r = (float)g.nRangeRight;
+            }
             if (raw_hs) {
                 // [shift][start_0 .. start_nb]: nb is where the starts reach n
                 // (the last bucket holds the list's maximum, so no earlier
@@ -792,6 +801,14 @@ extern "C" int exc_index_load_files(exc_ctx* c, int codec, uint32_t seg, const c
         h.ind1 = (const exc_term_ind1*)ind1.data();
         h.ind2 = (const uint32_t*)ind2.data();
         h.ind2_words = ind2.size() / 4;
+        if (codec == EXC_RAW) {  // GU/cuda_runRegressionAlgorithm's at_regression_info, when present
+            if ((rc = map_file(p + "_regression_info", reg, true))) return rc;
+            if (!reg.empty()) {
+                if (reg.size() != (size_t)h.nterms * sizeof(exc_gu_regression_info))
+                    return set_error(EXC_ERR_FORMAT, "_regression_info length differs from .ind1");
+                h.gu_regression = (const exc_gu_regression_info*)reg.data();
+            }
+        }
         if (codec == EXC_RAW_HS) {
             if ((rc = map_file(p + "_HashSeg_offset", hoff, false)) || (rc = map_file(p + "_HashSeg_info", hinfo, false)))
                 return rc;
@@ -2276,6 +2293,7 @@ extern "C" int exc_autotune(exc_ctx* c, exc_index* ix, const uint32_t* qterms, c
     else if (ix->codec == EXC_HS_SEGLRC || ix->codec == EXC_RAW || ix->codec == EXC_RAW_HS) coops = {0};
     else coops = {0, 2, 4, 8, 16, 32, 64};
     std::vector<uint32_t> searches = {SEARCH_BINARY, SEARCH_FIT};
+    if (ix->codec == EXC_RAW) searches.push_back(SEARCH_INTERP);
     if (!ix->models.empty()) searches.push_back(SEARCH_LEARNED);
     int rc = EXC_OK;
     for (int k = 0; k < kRatioClasses && !rc; ++k) {
@@ -2350,8 +2368,8 @@ extern "C" int exc_ctx_set_tuning(exc_ctx* c, const exc_tune_table* t)
     if (!c) return set_error(EXC_ERR_ARG, "exc_ctx_set_tuning: null ctx");
     if (t) {
         for (int k = 0; k < kRatioClasses; ++k)
-            if (t->coop_max[k] > 64 || t->search[k] > 2)
-                return set_error(EXC_ERR_ARG, "tuning table class %d: coop_max <= 64 and search <= 2", k);
+            if (t->coop_max[k] > 64 || t->search[k] > 3)
+                return set_error(EXC_ERR_ARG, "tuning table class %d: coop_max <= 64 and search <= 3", k);
         c->tune = *t;
         c->persistent = t->persistent != 0;
     }

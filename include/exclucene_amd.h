@@ -44,7 +44,9 @@ enum {
                           (GC/NewPFD/NewPFD_Decompression)                  */
     EXC_RAW = 5,       /* GU (src/gpu/Uncompressed): .ind2 holds the sorted
                           docIDs themselves; EXC_OPT_SEARCH 0 = binary search
-                          (GU/cuda_binarySearch), else interpolation search
+                          (GU/cuda_binarySearch), 1 = the list's regression
+                          window (GU/cuda_runRegressionAlgorithm), 2 = its
+                          learned window, 3 = interpolation search
                           (GU/cuda_interpolationSearch)                      */
     EXC_RAW_HS = 6     /* GU/cuda_hashSegment_new: the sorted docIDs as in
                           EXC_RAW plus per-list bucket tables on the top docID
@@ -62,6 +64,15 @@ typedef struct { uint32_t m_urlcount; uint64_t m_off; } exc_term_ind1;
 typedef struct { float fSlope; int32_t iIntercept; float fSearchRange; } exc_lrc_info;
 /* SegLRC/HS .regression_info row (GC/SegLRC/SegLRC_Intersection/SegLRC_Intersection_kernel.cu:89-94). */
 typedef struct { float fSlope; float fIntercept; uint32_t nRangeLeft; uint32_t nRangeRight; } exc_seg_info;
+/* Uncompressed-list regression row, the at_regression_info record of
+ * GU/cuda_runRegressionAlgorithm (cuda_linearRegression_kernel.cu:105-120),
+ * 36 bytes: the list's least-squares line over (position, docID) and the
+ * furthest positions on either side of it (cuda_linearRegression.cu:792-873).
+ * The probe reads fSlope, fIntercept, nRangeLeft and nRangeRight. */
+typedef struct {
+    float fSlope, fIntercept, fRSquare, fMultiple, fInterceptHigh, fInterceptLow;
+    uint32_t nRangeLeft, nRangeRight, nMagnification;
+} exc_gu_regression_info;
 /* batchInfo (GC/LRC/LRC_Intersection/LRC_Intersection_kernel.cu:68-74). */
 typedef struct {
     uint32_t blockNum;           /* work units (query, shortest-list segment) in the batch */
@@ -99,6 +110,8 @@ typedef struct {
     const uint32_t *hash_info;     /* HS tables [shift][nb][start_0..start_nb];
                                       RAW_HS tables [shift][start_0..start_nb]    */
     uint64_t hash_words;
+    const exc_gu_regression_info *gu_regression; /* nterms: RAW (optional;
+                                      <name>_regression_info on disk)             */
 } exc_index_host;
 
 typedef struct exc_ctx exc_ctx;
@@ -122,7 +135,8 @@ int exc_memcpy(exc_ctx *ctx, void *dst, const void *src, uint64_t bytes, int to_
 
 /* options (no reference counterpart: the reference fixes these with #define) */
 enum {
-    EXC_OPT_SEARCH = 0,     /* 0 plain binary search, 1 fit window + fallback, 2 learned window + fallback */
+    EXC_OPT_SEARCH = 0,     /* 0 plain binary search, 1 fit window + fallback, 2 learned window + fallback,
+                               3 interpolation search (uncompressed lists; a compressed codec takes it as 1) */
     EXC_OPT_TPE = 1,        /* threads per short-list element: 1 random access, 64 wave-cooperative decode,
                                0 cooperative for runs up to EXC_OPT_COOP_MAX blocks.  Setting SEARCH, TPE
                                or COOP_MAX drops a table installed by exc_autotune/exc_ctx_set_tuning. */

@@ -165,3 +165,80 @@ def test_raw_hash_segment_tables_equal_oracle(shift, tmp_path):
     assert "at_HashSeg_info" in names and "at_HashSeg_offset" in names
     assert np.array_equal(np.fromfile(tmp_path / "at_HashSeg_info", np.uint32), b.hash_info)
     assert np.array_equal(np.fromfile(tmp_path / "at_HashSeg_offset", np.uint32), b.hash_offset)
+
+
+# ---------------------------------------------------------------- uncompressed regression rows
+
+def _gu_lists(seed):
+    rng = np.random.default_rng(seed)
+    lists = [np.sort(rng.choice(1 << 26, n, replace=False)).astype(np.uint32) for n in (2, 3, 100, 5000, 200000)]
+    g = rng.integers(1, 40, 30000).astype(np.uint64)  # a dense head then a sparse tail: a poor line
+    g[15000:] *= 500
+    lists.append(np.cumsum(g).astype(np.uint32))
+    lists.append(np.array([77], np.uint32))
+    lists.append(np.zeros(0, np.uint32))
+    return lists
+
+
+def _gu_ranges(L, slope, icpt):
+    """Each element's signed distance from the line in float32, as the
+    reference's getRegressionInfo takes it: (float)docID, minus the float
+    intercept, over the float slope, minus (float)position."""
+    x = (L.astype(np.float32) - np.float32(icpt)) / np.float32(slope)
+    return x - np.arange(len(L)).astype(np.float32)
+
+
+def test_gu_regression_rows(tmp_path):
+    """The regression rows of an uncompressed index
+    (GU/cuda_runRegressionAlgorithm/cuda_linearRegression.cu:792-925):
+    slope and intercept agree with numpy's least-squares line, the ranges
+    are exactly floor(furthest distance) + 1 on either side (so every element
+    lies strictly inside its window), the one-element and empty lists take
+    the reference's special rows, and the rows travel through
+    <name>_regression_info."""
+    lists = _gu_lists(21)
+    docs, off = ea.lists_to_flat(lists)
+    b = ea.build_index(docs, off, ea.RAW, nthreads=2)
+    try:
+        g = b.gu_regression
+        assert g is not None and len(g) == len(lists)
+        for t, L in enumerate(lists):
+            row = g[t]
+            if len(L) == 1:
+                assert (row["fSlope"], row["fIntercept"], row["nRangeLeft"], row["nRangeRight"]) == (1.0, 77.0, 0, 0)
+                continue
+            if len(L) == 0:
+                assert np.isnan(row["fSlope"]) and row["nRangeLeft"] == 1 and row["nRangeRight"] == 1
+                continue
+            s, c = np.polyfit(np.arange(len(L), dtype=np.float64), L.astype(np.float64), 1)
+            assert abs(row["fSlope"] - s) <= 1e-5 * abs(s), t
+            assert abs(row["fIntercept"] - c) <= 1e-5 * max(1.0, abs(c), abs(s) * len(L)), t
+            d = _gu_ranges(L, row["fSlope"], row["fIntercept"])
+            assert row["nRangeLeft"] == int(max(0.0, float(d.max()))) + 1, t
+            assert row["nRangeRight"] == int(max(0.0, float(-d.min()))) + 1, t
+            assert np.all(d < row["nRangeLeft"]) and np.all(-d < row["nRangeRight"]), t
+            assert row["nMagnification"] == 1 and row["fInterceptHigh"] == 0 and row["fInterceptLow"] == 0
+            if len(L) > 2:
+                r2 = np.corrcoef(np.arange(len(L)), L.astype(np.float64))[0, 1]
+                assert abs(row["fRSquare"] - r2) <= 1e-5, t
+            assert abs(row["fMultiple"] - np.sqrt(1 + np.float32(row["fSlope"]) ** 2)) <= 1e-6 * row["fMultiple"]
+        b.save(str(tmp_path), "at")
+        raw = np.fromfile(str(tmp_path / "at_regression_info"), dtype=ea.GU_REG_INFO)
+        assert raw.tobytes() == g.tobytes()
+    finally:
+        b.close()
+
+
+def test_gu_range_check_catches_a_short_window():
+    """The containment check above fails on a window one position too
+    narrow on either side (the slip of dropping the + 1), so it pins the
+    ranges rather than restating them."""
+    L = _gu_lists(21)[5]
+    b = ea.build_index(*ea.lists_to_flat([L]), ea.RAW)
+    try:
+        row = b.gu_regression[0]
+        d = _gu_ranges(L, row["fSlope"], row["fIntercept"])
+        assert not np.all(d < row["nRangeLeft"] - 1)
+        assert not np.all(-d < row["nRangeRight"] - 1)
+    finally:
+        b.close()

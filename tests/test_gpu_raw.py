@@ -39,7 +39,7 @@ def raw(ctx):
     built.close()
 
 
-@pytest.mark.parametrize("search", [0, 1])
+@pytest.mark.parametrize("search", [0, 1, 3], ids=["binary", "regression", "interpolation"])
 @pytest.mark.parametrize("persistent", [0, 1])
 def test_raw_queries_exact(ctx, raw, search, persistent):
     lists, docs, off, built, dix, qs, ocum, ores = raw
@@ -61,12 +61,32 @@ def test_raw_files_decompress_and_autotune(ctx, raw, tmp_path):
     lix = ctx.load_files(ea.RAW, 0, str(tmp_path), "raw")
     try:
         t = ctx.autotune(lix, qs[:200], reps=1)
-        assert all(s in (0, 1) for s in t["search"])
+        assert all(s in (0, 1, 3) for s in t["search"])
         cum, res = ctx.intersect(lix, qs)
         assert np.array_equal(cum, ocum) and np.array_equal(res, ores)
     finally:
         ctx.set_tuning(None)
         lix.close()
+
+
+@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP], ids=["pla", "mlp"])
+def test_raw_learned_window_exact(ctx, raw, kind):
+    """Learned windows on uncompressed lists: the lists longer than 1,000
+    docIDs get a model; each candidate searches its predicted window
+    (+- the model's recorded error) first and the rest of the wave's range
+    only when the window's answer leaves its place open.  Every result equals
+    the oracle's, with the regression window too on the same index."""
+    lists, docs, off, built, dix, qs, ocum, ores = raw
+    st = ctx.train_models(dix, kind=kind, min_len=1000, epochs=60, samples=2048)
+    assert st["nmodels"] >= 5
+    try:
+        for search in (ea.SEARCH_LEARNED, ea.SEARCH_FIT):
+            print("case raw learned kind=%d search=%d" % (kind, search), flush=True)
+            ctx.set_option(ea.OPT_SEARCH, search)
+            cum, res = ctx.intersect(dix, qs)
+            assert np.array_equal(cum, ocum) and np.array_equal(res, ores), search
+    finally:
+        ctx.set_option(ea.OPT_SEARCH, 1)
 
 
 def test_raw_unsorted_list_refused(ctx):
@@ -149,7 +169,7 @@ def test_raw_hs_files_decompress_and_autotune(ctx, raw_hs, tmp_path):
     lix = ctx.load_files(ea.RAW_HS, 0, str(tmp_path), "at")
     try:
         t = ctx.autotune(lix, qs[:200], reps=1)
-        assert all(s in (0, 1) for s in t["search"])
+        assert all(s in (0, 1, 3) for s in t["search"])
         cum, res = ctx.intersect(lix, qs)
         assert np.array_equal(cum, ocum) and np.array_equal(res, ores)
     finally:
