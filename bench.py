@@ -344,12 +344,13 @@ def plain_leg(ea, ctx, dix, qs, ref_cum, ref_res, n=300):
 
 def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
     """C4: 10,000 queries of 2-4 terms (seed 44); the probe kernel with plain
-    binary search, the list-fit window and the learned window (PLA-256, then
-    MLP 1-32-1) for lists longer than 65,536; results must be identical."""
+    binary search, the list-fit window and the learned window (PLA-256,
+    PLA-Q, then MLP 1-32-1) for lists longer than 65,536; results must be
+    identical."""
     qs = W.df_weighted_queries(df, nq, 2, 4, 44)
     coop = tune["coop_max"] if tune else [16] * 6
     out, ref = {"queries": nq, "models_on_lists_longer_than": 65536}, None
-    for kind, name in ((ea.MODEL_PLA, "pla"), (ea.MODEL_MLP, "mlp")):
+    for kind, name in ((ea.MODEL_PLA, "pla"), (ea.MODEL_PLA_Q, "pla_q"), (ea.MODEL_MLP, "mlp")):
         st = ctx.train_models(dix, kind, min_len=65536, epochs=300, samples=4096, lr=1e-3)
         row = {"models": st["nmodels"], "train_ms": st["train_ms"], "mean_err": st["mean_err"], "max_err": st["max_err"]}
         for search, sname in ((0, "binary"), (1, "fit"), (2, "learned")):
@@ -374,6 +375,55 @@ def learned_leg(ea, W, ctx, dix, df, tune, nq=10_000):
     out["batched_learned_vs_plain"] = out["pla"]["learned_qps"] / out["plain_single_query"]["qps"]
     if tune:
         ctx.set_tuning(tune)
+    return out
+
+
+def clustered_leg(ea, W, ctx, coop=16, nq=10_000):
+    """C4s: the learned window where one line fits a list poorly.  Clustered
+    lists (shared topic clusters, a staircase docID -> position curve) as
+    SegLRC-128 and uncompressed; per layout and model kind (PLA-256, MLP
+    1-32-1 on lists longer than 65,536) the 10,000-query batch's probe time
+    with plain binary search, the list-fit window (SegLRC's list fit, the
+    regression program's window on the uncompressed lists) and the learned
+    window; every mode's results must equal binary search's.  The window
+    widths say why: the fit's nRangeLeft + nRangeRight against the model's
+    2 * (max error + 2), in positions, over the modelled lists."""
+    df, docs, off = W.c4s_corpus()
+    qs = W.c4s_queries(df, nq)
+    out = {"postings": int(len(docs)), "queries": nq, "models_on_lists_longer_than": 65536}
+    long_terms = [t for t in range(len(df)) if int(off[t + 1] - off[t]) > 65536]
+    for codec, lname in ((ea.SEGLRC, "seglrc128"), (ea.RAW, "uncompressed")):
+        b = ea.build_index(docs, off, codec, seg=128)
+        dix = ctx.upload(b)
+        fit = b.list_info if codec == ea.SEGLRC else b.gu_regression
+        fit_w = np.array([int(fit[t]["nRangeLeft"]) + int(fit[t]["nRangeRight"]) for t in long_terms], np.float64)
+        lay, ref = {"fit_window_positions_p50": float(np.median(fit_w))}, None
+        for kind, name in ((ea.MODEL_PLA_Q, "pla_q"), (ea.MODEL_PLA, "pla"), (ea.MODEL_MLP, "mlp")):
+            st = ctx.train_models(dix, kind, min_len=65536, epochs=300, samples=4096, lr=1e-3)
+            mw = np.array([2 * (ctx.model_info(dix, t)["err"] + 2) for t in long_terms], np.float64)
+            row = {"models": st["nmodels"], "train_ms": st["train_ms"], "model_window_positions_p50": float(np.median(mw))}
+            for search, sname in ((ea.SEARCH_BINARY, "binary"), (ea.SEARCH_FIT, "fit"), (ea.SEARCH_LEARNED, "learned")):
+                ctx.set_tuning({"coop_max": [coop] * 6, "search": [search] * 6, "persistent": 0})
+                ctx.plan(dix, qs)
+                bi = ctx._bi
+                ctx.kernelInvoke(bi)
+                ms = []
+                for _ in range(5):
+                    ctx.kernelInvoke(bi)
+                    ms.append(ctx.last_times()[0])
+                cum, res = ctx.dtohTransfer(bi)
+                if ref is None:
+                    ref = (cum, res)
+                row[sname + "_probe_ms"] = float(np.median(ms))
+                row[sname + "_matches_binary"] = bool(np.array_equal(cum, ref[0]) and np.array_equal(res, ref[1]))
+            row["learned_vs_fit"] = row["fit_probe_ms"] / row["learned_probe_ms"]
+            row["learned_vs_binary"] = row["binary_probe_ms"] / row["learned_probe_ms"]
+            lay[name] = row
+        out[lname] = lay
+        log("C4s %s: %s" % (lname, lay))
+        dix.close()
+        b.close()
+    ctx.set_tuning(None)
     return out
 
 
@@ -894,6 +944,9 @@ def run_c2(args, ea, W, ctx):
     if not args.no_c4:
         extra["learned"] = learned_leg(ea, W, ctx, dix, df, tune)
         log("learned: %s" % extra["learned"])
+        extra["learned"]["clustered"] = clustered_leg(ea, W, ctx)
+        if tune:
+            ctx.set_tuning(tune)
 
     if not args.no_cpu:
         c2["cpu_baseline"] = cpu_baseline(seg, queries, "C2")

@@ -39,7 +39,7 @@ EXPORTS = [
     "exc_router_create", "exc_router_autotune", "exc_router_set_routes", "exc_router_set_option",
     "exc_router_plan", "exc_router_invoke", "exc_router_fetch", "exc_router_intersect", "exc_router_destroy",
 ]
-MODEL_PLA, MODEL_MLP = 1, 2
+MODEL_PLA, MODEL_MLP, MODEL_PLA_Q = 1, 2, 3
 SERVER_COUNTS = 1
 
 _vp = ctypes.c_void_p
@@ -100,7 +100,7 @@ class ModelStats(ctypes.Structure):
 
 class ModelHost(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("n", _u32), ("err", ctypes.c_float), ("x0", ctypes.c_float),
-                ("xs", ctypes.c_float), ("nparams", _u32), ("params", ctypes.c_float * 257)]
+                ("xs", ctypes.c_float), ("nparams", _u32), ("params", ctypes.c_float * 770)]
 
 
 class BatchInfo(ctypes.Structure):

@@ -324,6 +324,7 @@ struct TrainArgs {
     float* grad_out;           // MLP gradient-only mode: per model kMlpParams floats + loss
 };
 hipError_t launch_pla_fit(const TrainArgs& a, hipStream_t s);
+hipError_t launch_plaq_fit(const TrainArgs& a, hipStream_t s);
 hipError_t launch_mlp_train(const TrainArgs& a, hipStream_t s);
 hipError_t launch_model_err(const TrainArgs& a, uint64_t total_elems, hipStream_t s);
 

@@ -585,6 +585,9 @@ __device__ __forceinline__ void model_window(const DevIndex& ix, const TermRec& 
     if (R.mkind == kModelPla) {
         plo = pla_pred(prm, R.mx0, R.mxs, vmin);
         phi = pla_pred(prm, R.mx0, R.mxs, vmax);
+    } else if (R.mkind == kModelPlaQ) {
+        plo = plaq_pred(prm, vmin);
+        phi = plaq_pred(prm, vmax);
     } else {
         mlp_pred_pair(prm, R.mx0, R.mxs, R.n, vmin, vmax, plo, phi);
     }
@@ -1096,7 +1099,7 @@ __device__ __forceinline__ void raw_window(const DevIndex& ix, const TermRec& R,
     float x, l, r;
     if (learned) {
         const float* prm = ix.mparams + (uint64_t)R.model * model_stride(R.mkind);
-        x = R.mkind == kModelPla ? pla_pred(prm, R.mx0, R.mxs, v) : mlp_pred(prm, R.mx0, R.mxs, R.n, v);
+        x = model_pred(R.mkind, prm, R.mx0, R.mxs, R.n, v);
         l = r = R.merr + 2.0f;
     } else {
         x = ((float)v - __int_as_float(R.icpt)) / R.slope;
@@ -1114,46 +1117,15 @@ __device__ __forceinline__ uint32_t clamp_pos(int64_t x, uint32_t lo, uint32_t h
     return x < (int64_t)lo ? lo : x > (int64_t)hi ? hi : (uint32_t)x;
 }
 
-__device__ __forceinline__ bool bsearch_u32(const uint32_t* __restrict__ L, uint32_t a, uint32_t b, uint32_t v)
-{
-    int32_t lo = (int32_t)a, hi = (int32_t)b - 1;
-    while (lo <= hi) {
-        const int32_t m = (lo + hi) >> 1;
-        const uint32_t q = L[m];
-        if (q == v) return true;
-        if (q < v) lo = m + 1; else hi = m - 1;
-    }
-    return false;
-}
-
-// Is v in L[a .. b)?  Binary search of the part of [a, b) inside the window
-// [wl, wh) first; when that leaves v's place open (v below the part's first
-// element or above its last), the rest of [a, b) on that side.  Exact for any
-// window; reads stay in L[a .. b).
-__device__ __forceinline__ bool raw_find_window(const uint32_t* __restrict__ L, uint32_t a, uint32_t b, int64_t wl,
-                                                int64_t wh, uint32_t v)
-{
-    const uint32_t lo = clamp_pos(wl, a, b);
-    const uint32_t hi = clamp_pos(wh, lo, b);
-    if (lo >= hi) return bsearch_u32(L, a, b, v);
-    uint32_t x = lo, y = hi;  // lower bound of v in [lo, hi)
-    while (x < y) {
-        const uint32_t m = (x + y) >> 1;
-        if (L[m] < v) x = m + 1; else y = m;
-    }
-    if (x < hi && L[x] == v) return true;
-    if (x > lo && x < hi) return false;  // L[x - 1] < v < L[x]
-    if (x == lo) return lo > a && bsearch_u32(L, a, lo, v);
-    return hi < b && bsearch_u32(L, hi, b, v);
-}
-
 // Uncompressed list R filters the unit's candidates: the wave bounds the
 // part [a, b) of R that can hold [vmin, vmax] (inside the regression or
 // learned window of vmin .. vmax when the list's elements around it show
-// that the window holds them all), then each lane searches it for its
-// candidates, row by row: binary search, interpolation search, or binary
-// search of the candidate's own window first; order-preserving compaction as
-// for the codecs.
+// that the window holds them all -- the reference's per-block borders), then
+// each lane searches [a, b) for its candidates, row by row, by binary or
+// interpolation search; order-preserving compaction as for the codecs.
+// (A window per candidate does not pay here: its lookup costs what the top
+// levels of the binary search it would skip cost, and those are cache hits
+// shared by the wave.)
 __device__ uint32_t probe_raw(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs)
 {
     constexpr int V = (int)(kChunk / 64);
@@ -1181,22 +1153,11 @@ __device__ uint32_t probe_raw(const ProbeArgs& A, const TermRec& R, const LdsWav
     if (a >= b) return 0;
     const uint32_t t = uni(L[a]), r = uni(L[b - 1]);
     const bool interp = search == SEARCH_INTERP;
-    const bool lane_window = windowed && b - a > 64;
     bool found[V];
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         const uint32_t e = ln + 64 * i;
-        found[i] = false;
-        if (e < nlive) {
-            const uint32_t v = W.cand[e];
-            if (lane_window) {
-                int64_t wl, wh;
-                raw_window(A.ix, R, v, learned, wl, wh);
-                found[i] = v >= t && v <= r && raw_find_window(L, a, b, wl, wh, v);
-            } else {
-                found[i] = raw_find(L, a, b, t, r, v, interp);
-            }
-        }
+        found[i] = e < nlive && raw_find(L, a, b, t, r, W.cand[e], interp);
     }
     uint32_t cnt = 0;
     wave_sync();

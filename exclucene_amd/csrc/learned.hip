@@ -6,6 +6,8 @@
 //    lies below p -- the empirical CDF at 257 equally spaced knots, computed
 //    by one binary search per knot (a closed-form least-error fit of a
 //    monotone linear spline to the CDF at its knots).
+//  * PLA-Q: knots at 256 equal position steps, x = the docID there as a
+//    float, y = the position, and each piece's slope (read off the list).
 //  * MLP 1-32-1 (ReLU): full-batch Adam on the mean squared position error
 //    over `samples` evenly strided pairs; one workgroup per list, forward and
 //    backward passes written out by hand, gradients reduced across the wave
@@ -63,6 +65,29 @@ __global__ __launch_bounds__(256) void k_pla_fit(TrainArgs a)
     if (threadIdx.x == 0) {
         M.x0 = x0;
         M.xs = xs;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_plaq_fit(TrainArgs a)
+{
+    const uint32_t m = blockIdx.x;
+    Model& M = a.models[m];
+    const uint32_t n = M.n;
+    const uint32_t* L = a.dec + a.dec_off[m];
+    float* p = a.params + M.param_off;
+    const auto q = [&](uint32_t j) { return (uint32_t)(((uint64_t)j * (n - 1)) / 256u); };
+    for (uint32_t j = threadIdx.x; j < (uint32_t)kPlaKnots; j += blockDim.x) {
+        const float x = (float)L[q(j)], y = (float)q(j);
+        p[j] = x;
+        p[kPlaKnots + j] = y;
+        if (j + 1 < (uint32_t)kPlaKnots) {
+            const float dx = (float)L[q(j + 1)] - x;
+            p[2 * kPlaKnots + j] = dx > 0.0f ? ((float)q(j + 1) - y) / dx : 0.0f;
+        }
+    }
+    if (threadIdx.x == 0) {
+        M.x0 = 0.0f;
+        M.xs = 0.0f;
     }
 }
 
@@ -163,7 +188,7 @@ __global__ __launch_bounds__(256) void k_model_err(TrainArgs a)
     const float* prm = a.params + M.param_off;
     float e = 0.0f;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < M.n; i += gridDim.x * blockDim.x) {
-        const float p = (M.kind == kModelPla) ? pla_pred(prm, M.x0, M.xs, L[i]) : mlp_pred(prm, M.x0, M.xs, M.n, L[i]);
+        const float p = model_pred(M.kind, prm, M.x0, M.xs, M.n, L[i]);
         e = fmaxf(e, fabsf(p - (float)i));
     }
     e = wave_max(e);
@@ -175,6 +200,12 @@ __global__ __launch_bounds__(256) void k_model_err(TrainArgs a)
 hipError_t launch_pla_fit(const TrainArgs& a, hipStream_t s)
 {
     if (a.nmodels) hipLaunchKernelGGL(k_pla_fit, dim3(a.nmodels), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plaq_fit(const TrainArgs& a, hipStream_t s)
+{
+    if (a.nmodels) hipLaunchKernelGGL(k_plaq_fit, dim3(a.nmodels), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

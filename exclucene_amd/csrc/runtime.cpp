@@ -1540,7 +1540,8 @@ extern "C" int exc_index_train_models(exc_ctx* c, exc_index* ix, int kind, uint3
                                       uint32_t samples, float lr, uint64_t seed, exc_model_stats* st)
 {
     if (!c || !ix) return set_error(EXC_ERR_ARG, "exc_index_train_models: null argument");
-    if (kind != EXC_MODEL_PLA && kind != EXC_MODEL_MLP) return set_error(EXC_ERR_ARG, "unknown model kind %d", kind);
+    if (kind != EXC_MODEL_PLA && kind != EXC_MODEL_MLP && kind != EXC_MODEL_PLA_Q)
+        return set_error(EXC_ERR_ARG, "unknown model kind %d", kind);
     if (ix->codec == EXC_HS_SEGLRC) return set_error(EXC_ERR_ARG, "HS_SegLRC lists are searched by bucket, not by position");
     if (ix->codec == EXC_NEWPFD) return set_error(EXC_ERR_ARG, "NewPFD indexes are for decompression only");
     if (kind == EXC_MODEL_MLP && (epochs == 0 || samples == 0)) return set_error(EXC_ERR_ARG, "MLP needs epochs and samples");
@@ -1596,7 +1597,10 @@ extern "C" int exc_index_train_models(exc_ctx* c, exc_index* ix, int kind, uint3
         a.samples = samples;
         a.lr = lr;
         a.seed = seed;
-        if (e == hipSuccess) e = kind == EXC_MODEL_PLA ? launch_pla_fit(a, c->stream) : launch_mlp_train(a, c->stream);
+        if (e == hipSuccess)
+            e = kind == EXC_MODEL_PLA     ? launch_pla_fit(a, c->stream)
+                : kind == EXC_MODEL_PLA_Q ? launch_plaq_fit(a, c->stream)
+                                          : launch_mlp_train(a, c->stream);
         if (e == hipSuccess) e = launch_model_err(a, tot, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         dev_free(c, dec);
@@ -1667,7 +1671,7 @@ extern "C" int exc_index_model_info(exc_ctx* c, exc_index* ix, uint32_t term, ex
     out->err = M.err;
     out->x0 = M.x0;
     out->xs = M.xs;
-    out->nparams = M.kind == kModelPla ? (uint32_t)kPlaKnots : (uint32_t)kMlpParams;
+    out->nparams = model_nparams(M.kind);
     HIP_TRY(hipMemcpy(out->params, dptr<float>(ix->d_params) + M.param_off, out->nparams * 4, hipMemcpyDeviceToHost));
     return EXC_OK;
 }

@@ -14,6 +14,14 @@ C5  universe 2^26, 1,000,000 terms, df_k = min(2^25, floor(C / k^1.1)) with
     C = 5.8e9, lists synthesized and SegLRC-encoded on the GPU (about 40 GB
     resident);
     2-6-term queries drawn proportionally to df arriving at 20k per second.
+C4s the learned-window workload on clustered lists (no reference counterpart:
+    the reference's datasets are crawled documents, whose docIDs bunch by site
+    and crawl order): universe 2^25, 10,000 terms, df_k = min(2^21,
+    floor(2^23 / k)); 64 shared topic clusters at uniform centres with
+    half-widths 2^U(-1,3) * 2^14; each term draws 90 % of its docIDs from 2-10
+    of them (Zipf 1.2 weights) and 10 % uniformly, so a long list's docID ->
+    position curve is a staircase that one line fits poorly; 10,000 queries of
+    2-4 terms drawn proportionally to df (seed 45).
 Lists come from exc_synth_lists (splitmix64 per list, Floyd sampling for dense
 lists, sort-unique-refill for sparse ones); query streams from numpy PCG64.
 """
@@ -111,3 +119,44 @@ def ratio_queries(df, nq=10_000, seed=3, lo=1.0, hi=4.0):
 def flat(queries):
     from . import queries_to_flat
     return queries_to_flat(queries)
+
+
+C4S_UNIVERSE = 1 << 25
+C4S_TERMS = 10_000
+
+
+def clustered_list(rng, n, centres, halfw, universe=C4S_UNIVERSE):
+    """n distinct docIDs, 90 % from a few of the shared clusters, 10 % uniform."""
+    if n == 0:
+        return np.zeros(0, np.uint32)
+    k = int(rng.integers(2, 11))
+    pick = rng.choice(len(centres), k, replace=False)
+    w = 1.0 / np.arange(1, k + 1) ** 1.2
+    w /= w.sum()
+    lo = np.maximum(centres[pick] - halfw[pick], 0)
+    hi = np.minimum(centres[pick] + halfw[pick], universe)
+    ncl = int(n * 0.9)
+    c = rng.choice(k, ncl, p=w)
+    v = lo[c] + (rng.random(ncl) * (hi - lo)[c]).astype(np.int64)
+    v = np.unique(np.concatenate([v, rng.integers(0, universe, n - ncl)]))
+    while len(v) < n:  # saturated clusters: the rest uniformly
+        v = np.unique(np.concatenate([v, rng.integers(0, universe, (n - len(v)) * 2 + 16)]))
+    if len(v) > n:
+        v = np.sort(rng.choice(v, n, replace=False))
+    return v.astype(np.uint32)
+
+
+def c4s_corpus(seed=7, nterms=C4S_TERMS):
+    """The clustered corpus of C4s: (df, docs, off)."""
+    df = zipf_df(nterms, universe=C4S_UNIVERSE, cap=1 << 21, scale=1 << 23)
+    rng = np.random.default_rng(seed)
+    centres = rng.integers(0, C4S_UNIVERSE, 64)
+    halfw = (2.0 ** rng.uniform(-1, 3, 64) * (1 << 14)).astype(np.int64)
+    lists = [clustered_list(rng, int(n), centres, halfw) for n in df]
+    off = np.zeros(nterms + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    return df, np.concatenate(lists), off
+
+
+def c4s_queries(df, nq=10_000, seed=45):
+    return df_weighted_queries(df, nq, 2, 4, seed)

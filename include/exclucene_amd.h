@@ -291,15 +291,23 @@ int exc_decompress_device(exc_ctx *ctx, exc_index *ix, const uint32_t *terms, ui
  *      GPU from the decoder's (docID, position) pairs; its recorded max error
  *      bounds the window of the search (EXC_OPT_SEARCH = 2), whose exact
  *      fallback keeps results identical to plain binary search. */
-enum { EXC_MODEL_PLA = 1, EXC_MODEL_MLP = 2 };
+/* EXC_MODEL_PLA: 256 linear pieces at equal docID steps (knot p = the count
+ * of elements below x0 + p / xs); EXC_MODEL_MLP: 1-32-1 ReLU network;
+ * EXC_MODEL_PLA_Q: 256 linear pieces at equal position steps (knot j at
+ * position floor(j (n - 1) / 256), its docID as a float), found by binary
+ * search over the knots -- its error stays within about n / 256 positions
+ * (plus the float spacing of the docIDs) whatever their distribution, where
+ * a clustered list defeats equal docID steps. */
+enum { EXC_MODEL_PLA = 1, EXC_MODEL_MLP = 2, EXC_MODEL_PLA_Q = 3 };
 typedef struct { uint32_t nmodels; float max_err; float mean_err; float train_ms; } exc_model_stats;
 typedef struct {
-    int32_t kind;       /* EXC_MODEL_PLA (257 knots) | EXC_MODEL_MLP (w1[32] b1[32] w2[32] b2) */
+    int32_t kind;       /* EXC_MODEL_PLA (257 knots) | EXC_MODEL_MLP (w1[32] b1[32] w2[32] b2) |
+                           EXC_MODEL_PLA_Q (x[257] docIDs, y[257] positions, s[256] piece slopes) */
     uint32_t n;         /* list length */
     float err;          /* recorded max |prediction - position| */
-    float x0, xs;       /* input offset and scale */
+    float x0, xs;       /* input offset and scale (PLA, MLP) */
     uint32_t nparams;
-    float params[257];
+    float params[770];
 } exc_model_host;
 int exc_index_train_models(exc_ctx *ctx, exc_index *ix, int kind, uint32_t min_len, uint32_t epochs,
                            uint32_t samples, float lr, uint64_t seed, exc_model_stats *stats);

@@ -98,6 +98,9 @@ def lib():
         L.orc_pla_pred.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]
         L.orc_mlp_pred.restype = ctypes.c_float
         L.orc_mlp_pred.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_plaq_fit.argtypes = [_vp, ctypes.c_uint32, _vp]
+        L.orc_plaq_pred.restype = ctypes.c_float
+        L.orc_plaq_pred.argtypes = [_vp, ctypes.c_uint32]
         L.orc_model_max_err.restype = ctypes.c_float
         L.orc_model_max_err.argtypes = [ctypes.c_int, _vp, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_uint32]
         L.orc_query_bytes.restype = ctypes.c_uint64
@@ -407,6 +410,20 @@ def pla_fit(L):
     y = np.zeros(257, np.float32)
     lib().orc_pla_fit(_p(a), len(a), x0.ctypes.data, xs.ctypes.data, y.ctypes.data)
     return float(x0[0]), float(xs[0]), y
+
+
+def plaq_fit(L):
+    """params (x[257], y[257], s[256]) of the PLA-Q position model of sorted
+    list L: knots at 256 equal position steps."""
+    a = _u32(L)
+    prm = np.zeros(770, np.float32)
+    lib().orc_plaq_fit(_p(a), len(a), prm.ctypes.data)
+    return prm
+
+
+def plaq_pred(params, v):
+    prm = np.ascontiguousarray(params, np.float32)
+    return float(lib().orc_plaq_pred(prm.ctypes.data, int(v)))
 
 
 def model_max_err(kind, params, x0, xs, L):

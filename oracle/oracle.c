@@ -1000,11 +1000,40 @@ float orc_mlp_pred(const float *w, float x0, float xs, uint32_t n, uint32_t v)
     return acc * (float)n;
 }
 
+/* PLA-Q: knot j at position q_j = floor(j (n - 1) / 256): x[j] = (float)L[q_j],
+ * y[j] = q_j, s[j] = (y[j+1] - y[j]) / (x[j+1] - x[j]) (0 for coinciding
+ * knots); params = x[257] y[257] s[256]. */
+void orc_plaq_fit(const uint32_t *L, uint32_t n, float *prm)
+{
+    for (uint32_t j = 0; j < 257; ++j) {
+        const uint32_t q = (uint32_t)(((uint64_t)j * (n - 1)) / 256u);
+        prm[j] = (float)L[q];
+        prm[257 + j] = (float)q;
+    }
+    for (uint32_t j = 0; j < 256; ++j) {
+        const float dx = prm[j + 1] - prm[j];
+        prm[514 + j] = dx > 0.0f ? (prm[257 + j + 1] - prm[257 + j]) / dx : 0.0f;
+    }
+}
+
+/* the piece: linear scan for the last knot <= v (the device halves) */
+float orc_plaq_pred(const float *prm, uint32_t v)
+{
+    const float fv = (float)v;
+    if (fv < prm[0]) return prm[257];
+    if (fv >= prm[256]) return prm[513];
+    int p = 0;
+    while (p + 1 < 256 && prm[p + 1] <= fv) ++p;
+    return fmaf(fv - prm[p], prm[514 + p], prm[257 + p]);
+}
+
 float orc_model_max_err(int kind, const float *params, float x0, float xs, const uint32_t *L, uint32_t n)
 {
     float e = 0.0f;
     for (uint32_t i = 0; i < n; ++i) {
-        const float p = kind == 1 ? orc_pla_pred(params, x0, xs, L[i]) : orc_mlp_pred(params, x0, xs, n, L[i]);
+        const float p = kind == 1   ? orc_pla_pred(params, x0, xs, L[i])
+                        : kind == 3 ? orc_plaq_pred(params, L[i])
+                                    : orc_mlp_pred(params, x0, xs, n, L[i]);
         e = fmaxf(e, fabsf(p - (float)i));
     }
     return e;

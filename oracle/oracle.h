@@ -158,6 +158,8 @@ uint64_t orc_query_bytes(const orc_index *ix, const uint32_t *qterms, const uint
 void orc_pla_fit(const uint32_t *L, uint32_t n, float *x0, float *xs, float *y /* 257 */);
 float orc_pla_pred(const float *y, float x0, float xs, uint32_t v);
 float orc_mlp_pred(const float *w /* 97 */, float x0, float xs, uint32_t n, uint32_t v);
+void orc_plaq_fit(const uint32_t *L, uint32_t n, float *params /* 770 */);
+float orc_plaq_pred(const float *params, uint32_t v);
 float orc_model_max_err(int kind, const float *params, float x0, float xs, const uint32_t *L, uint32_t n);
 
 #ifdef __cplusplus

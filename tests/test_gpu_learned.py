@@ -83,7 +83,65 @@ def test_mlp_training_gain_and_error(ctx, indexed):
     assert min(gains) > 1.6, gains
 
 
-@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP])
+def test_plaq_knots_and_error_match_oracle(ctx, indexed):
+    """PLA-Q on the device: knots, piece slopes and recorded max error equal
+    the oracle's restatement bit for bit."""
+    lists, built, dix = indexed
+    st = ctx.train_models(dix, ea.MODEL_PLA_Q, min_len=65536)
+    modelled = [t for t, L in enumerate(lists) if len(L) > 65536]
+    assert st["nmodels"] == len(modelled)
+    for t in modelled:
+        m = ctx.model_info(dix, t)
+        prm = oracle.plaq_fit(lists[t])
+        assert m["kind"] == ea.MODEL_PLA_Q and len(m["params"]) == 770
+        assert np.array_equal(m["params"], prm)
+        assert m["err"] == oracle.model_max_err(3, prm, 0.0, 0.0, lists[t])
+
+
+def test_learned_windows_beat_the_fit_on_clustered_lists(ctx):
+    """The gain of a learned model against the method without it: on
+    clustered lists (docIDs bunched in shared topic clusters) the window the
+    learned search scans, 2 * (max error + 2) positions, against the window
+    of the list-level fit it replaces (SegLRC's list fit, the uncompressed
+    lists' regression rows), nRangeLeft + nRangeRight.  PLA-Q's is narrower
+    on every modelled list by 8x or more (its error stays within about n / 256
+    where one line misses by a few per cent of the list); the results of every
+    search mode stay equal to the oracle's."""
+    from exclucene_amd import workloads as W
+    rng = np.random.default_rng(3)
+    centres = rng.integers(0, 1 << 25, 64)
+    halfw = (2.0 ** rng.uniform(-1, 3, 64) * (1 << 14)).astype(np.int64)
+    lens = [300000, 200000, 150000, 100000, 80000, 70000, 20000, 3000, 500, 50, 1]
+    lists = [W.clustered_list(rng, n, centres, halfw) for n in lens]
+    docs, off = ea.lists_to_flat(lists)
+    qs = [[int(x) for x in rng.choice(len(lists), int(rng.integers(2, 5)), replace=False)] for _ in range(300)]
+    ocum, ores = oracle.eval_queries(oracle.build_index(lists, oracle.SEGLRC, seg=128), qs)
+    modelled = [t for t, L in enumerate(lists) if len(L) > 65536]
+    for codec in (ea.SEGLRC, ea.RAW):
+        built = ea.build_index(docs, off, codec, seg=128)
+        dix = ctx.upload(built)
+        fit = built.list_info if codec == ea.SEGLRC else built.gu_regression
+        try:
+            for kind in (ea.MODEL_PLA_Q, ea.MODEL_PLA, ea.MODEL_MLP):
+                ctx.train_models(dix, kind, min_len=65536, epochs=300, samples=4096, lr=1e-3)
+                ratios = []
+                for t in modelled:
+                    fw = int(fit[t]["nRangeLeft"]) + int(fit[t]["nRangeRight"])
+                    ratios.append(fw / (2 * (ctx.model_info(dix, t)["err"] + 2)))
+                print("codec %d kind %d fit/model window: %s" % (codec, kind, np.round(ratios, 2)), flush=True)
+                if kind == ea.MODEL_PLA_Q:
+                    assert min(ratios) >= 8.0, ratios
+                for search in (ea.SEARCH_FIT, ea.SEARCH_LEARNED):
+                    ctx.set_option(ea.OPT_SEARCH, search)
+                    cum, res = ctx.intersect(dix, qs)
+                    assert np.array_equal(cum, ocum) and np.array_equal(res, ores), (codec, kind, search)
+        finally:
+            ctx.set_option(ea.OPT_SEARCH, 1)
+            dix.close()
+            built.close()
+
+
+@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP, ea.MODEL_PLA_Q])
 def test_learned_search_is_exact(ctx, indexed, kind):
     lists, built, dix = indexed
     ctx.train_models(dix, kind, min_len=65536, epochs=200, samples=4096, lr=1e-3)

@@ -69,13 +69,14 @@ def test_raw_files_decompress_and_autotune(ctx, raw, tmp_path):
         lix.close()
 
 
-@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP], ids=["pla", "mlp"])
+@pytest.mark.parametrize("kind", [ea.MODEL_PLA, ea.MODEL_MLP, ea.MODEL_PLA_Q], ids=["pla", "mlp", "pla_q"])
 def test_raw_learned_window_exact(ctx, raw, kind):
     """Learned windows on uncompressed lists: the lists longer than 1,000
-    docIDs get a model; each candidate searches its predicted window
-    (+- the model's recorded error) first and the rest of the wave's range
-    only when the window's answer leaves its place open.  Every result equals
-    the oracle's, with the regression window too on the same index."""
+    docIDs get a model; a wave searches only the window its model predicts
+    for its candidates' span (+- the recorded error), when the list's
+    elements around the window show it holds the span, else the whole list.
+    Every result equals the oracle's, with the regression window too on the
+    same index."""
     lists, docs, off, built, dix, qs, ocum, ores = raw
     st = ctx.train_models(dix, kind=kind, min_len=1000, epochs=60, samples=2048)
     assert st["nmodels"] >= 5

@@ -354,6 +354,48 @@ def test_pla_prediction_interpolates_the_knots():
     assert abs(got - err) <= 1e-3 * max(1.0, err)
 
 
+def test_plaq_knots_are_list_quantiles():
+    """PLA-Q knots sit at positions floor(j (n - 1) / 256): x = the fp32
+    docID there, y = the position, each slope the rise over the run of its
+    piece (0 where the fp32 knots coincide)."""
+    rng = np.random.default_rng(45)
+    for L in [rand_list(rng, n, 1 << 25) for n in (2, 300, 70000)] + [fixture_list()]:
+        prm = oracle.plaq_fit(L)
+        q = (np.arange(257, dtype=np.uint64) * np.uint64(len(L) - 1)) // np.uint64(256)
+        x, y, s = prm[:257], prm[257:514], prm[514:770]
+        assert np.array_equal(x, L[q].astype(np.float32))
+        assert np.array_equal(y, q.astype(np.float32))
+        dx = np.diff(x)
+        want = np.where(dx > 0, np.diff(y) / np.where(dx > 0, dx, np.float32(1)), np.float32(0)).astype(np.float32)
+        assert np.array_equal(s, want)
+
+
+def test_plaq_prediction_and_error_bound():
+    """PLA-Q predicts each knot's own position at its docID and the linear
+    interpolation of the knots in between (np.interp, up to fp32 rounding);
+    its max error over any list stays within one piece's positions plus the
+    fp32 spacing of the docIDs, here on a clustered list a line fits poorly."""
+    from exclucene_amd import workloads as W
+    rng = np.random.default_rng(46)
+    centres = rng.integers(0, 1 << 25, 64)
+    halfw = (2.0 ** rng.uniform(-1, 3, 64) * (1 << 14)).astype(np.int64)
+    for L in (rand_list(rng, 50000, 1 << 25), W.clustered_list(rng, 120000, centres, halfw)):
+        prm = oracle.plaq_fit(L)
+        x, y = prm[:257], prm[257:514]
+        for j in range(0, 257, 16):
+            assert oracle.plaq_pred(prm, int(L[int(y[j])])) == y[j]
+        vs = np.sort(rng.choice(L, 200, replace=False))
+        got = np.array([oracle.plaq_pred(prm, int(v)) for v in vs])
+        want = np.interp(vs.astype(np.float32).astype(np.float64), x.astype(np.float64), y.astype(np.float64))
+        assert np.abs(got - want).max() <= 1e-3 * len(L)
+        err = oracle.model_max_err(3, prm, 0.0, 0.0, L)
+        assert err <= (len(L) - 1) / 256 + 2 + 4, err
+    # one line misses the clustered list by far more
+    f = np.polyfit(np.arange(len(L)), L.astype(np.float64), 1)
+    line_err = np.abs((L - f[1]) / f[0] - np.arange(len(L))).max()
+    assert line_err > 8 * err
+
+
 def _fd_max_rel_err(params, L, samples, grad_fn, h=1e-6):
     """Largest relative difference between grad_fn's gradient and float64
     central finite differences of the oracle's loss."""
