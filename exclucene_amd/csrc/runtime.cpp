@@ -37,18 +37,6 @@
 
 using namespace exc;
 
-static thread_local char g_err[512] = "";
-
-int set_error(int code, const char* fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-extern "C" const char* exc_last_error(void) { return g_err; }
 
 #define HIP_TRY(call)                                                                          \
     do {                                                                                       \
