@@ -36,6 +36,7 @@ EXPORTS = [
     "exc_device_alloc", "exc_device_free", "exc_memcpy",
     "exc_synth_lists_device", "exc_build_index_device", "exc_synth_index_device", "exc_index_export",
     "exc_intersect_noalloc",
+    "exc_host_alloc", "exc_host_free",
     "exc_router_create", "exc_router_autotune", "exc_router_set_routes", "exc_router_set_option",
     "exc_router_plan", "exc_router_invoke", "exc_router_fetch", "exc_router_intersect", "exc_router_destroy",
 ]
@@ -182,6 +183,8 @@ def lib():
         L.exc_device_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
         L.exc_device_free.argtypes = [_vp, _vp]
         L.exc_memcpy.argtypes = [_vp, _vp, _vp, _u64, ctypes.c_int]
+        L.exc_host_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
+        L.exc_host_free.argtypes = [_vp, _vp]
         L.exc_decompress_device.argtypes = [_vp, _vp, _vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]
         L.exc_autotune.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32, ctypes.POINTER(TuneTable)]
         L.exc_ctx_set_tuning.argtypes = [_vp, ctypes.POINTER(TuneTable)]
@@ -350,16 +353,37 @@ class DeviceIndex:
 
 
 def _result_cap(ucnt, qterms, qoff):
-    """Results a batch can hold at most: the summed shortest-list lengths."""
+    """Results a batch can hold at most: the summed shortest-list lengths
+    (terms out of range count as empty lists; the library reports them)."""
     if ucnt is None:
         return 1 << 26
-    cap = 0
-    for q in range(len(qoff) - 1):
-        ts = qterms[int(qoff[q]):int(qoff[q + 1])]
-        ts = ts[ts < len(ucnt)]  # out-of-range terms are reported by the library
-        if len(ts):
-            cap += int(ucnt[ts].min())
-    return cap
+    qoff = np.asarray(qoff, np.int64)
+    nq = len(qoff) - 1
+    if nq <= 0 or qoff[-1] == 0:
+        return 0
+    t = np.asarray(qterms, np.int64)[:qoff[-1]]
+    n = np.where(t < len(ucnt), ucnt[np.minimum(t, len(ucnt) - 1)].astype(np.int64), 0)
+    starts = qoff[:-1]
+    nonempty = qoff[1:] > starts
+    mins = np.minimum.reduceat(n, np.minimum(starts, len(n) - 1))
+    return int(mins[nonempty].sum())
+
+
+class _Pinned:
+    """One exc_host_alloc buffer, freed when the last array over it goes."""
+
+    def __init__(self, ctx, nbytes):
+        p = _vp()
+        _check(lib().exc_host_alloc(ctx.handle, nbytes, ctypes.byref(p)))
+        self.ctx, self.ptr, self.nbytes = ctx, p.value, nbytes
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx._h:
+                lib().exc_host_free(self.ctx._h, self.ptr)
+        except Exception:
+            pass
+        self.ptr = None
 
 
 class Context:
@@ -416,11 +440,14 @@ class Context:
     def kernelInvoke(self, bi):
         _check(lib().exc_kernelInvoke(self._h, ctypes.byref(bi)))
 
-    def dtohTransfer(self, bi, cap=None):
+    def dtohTransfer(self, bi, cap=None, out=None):
+        """(cum_counts, results) of the invoked batch; out = (cum, res)
+        buffers to fill (page-locked ones from host_empty copy fastest),
+        else new arrays."""
         nq = bi.nTotalQueryNum
         cap = bi.ucntShortest_sum if cap is None else cap
-        cum = np.zeros(max(nq, 1), np.uint32)
-        res = np.zeros(max(cap, 1), np.uint32)
+        cum, res = self._out_arrays(nq, cap, out)
+        cap = min(cap, len(res))
         _check(lib().exc_dtohTransfer(self._h, ctypes.byref(bi), _p(cum), _p(res), cap))
         n = int(cum[nq - 1]) if nq else 0
         return cum[:nq], res[:n]
@@ -441,21 +468,41 @@ class Context:
         self._bi = bi
         return bi
 
-    def intersect(self, index, queries):
-        """Exact results of every query: (cum_counts, results)."""
-        return self._intersect_with(lib().exc_intersect, index, queries)
+    def intersect(self, index, queries, out=None):
+        """Exact results of every query: (cum_counts, results), filled into
+        out = (cum, res) when given (views of them are returned)."""
+        return self._intersect_with(lib().exc_intersect, index, queries, out)
 
     def intersect_noalloc(self, index, queries):
         """As intersect, through exc_intersect_noalloc: no allocation, the
         batch planned on the device inside the library's fixed scratch area."""
         return self._intersect_with(lib().exc_intersect_noalloc, index, queries)
 
-    def _intersect_with(self, fn, index, queries):
+    @staticmethod
+    def _out_arrays(nq, cap, out):
+        if out is None:
+            return np.zeros(max(nq, 1), np.uint32), np.zeros(max(cap, 1), np.uint32)
+        cum, res = out
+        if len(cum) < max(nq, 1) or cum.dtype != np.uint32 or res.dtype != np.uint32 or len(res) < 1:
+            raise ValueError("out buffers: uint32, at least one count per query")
+        return cum, res
+
+    def host_empty(self, count, dtype=np.uint32):
+        """A numpy array of `count` elements in page-locked host memory
+        (exc_host_alloc), freed when the array is garbage (or with the
+        context)."""
+        dt = np.dtype(dtype)
+        pin = _Pinned(self, max(1, int(count)) * dt.itemsize)
+        buf = (ctypes.c_char * pin.nbytes).from_address(pin.ptr)
+        buf._pin = pin  # the array's base keeps the allocation alive
+        return np.frombuffer(buf, dtype=dt, count=int(count))
+
+    def _intersect_with(self, fn, index, queries, out=None):
         qterms, qoff = queries if isinstance(queries, tuple) else queries_to_flat(queries)
         nq = len(qoff) - 1
         cap = _result_cap(index.ucnt, qterms, qoff)
-        cum = np.zeros(max(nq, 1), np.uint32)
-        res = np.zeros(max(cap, 1), np.uint32)
+        cum, res = self._out_arrays(nq, cap, out)
+        cap = min(cap, len(res))
         n = _u64()
         _check(fn(self._h, index.handle, _p(qterms), _p(qoff), nq, _p(cum), _p(res), cap, ctypes.byref(n)))
         return cum[:nq], res[:int(n.value)]

@@ -97,6 +97,8 @@ struct exc_ctx {
     uint64_t plan_model_gen = 0;         // model generation of the batch's TermRec copies
     // device buffers handed to the caller (exc_device_alloc)
     std::unordered_map<void*, size_t> user_bufs;
+    // page-locked host buffers handed to the caller (exc_host_alloc)
+    std::unordered_map<void*, size_t> host_bufs;
 };
 
 struct exc_index {
@@ -351,6 +353,7 @@ extern "C" void exc_ctx_destroy(exc_ctx* c)
                       &c->d_bsum, &c->d_result, &c->d_qcum, &c->d_counter})
         dev_free(c, *b);
     for (auto& kv : c->user_bufs) (void)hipFree(kv.first);
+    for (auto& kv : c->host_bufs) (void)hipHostFree(kv.first);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->pev)
@@ -383,6 +386,27 @@ extern "C" int exc_device_free(exc_ctx* c, void* d_ptr)
     (void)hipFree(d_ptr);
     c->used -= it->second;
     c->user_bufs.erase(it);
+    return EXC_OK;
+}
+
+extern "C" int exc_host_alloc(exc_ctx* c, uint64_t bytes, void** h_ptr)
+{
+    if (!c || !h_ptr || !bytes) return set_error(EXC_ERR_ARG, "exc_host_alloc: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    void* p = nullptr;
+    HIP_TRY(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    c->host_bufs[p] = bytes;
+    *h_ptr = p;
+    return EXC_OK;
+}
+
+extern "C" int exc_host_free(exc_ctx* c, void* h_ptr)
+{
+    if (!c || !h_ptr) return set_error(EXC_ERR_ARG, "exc_host_free: bad argument");
+    auto it = c->host_bufs.find(h_ptr);
+    if (it == c->host_bufs.end()) return set_error(EXC_ERR_ARG, "exc_host_free: not a buffer of this context");
+    HIP_TRY(hipHostFree(h_ptr));
+    c->host_bufs.erase(it);
     return EXC_OK;
 }
 

@@ -132,6 +132,13 @@ const char *exc_last_error(void);
 int exc_device_alloc(exc_ctx *ctx, uint64_t bytes, void **d_ptr);
 int exc_device_free(exc_ctx *ctx, void *d_ptr);
 int exc_memcpy(exc_ctx *ctx, void *dst, const void *src, uint64_t bytes, int to_device);
+/* Page-locked host buffers (the reference's cudaMallocHost buffers of
+ * allocateResource, GC/LRC/LRC_Intersection/LRC_Intersection.cu:81-82): a
+ * result buffer here takes exc_dtohTransfer / exc_intersect's copy at full
+ * PCIe rate instead of through the runtime's pageable staging.  Freed with
+ * the context at the latest. */
+int exc_host_alloc(exc_ctx *ctx, uint64_t bytes, void **h_ptr);
+int exc_host_free(exc_ctx *ctx, void *h_ptr);
 
 /* options (no reference counterpart: the reference fixes these with #define) */
 enum {

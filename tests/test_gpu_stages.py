@@ -211,3 +211,30 @@ def test_scan_over_many_blocks(ctx):
         ctx.set_option(ea.OPT_GRAPH, 0)
         ctx.set_option(ea.OPT_PERSISTENT, 1)
         dix.close()
+
+
+def test_page_locked_result_buffers(ctx):
+    """Results copied into page-locked buffers (exc_host_alloc) equal the
+    oracle's, through exc_intersect and through the stages, reused across
+    batches; a pointer that is not the context's is refused."""
+    lists = _corpus(23)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, ea.SEGLRC, seg=128)
+    ref = oracle.build_index(lists, oracle.SEGLRC, seg=128)
+    dix = ctx.upload(built)
+    try:
+        batches = [_queries(len(lists), 400, seed) for seed in (6, 7)]
+        cap = max(ea._result_cap(dix.ucnt, *ea.queries_to_flat(qs)) for qs in batches)
+        cum_buf, res_buf = ctx.host_empty(400), ctx.host_empty(cap)
+        for seed, qs in zip((6, 7), batches):
+            ocum, ores = oracle.eval_queries(ref, qs)
+            cum, res = ctx.intersect(dix, qs, out=(cum_buf, res_buf))
+            assert np.array_equal(cum, ocum) and np.array_equal(res, ores), seed
+            bi = ctx.plan(dix, qs)
+            ctx.kernelInvoke(bi)
+            cum, res = ctx.dtohTransfer(bi, out=(cum_buf, res_buf))
+            assert np.array_equal(cum, ocum) and np.array_equal(res, ores), seed
+        with pytest.raises(ea.ExcError):  # inside a buffer, not its start
+            ea._check(ea.lib().exc_host_free(ctx.handle, res_buf[8:].ctypes.data))
+    finally:
+        dix.close()
