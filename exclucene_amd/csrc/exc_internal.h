@@ -89,7 +89,10 @@ struct DevIndex {
     const float* mparams;
     int32_t codec;
     uint32_t seg;
+    uint32_t flags;  // kIxMedians
 };
+// DevIndex::flags: ParaPFD records carry the block medians (.segMedian) in rl
+constexpr uint32_t kIxMedians = 1;
 
 struct QueryRec {
     uint32_t toff;        // first sorted term of the query in qterms

@@ -39,6 +39,9 @@ namespace {
 constexpr int kWaves = 4;
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kClaim = 8;        // units per workgroup claim on the device counter (persistent launch)
+// ParaPFD probes take the sparse path (one candidate's half block per row of
+// 16 lanes) when the candidates average at most this many per distinct block
+constexpr uint32_t kPfdSparse = 4;
 constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
 constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative LRC/SegLRC probe
 constexpr uint32_t kBitmapPiece = 768; // docIDs decoded per load round of the bitmap path
@@ -964,24 +967,113 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 #pragma unroll
         for (int i = 0; i < V; ++i) s[i] = seg_of(v[i], ln + 64 * i < nlive);
         if constexpr (CODEC == EXC_PARAPFD) {
-            // ---- decode each distinct needed segment once, ND per round into
-            // the buffer's first kPfdDecWords (its tail holds the list)
-            constexpr int ND = (int)(kPfdDecWords / SEG);
-            static_assert(ND >= 1, "one segment fits the decode area");
-            uint32_t nd = 0, di[V];
+            // distinct segments of the candidates (candidate-order runs)
+            uint64_t fm[V];
+            uint32_t nd = 0;
 #pragma unroll
             for (int i = 0; i < V; ++i) {
                 // segment of the previous candidate: lane - 1 (a DPP wave_shr:1
                 // move), or lane 63 of the previous row
                 int32_t sp = __builtin_amdgcn_update_dpp(0, s[i], 0x138, 0xf, 0xf, false);
                 if (ln == 0) sp = (i > 0) ? (int32_t)__builtin_amdgcn_readlane((uint32_t)s[i > 0 ? i - 1 : 0], 63) : -2;
-                const bool first = s[i] >= 0 && s[i] != sp;
-                const uint64_t m = __ballot(first);
-                di[i] = nd + below_me(m) + (first ? 1u : 0u) - 1u;
-                if (first) W.dlist[di[i]] = (uint32_t)s[i];
-                nd += popc(m);
+                fm[i] = __ballot(s[i] >= 0 && s[i] != sp);
+                nd += popc(fm[i]);
             }
             nd = uni(nd);
+            bool sparse = false;
+            if constexpr (SEG == 128) sparse = (A.ix.flags & kIxMedians) && nlive <= kPfdSparse * nd;
+            if (sparse) {
+                if constexpr (SEG == 128) {
+                // ---- sparse: few candidates per block.  Each candidate's
+                // half block, the one its value falls in by the block's median
+                // (.segMedian, the reference's half-select,
+                // GC/ParaPFD/ParaPFD_Intersection_FRAC/ParaPFD_Intersection_kernel.cu:322-455),
+                // is decoded by one row of 16 lanes, 4 positions a lane: fields,
+                // exceptions ORed into the row's LDS stage, prefix from the
+                // block head or the median, compare.  Four candidates a pass;
+                // their segments wait in the buffer's first kChunk words.
+                constexpr uint32_t HP = SEG / 2, PL = HP / 16;
+                uint32_t* segs = W.buf;
+                uint32_t* stage = W.buf + kChunk;
+                static_assert(kChunk + 4 * HP <= kBufWords, "segments and four stages fit the buffer");
+#pragma unroll
+                for (int i = 0; i < V; ++i) segs[ln + 64 * i] = (uint32_t)s[i];
+                wave_sync();
+                const uint32_t r = ln >> 4, k = ln & 15u;
+                uint32_t hitm = 0;  // bit i: this lane's row-i candidate was found
+                for (uint32_t c0 = 0; c0 < nlive; c0 += 4) {
+                    const uint32_t c = c0 + r;
+                    const int32_t sc = c < nlive ? (int32_t)segs[c] : -1;
+                    const bool ok = sc >= 0;
+                    const uint32_t vc = ok ? cand[c] : 0u;
+                    const uint32_t si = ok ? (uint32_t)sc : sf;
+                    const SegRec rc = A.ix.rec[R.segnum + si];
+                    const uint32_t len = seg_len(R, si, SEG), mid = len >> 1;
+                    const bool upper = vc >= rc.rl;
+                    const uint32_t p0 = upper ? mid : 0u, pn = ok ? (upper ? len - mid : mid) : 0u;
+                    const uint32_t* data = A.ix.words + R.woff + rc.off + 1;
+                    const uint32_t lb = desc_lb(rc.desc), en = ok ? desc_en(rc.desc) : 0u;
+                    uint32_t f[PL];
+#pragma unroll
+                    for (uint32_t t = 0; t < PL; ++t) f[t] = bits_at(data, lb, p0 + k * PL + t);
+                    if (__ballot(en != 0)) {
+                        uint32_t* st = stage + r * HP;
+#pragma unroll
+                        for (uint32_t t = 0; t < PL; ++t) st[k * PL + t] = f[t];
+                        wave_sync();
+                        const uint32_t eb = desc_eb(rc.desc), hb = desc_hb(rc.desc);
+                        const uint32_t* pp = data + words_for(len, lb);
+                        const uint32_t* hp = pp + words_for(en, eb);
+                        for (uint32_t e = k; e < en; e += 16) {
+                            const uint32_t q = bits_at(pp, eb, e) - p0;
+                            if (q < pn) st[q] |= bits_at(hp, hb, e) << lb;
+                        }
+                        wave_sync();
+#pragma unroll
+                        for (uint32_t t = 0; t < PL; ++t) f[t] = st[k * PL + t];
+                        wave_sync();
+                    }
+                    // values: the head or the median, then the half's gaps
+                    // (its first gap stands for that value)
+                    if (k == 0) f[0] = 0;
+#pragma unroll
+                    for (uint32_t t = 1; t < PL; ++t) f[t] += f[t - 1];
+                    uint32_t x = f[PL - 1];
+                    x = dpp_add<0x111, 0xf>(x);
+                    x = dpp_add<0x112, 0xf>(x);
+                    x = dpp_add<0x114, 0xf>(x);
+                    x = dpp_add<0x118, 0xf>(x);
+                    const uint32_t off = x - f[PL - 1] + (upper ? rc.rl : rc.head);
+                    bool hit = false;
+#pragma unroll
+                    for (uint32_t t = 0; t < PL; ++t) hit = hit || (k * PL + t < pn && f[t] + off == vc);
+                    const uint64_t m = __ballot(hit);
+                    uint32_t hit4 = 0;
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; ++q) hit4 |= ((m >> (16 * q)) & 0xffffull) ? 1u << q : 0u;
+                    // candidates c0 .. c0 + 3 belong to lanes c0 % 64 .. + 3 of row c0 / 64
+                    if ((ln >> 2) == ((c0 & 63u) >> 2)) hitm |= ((hit4 >> (ln & 3u)) & 1u) << (c0 >> 6);
+                }
+#pragma unroll
+                for (int i = 0; i < V; ++i) found[i] = (hitm >> i) & 1u;
+                wave_sync();
+                }
+            } else {
+            // ---- decode each distinct needed segment once, ND per round into
+            // the buffer's first kPfdDecWords (its tail holds the list)
+            constexpr int ND = (int)(kPfdDecWords / SEG);
+            static_assert(ND >= 1, "one segment fits the decode area");
+            uint32_t di[V];
+            {
+                uint32_t before = 0;
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    const bool first = (fm[i] >> ln) & 1ull;
+                    di[i] = before + below_me(fm[i]) + (first ? 1u : 0u) - 1u;
+                    if (first) W.dlist[di[i]] = (uint32_t)s[i];
+                    before += popc(fm[i]);
+                }
+            }
             wave_sync();
             for (uint32_t r0 = 0; r0 < nd; r0 += ND) {
                 const uint32_t gn = min((uint32_t)ND, nd - r0);
@@ -993,6 +1085,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                         found[i] = lds_contains(W.buf + (di[i] - r0) * SEG, seg_len(R, (uint32_t)s[i], SEG), v[i]);
                 }
                 wave_sync();
+            }
             }
         } else {
             // LRC: binary search inside the segment

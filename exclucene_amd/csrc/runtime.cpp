@@ -698,6 +698,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     };
     ix->dix.codec = codec;
     ix->dix.seg = ix->seg;
+    ix->dix.flags = (codec == EXC_PARAPFD && h->seg_median) ? kIxMedians : 0u;
     ix->dix.words = (const uint32_t*)up(h->ind2, W * 4);
     ix->dix.terms = (const TermRec*)up(recs.data(), recs.size() * sizeof(TermRec));
     ix->d_terms = const_cast<TermRec*>(ix->dix.terms);
@@ -1961,6 +1962,7 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
     for (uint32_t t = 0; t < nterms; ++t) ix->n[t] = ix->recs[t].n;
     ix->dix.codec = codec;
     ix->dix.seg = seg;
+    ix->dix.flags = codec == EXC_PARAPFD ? kIxMedians : 0u;  // k_blk_fit keeps each block's median
     ix->dix.words = a.words;
     ix->dix.terms = a.terms;
     ix->d_terms = a.terms;
