@@ -361,6 +361,13 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
     constexpr int V = SEG / 64;
     constexpr int GG = PREFETCH ? 4 : 2;  // d-gap blocks in flight: the decoder's register budget allows more
     constexpr int G = gap_codec(CODEC) ? (NB < GG ? NB : GG) : NB;
+    // LANE_RECS: lane l < NB holds block min(l, r - 1)'s record, all in one load
+    uint32_t sl_all = 0;
+    SegRec gr_all;
+    if constexpr (LANE_RECS) {
+        sl_all = sid((int)min(min(lane_id(), (uint32_t)(NB - 1)), r - 1));
+        gr_all = ix.rec[R.segnum + sl_all];
+    }
     for (uint32_t g0 = 0; g0 < r; g0 += G) {
         SegRec rec[G];
         uint32_t len[G], sg[G];
@@ -368,18 +375,19 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
         uint32_t f[G][V];
         // past r: a copy of the last block, loaded but not used
         if constexpr (LANE_RECS) {
-            const uint32_t sl = sid((int)min(g0 + min(lane_id(), (uint32_t)(G - 1)), r - 1));
-            const SegRec gr = ix.rec[R.segnum + sl];
+            const uint32_t sl = sl_all;
+            const SegRec gr = gr_all;
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                sg[g] = uni(__builtin_amdgcn_readlane(sl, g));
-                rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, g));
-                rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, g));
-                rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), g)));
-                rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), g)));
+                const uint32_t src = min(g0 + (uint32_t)g, r - 1);
+                sg[g] = uni(__builtin_amdgcn_readlane(sl, src));
+                rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
+                rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, src));
+                rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), src)));
+                rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
                 rec[g].rl = rec[g].rr = 0;
-                rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, g));
-                rec[g].head = uni(__builtin_amdgcn_readlane(gr.head, g));
+                rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
+                rec[g].head = uni(__builtin_amdgcn_readlane(gr.head, src));
             }
         } else {
 #pragma unroll
