@@ -103,3 +103,42 @@ def test_c1_plain_query_on_device(ctx):
         ctx.set_option(ea.OPT_TPE, 0)
         ctx.set_option(ea.OPT_PERSISTENT, 1)
         dix.close()
+
+
+@pytest.mark.parametrize("frac", [0.1, 0.6])
+def test_parapfd_sparse_and_distinct_paths_agree(ctx, frac):
+    """High-ratio ParaPFD probes take the half-block path when the index
+    carries its block medians (.segMedian) and the distinct-block rounds when
+    it does not (an index written without them): both equal the oracle, in
+    both launch modes, on queries of one short list against long ones."""
+    lists = _lists(7)
+    docs, off = ea.lists_to_flat(lists)
+    built = ea.build_index(docs, off, ea.PARAPFD, seg=128, frac=frac)
+    ref = oracle.build_index(lists, ea.PARAPFD, seg=128, frac=frac)
+    n = [len(L) for L in lists]
+    shorts = [i for i in range(len(lists)) if 2 <= n[i] <= 1000]
+    longs = [i for i in range(len(lists)) if n[i] >= 20000]
+    qs = [[s, t] for s in shorts for t in longs] + [[s, t, u] for s in shorts for t in longs for u in longs if t < u]
+    ocum, ores = oracle.eval_queries(ref, qs)
+    with_medians = ctx.upload(built)
+    view = ea.IndexHost.from_buffer_copy(built.view)
+    view.seg_median = None
+    no_medians = ea.DeviceIndex(ctx, _upload_view(ctx, view), lengths=np.diff(off).astype(np.uint32))
+    try:
+        for dix in (with_medians, no_medians):
+            for persistent in (0, 1):
+                ctx.set_option(ea.OPT_PERSISTENT, persistent)
+                cum, res = ctx.intersect(dix, qs)
+                assert np.array_equal(cum, ocum) and np.array_equal(res, ores), (dix is no_medians, persistent)
+    finally:
+        ctx.set_option(ea.OPT_PERSISTENT, 1)
+        with_medians.close()
+        no_medians.close()
+        built.close()
+
+
+def _upload_view(ctx, view):
+    import ctypes
+    h = ctypes.c_void_p()
+    ea._check(ea.lib().exc_index_upload(ctx.handle, ctypes.byref(view), ctypes.byref(h)))
+    return h
