@@ -1068,31 +1068,42 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                 }
             } else {
             // ---- decode each distinct needed segment once, ND per round into
-            // the buffer's first kPfdDecWords (its tail holds the list)
-            constexpr int ND = (int)(kPfdDecWords / SEG);
-            static_assert(ND >= 1, "one segment fits the decode area");
+            // the buffer's head, the list of segments after the decode area:
+            // four a round when the list fits the buffer's last 256 words,
+            // else two (the list taking kChunk words)
             uint32_t di[V];
-            {
+            const auto rounds = [&](auto nd_c, uint32_t* dl) {
+                constexpr int ND = decltype(nd_c)::value;
+                static_assert(ND >= 1, "one segment fits the decode area");
                 uint32_t before = 0;
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
                     const bool first = (fm[i] >> ln) & 1ull;
                     di[i] = before + below_me(fm[i]) + (first ? 1u : 0u) - 1u;
-                    if (first) W.dlist[di[i]] = (uint32_t)s[i];
+                    if (first) dl[di[i]] = (uint32_t)s[i];
                     before += popc(fm[i]);
                 }
-            }
-            wave_sync();
-            for (uint32_t r0 = 0; r0 < nd; r0 += ND) {
-                const uint32_t gn = min((uint32_t)ND, nd - r0);
-                decode_blocks<CODEC, SEG, ND, true>(A.ix, R, gn, [&](int g) { return W.dlist[r0 + g]; }, W.buf);
                 wave_sync();
+                for (uint32_t r0 = 0; r0 < nd; r0 += ND) {
+                    const uint32_t gn = min((uint32_t)ND, nd - r0);
+                    decode_blocks<CODEC, SEG, ND, true>(A.ix, R, gn, [&](int g) { return dl[r0 + g]; }, W.buf);
+                    wave_sync();
 #pragma unroll
-                for (int i = 0; i < V; ++i) {
-                    if (s[i] >= 0 && di[i] >= r0 && di[i] < r0 + ND)
-                        found[i] = lds_contains(W.buf + (di[i] - r0) * SEG, seg_len(R, (uint32_t)s[i], SEG), v[i]);
+                    for (int i = 0; i < V; ++i) {
+                        if (s[i] >= 0 && di[i] >= r0 && di[i] < r0 + ND)
+                            found[i] = lds_contains(W.buf + (di[i] - r0) * SEG, seg_len(R, (uint32_t)s[i], SEG), v[i]);
+                    }
+                    wave_sync();
                 }
-                wave_sync();
+            };
+            constexpr uint32_t kWideDec = 4 * SEG;  // four segments, then a list of at most kBufWords - 4 SEG
+            if constexpr (kWideDec + 64 <= kBufWords) {
+                if (nd <= kBufWords - kWideDec)
+                    rounds(std::integral_constant<int, 4>{}, W.buf + kWideDec);
+                else
+                    rounds(std::integral_constant<int, (int)(kPfdDecWords / SEG)>{}, W.dlist);
+            } else {
+                rounds(std::integral_constant<int, (int)(kPfdDecWords / SEG)>{}, W.dlist);
             }
             }
         } else {
