@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # profile figures reported beside this run's, not measured in it
 TRAFFIC_C2_KIB = 1302880.0
 TRAFFIC_C2_SOURCE = "profiles/k_probe_seglrc128_c2.md (rocprofv3 --pmc on the C2 batch)"
-TRAFFIC_C5_KIB = 5089390.0
+TRAFFIC_C5_KIB = 5091503.0
 TRAFFIC_C5_SOURCE = "profiles/k_probe_seglrc128_c5.md (rocprofv3 --pmc on the C5 batch)"
 META = {0: 8, 1: 28, 2: 32, 3: 8, 4: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD, NewPFD
 CODEC_NAME = {0: "LRC", 1: "SegLRC", 2: "HS_SegLRC", 3: "ParaPFD", 4: "NewPFD"}
