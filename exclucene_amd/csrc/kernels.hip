@@ -37,13 +37,15 @@ namespace exc {
 namespace {
 
 constexpr int kWaves = 4;
+constexpr int kProbeWaves = 1;  // waves (work units in flight) per probe workgroup
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kClaim = 8;        // units per workgroup claim on the device counter (persistent launch)
 // ParaPFD probes take the sparse path (one candidate's half block per row of
 // 16 lanes) when the candidates average at most this many per distinct block
 constexpr uint32_t kPfdSparse = 4;
 constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
-constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative LRC/SegLRC probe
+constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative ParaPFD probe
+constexpr uint32_t kMaxWindowsSeg = 12;  // bitmap windows per cooperative LRC/SegLRC probe (16,384 docIDs each)
 constexpr uint32_t kBitmapPiece = 768; // docIDs decoded per load round of the bitmap path
 constexpr uint32_t kWinWords = 12;     // words of a candidate's fit window staged in LDS (64 x 12 <= kBufWords)
 static_assert(64 * kWinWords <= kBufWords, "window staging fits the decode buffer");
@@ -74,22 +76,24 @@ __device__ __forceinline__ float vgpr_copy(float x)
     return r;
 }
 
-// The LRC/SegLRC window bitmap is transposed: docID offset x (< 32 * 768)
-// lives in word x % 768, bit x / 768.  Consecutive docIDs then land in
-// different words (and banks), so the atomic ORs of a dense run of values do
-// not queue on one word, as they do with word x / 32, bit x % 32 (C5's
-// SegLRC probe: 3.12 -> 2.68 ms).  768 = 3 * 256, so
-// x / 768 = ((x >> 8) * 0xAAAB) >> 17, exact for x >> 8 < 96.  The ParaPFD
-// bitmap keeps the plain layout: its values are gap sums, seldom adjacent,
-// and the division cost more than it saved there.
-struct BmPos {
-    uint32_t w, b;
-};
-__device__ __forceinline__ BmPos bm_pos(uint32_t x)
+// The LRC/SegLRC window bitmap: kBmWords words at a 2 KiB-aligned LDS
+// address inside the wave's buffer, docID offset x (< 32 * kBmWords) in word
+// x % kBmWords, bit x / kBmWords.  Consecutive docIDs land in different words
+// (and banks), so the atomic ORs of a dense run of values do not queue on one
+// word as they do with word x / 32, bit x % 32; a power-of-two word count
+// makes the position a mask and a shift (with 768 words it took a division
+// by 768, five instructions per value), and the aligned base makes the word
+// address one shift-or.  The ParaPFD bitmap keeps the plain layout: its values
+// are gap sums, seldom adjacent.
+constexpr uint32_t kBmWords = 512;
+// The bitmap inside a wave's buffer (kBufWords words): the buffer's first
+// 2 KiB boundary, or nullptr if the bitmap would run past the buffer's end
+// (k_probe aligns its LDS to 2 KiB and every wave's buffer starts at a
+// multiple of 1 KiB, so this does not happen).
+__device__ __forceinline__ uint32_t* bitmap_of(uint32_t* buf)
 {
-    static_assert(kBufWords == 768, "bm_pos divides by 768");
-    const uint32_t b = ((x >> 8) * 0xAAABu) >> 17;
-    return BmPos{x - b * 768u, b};
+    const uint32_t pad = (0u - (uint32_t)(uintptr_t)buf) & 2047u;
+    return uni(pad) + 4u * kBmWords <= 4u * kBufWords ? buf + pad / 4u : nullptr;
 }
 
 // x >> 5 as one instruction the compiler cannot look through: it otherwise
@@ -640,7 +644,7 @@ struct LdsWave {
 
 template <int CODEC, int SEG>
 __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWave& W, uint32_t nlive, uint32_t knobs,
-                              uint32_t pre_first, uint32_t pre_count)
+                              uint32_t pre_first, uint32_t pre_count, uint32_t pre_heads)
 {
     const uint32_t search = knob_search(knobs);
     constexpr int V = kV;
@@ -663,7 +667,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         hb0 = pre_first;
         rng = pre_count;
         small = rng <= 64;
-        H = (small && ln < (uint32_t)rng) ? (head + hb0)[ln] : kNone;
+        H = pre_heads;  // head[hb0 + lane] for lanes below rng (small), loaded by run_unit
     } else if (search != SEARCH_BINARY && nseg > 64) {
         int64_t wa, t0, t1, wb;
         if (search == SEARCH_LEARNED && R.model != kNoModel) {
@@ -719,74 +723,98 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     for (int i = 0; i < V; ++i) found[i] = false;
 
     const uint32_t cmax = knob_coop(knobs);
-    constexpr uint32_t kWin = kBufWords * 32u;  // docIDs one LDS bitmap covers
+    constexpr uint32_t kWin = kBmWords * 32u;   // docIDs one LRC/SegLRC window bitmap covers
     constexpr uint32_t kWinPfd = (kBufWords - SEG) * 32u;  // ParaPFD: the buffer's tail stages a block
     constexpr int NBB = kBitmapPiece / SEG;       // blocks per load round of the bitmap path
-    if (CODEC != EXC_PARAPFD && r <= cmax && ((vmax - vmin) / kWin) < kMaxWindows) {
+    uint32_t* const bm = bitmap_of(W.buf);
+    if (CODEC != EXC_PARAPFD && small && r <= cmax && ((vmax - vmin) / kWin) < kMaxWindowsSeg && bm != nullptr) {
         // ---- LRC/SegLRC: the run sf .. sf+r-1 decoded straight from
         // registers into a bitmap of a docID window in LDS; every candidate of
         // the window then tests its bit.  Windows of kWin docIDs cover
         // [vmin, vmax]; each decodes only the blocks that overlap it (the run's
         // heads tell which) and windows without candidates are skipped.
         // Independent LDS operations only, no per-candidate search chains.
-        uint32_t* bm = W.buf;
-        const SegRec gr = A.ix.rec[R.segnum + sf + min(ln, r - 1)];
-        const uint32_t hd = ln < r ? gr.head : kNone;
+        // Per block: the record in scalar registers (one scalar load), each
+        // lane's bit offset ln * lb once (the second field of a lane sits
+        // 64 * lb bits = 2 lb words further on, the same bit offset in its
+        // word), then per value a funnel shift, a mask, the fit and the
+        // insertion; only a list's last block tests j < len.  The run's
+        // heads are already in H (block hb0 + l in lane l, block sf in lane
+        // hoff).
+        const uint32_t hoff = uni((uint32_t)(sf - hb0));
         for (uint32_t lo = vmin;;) {
             const uint32_t hi = (vmax - lo < kWin) ? vmax : lo + (kWin - 1);
-            bool any = false;
+            bool any = lo == vmin;  // the first window holds the first candidate
+            if (!any) {
 #pragma unroll
-            for (int i = 0; i < V; ++i) {
-                const uint32_t vi = cand_at(i);
-                any = any || (ln + 64 * i < nlive && vi >= lo && vi <= hi);
+                for (int i = 0; i < V; ++i) {
+                    const uint32_t vi = cand_at(i);
+                    any = any || (ln + 64 * i < nlive && vi >= lo && vi <= hi);
+                }
             }
             if (__ballot(any)) {
-                const uint32_t c_lo = popc(__ballot(hd <= lo)), c_hi = popc(__ballot(hd <= hi));
-                const uint32_t b0 = c_lo ? c_lo - 1 : 0, b1 = c_hi ? c_hi - 1 : 0;
-                const uint32_t nw = min(kBufWords, hi - lo + 1);  // offset x sits in word x % kBufWords
+                // blocks b0 .. b1 of the run (relative to sf) overlap [lo, hi]
+                const uint32_t c_lo = popc(__ballot(H <= lo)), c_hi = popc(__ballot(H <= hi));
+                const uint32_t b0 = c_lo > hoff ? c_lo - 1 - hoff : 0;
+                const uint32_t b1 = min(c_hi > hoff ? c_hi - 1 - hoff : 0, r - 1);
+                const uint32_t span = hi - lo;
+                const uint32_t nw = min(kBmWords, span + 1);  // offset x sits in word x % kBmWords
                 for (uint32_t e = ln; e < nw; e += 64) bm[e] = 0;
                 wave_sync();
                 for (uint32_t p0 = b0; p0 <= b1; p0 += NBB) {
                     const uint32_t np = min((uint32_t)NBB, b1 + 1 - p0);
                     SegRec rec[NBB];
-                    uint32_t len[NBB];
-                    uint32_t f[NBB][SEG / 64];
+                    uint32_t sh[NBB];
+                    uint32_t w0[NBB][SEG / 64], w1[NBB][SEG / 64];
+                    // every record of the piece first (scalar loads, one wait),
+                    // past np a copy of the piece's last block, loaded but not used
+#pragma unroll
+                    for (int g = 0; g < NBB; ++g) rec[g] = A.ix.rec[R.segnum + sf + min(p0 + (uint32_t)g, b1)];
 #pragma unroll
                     for (int g = 0; g < NBB; ++g) {
-                        {  // past np: a copy of the piece's last block, loaded but not used
-                            const int src = (int)min(p0 + g, b1);
-                            rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, src));
-                            rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, src));
-                            rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), src)));
-                            rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), src)));
-                            rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, src));
-                            len[g] = seg_len(R, sf + (uint32_t)src, SEG);
-                            // fields past len are not zeroed: the insertion below tests j < len
-                            load_fields<CODEC, SEG, false>(A.ix.words + R.woff + rec[g].off + 1, desc_lb(rec[g].desc),
-                                                           len[g], f[g]);
+                        const uint32_t lb = desc_lb(rec[g].desc);
+                        const char* q = (const char*)(A.ix.words + R.woff + rec[g].off + 1);
+                        sh[g] = ln * lb;
+                        // 32-bit byte offsets from the block's (scalar) address, so
+                        // that each load takes the scalar base + vector offset form
+                        const uint32_t bo = (sh[g] >> 5) * 4u;
+#pragma unroll
+                        for (int t = 0; t < SEG / 64; ++t) {
+                            const uint32_t* w = (const uint32_t*)(q + (bo + 8u * (uint32_t)t * lb));
+                            w0[g][t] = w[0];
+                            w1[g][t] = w[1];
                         }
                     }
 #pragma unroll
                     for (int g = 0; g < NBB; ++g) {
                         if ((uint32_t)g < np) {
+                            const uint32_t s = sf + p0 + (uint32_t)g;
+                            const uint32_t lb = desc_lb(rec[g].desc), mask = (1u << lb) - 1u;
                             // the block fit's intercept in a vector register once per block, so
                             // that each value's fma reads one scalar operand (the slope) only
                             const float icv = vgpr_copy(rec[g].icpt);
+                            const uint32_t add = (uint32_t)rec[g].base - lo;
+                            const auto insert = [&](auto tail_c) {
+                                constexpr bool TAIL = decltype(tail_c)::value;
+                                const uint32_t len = TAIL ? seg_len(R, s, SEG) : (uint32_t)SEG;
 #pragma unroll
-                            for (int t = 0; t < SEG / 64; ++t) {
-                                const uint32_t j = ln + 64 * t;
-                                uint32_t x;
-                                if constexpr (CODEC == EXC_SEGLRC)
-                                    x = f[g][t] + (uint32_t)f2i_sat(fmaf((float)j, rec[g].slope, icv)) +
-                                        (uint32_t)rec[g].base - lo;
-                                else
-                                    x = block_value<CODEC, SEG>(R, sf + p0 + g, rec[g], j, f[g][t]) - lo;
-                                if (j < len[g] && x <= hi - lo) {
-                                    const BmPos bp = bm_pos(x);
-                                    __hip_atomic_fetch_or(bm + bp.w, 1u << bp.b, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_WAVEFRONT);
+                                for (int t = 0; t < SEG / 64; ++t) {
+                                    const uint32_t j = ln + 64 * t;
+                                    const uint32_t f = __builtin_amdgcn_alignbit(w1[g][t], w0[g][t], sh[g]) & mask;
+                                    uint32_t x;
+                                    if constexpr (CODEC == EXC_SEGLRC)
+                                        x = f + (uint32_t)f2i_sat(fmaf((float)j, rec[g].slope, icv)) + add;
+                                    else
+                                        x = block_value<CODEC, SEG>(R, s, rec[g], j, f) - lo;
+                                    if ((!TAIL || j < len) && x <= span)
+                                        __hip_atomic_fetch_or(bm + (x & (kBmWords - 1u)), 1u << (x / kBmWords),
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                                 }
-                            }
+                            };
+                            if (s + 1 == R.nseg)
+                                insert(std::true_type{});
+                            else
+                                insert(std::false_type{});
                         }
                     }
                 }
@@ -794,10 +822,8 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
                     const uint32_t vi = cand_at(i), x = vi - lo;
-                    if (ln + 64 * i < nlive && vi >= lo && x <= hi - lo) {
-                        const BmPos bp = bm_pos(x);
-                        found[i] = (bm[bp.w] >> bp.b) & 1u;
-                    }
+                    if (ln + 64 * i < nlive && x <= span)
+                        found[i] = (bm[x & (kBmWords - 1u)] >> (x / kBmWords)) & 1u;
                 }
                 wave_sync();
             }
@@ -1526,6 +1552,13 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     const uint32_t l1_first = uni(ur.l1_first), l1_count = uni(ur.l1_count);
     const TermRec* qr = A.qrecs + uni(ur.toff);
     const TermRec R0 = qr[0];
+    // the second list's heads over the prologue's blocks, requested before the
+    // shortest list is decoded so that their load overlaps the decode
+    uint32_t pre_heads = kNone;
+    if constexpr (!raw_codec(CODEC) && CODEC != EXC_HS_SEGLRC) {
+        if (tc > 1 && l1_count != kUnset && l1_count <= 64 && ln < l1_count)
+            pre_heads = A.ix.seg_head[qr[1].segnum + l1_first + ln];
+    }
     uint32_t nlive;
     if constexpr (CODEC == EXC_HS_SEGLRC) {
         nlive = decode_hs_chunk<kChunk>(A.ix, R0, k, W.cand);
@@ -1552,7 +1585,8 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
         else if constexpr (CODEC == EXC_RAW_HS)
             nlive = probe_raw_hs(A, R, W, nlive, knobs);
         else
-            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs, j == 1 ? l1_first : 0u, j == 1 ? l1_count : kUnset);
+            nlive = probe_seg<CODEC, SEG>(A, R, W, nlive, knobs, j == 1 ? l1_first : 0u, j == 1 ? l1_count : kUnset,
+                                              pre_heads);
     }
     uint32_t* out = A.stage + uni(ur.soff);
     for (uint32_t e = ln; e < nlive; e += 64) out[e] = W.cand[e];
@@ -1622,9 +1656,10 @@ __device__ uint32_t claim_unit(const ProbeArgs& A, ClaimLds* q)
 // 16-byte claim record takes it to 7 workgroups per CU: eight would need
 // 8 x 20,496 bytes of the 160 KiB.
 template <int CODEC, int SEG, bool PERSIST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERSIST ? 7 : 8, PERSIST ? 7 : 8))) void k_probe(ProbeArgs A)
+__global__ __launch_bounds__(64 * kProbeWaves) __attribute__((amdgpu_waves_per_eu(PERSIST ? 7 : 8, PERSIST ? 7 : 8))) void k_probe(ProbeArgs A)
 {
-    __shared__ uint32_t lds[kWaves * wave_lds_words<CODEC>()];
+    // 2 KiB-aligned: every wave's window bitmap then starts at a 2 KiB boundary (bitmap_of)
+    __shared__ __attribute__((aligned(2048))) uint32_t lds[kProbeWaves * wave_lds_words<CODEC>()];
     const uint32_t wid = uni(threadIdx.x >> 6);
     uint32_t* L = lds + wid * wave_lds_words<CODEC>();
     if constexpr (PERSIST) {
@@ -1636,7 +1671,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERSIST ? 7
         __syncthreads();
         for (uint32_t u; (u = claim_unit(A, &q)) != kNone;) run_unit<CODEC, SEG>(A, u, L);
     } else {
-        const uint32_t u = blockIdx.x * kWaves + wid;
+        const uint32_t u = blockIdx.x * kProbeWaves + wid;
         if (u < A.nunits) run_unit<CODEC, SEG>(A, u, L);
     }
 }
@@ -2020,14 +2055,14 @@ hipError_t probe_for_seg(const ProbeArgs& a, uint32_t grid, hipStream_t s)
 {
     if (a.persistent) {
         if (a.ix.seg == 256)
-            hipLaunchKernelGGL((k_probe<CODEC, 256, true>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_probe<CODEC, 256, true>), dim3(grid), dim3(64 * kProbeWaves), 0, s, a);
         else
-            hipLaunchKernelGGL((k_probe<CODEC, 128, true>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_probe<CODEC, 128, true>), dim3(grid), dim3(64 * kProbeWaves), 0, s, a);
     } else {
         if (a.ix.seg == 256)
-            hipLaunchKernelGGL((k_probe<CODEC, 256, false>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_probe<CODEC, 256, false>), dim3(grid), dim3(64 * kProbeWaves), 0, s, a);
         else
-            hipLaunchKernelGGL((k_probe<CODEC, 128, false>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_probe<CODEC, 128, false>), dim3(grid), dim3(64 * kProbeWaves), 0, s, a);
     }
     return hipGetLastError();
 }
@@ -2048,14 +2083,14 @@ int occ_for(uint32_t seg)
 {
     int nb = 0;
     hipError_t e = (seg == 256)
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 256, true>), 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 128, true>), 256, 0);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 256, true>), 64 * kProbeWaves, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_probe<CODEC, 128, true>), 64 * kProbeWaves, 0);
     return (e == hipSuccess && nb > 0) ? nb : 4;
 }
 
 }  // namespace
 
-uint32_t probe_waves_per_block() { return kWaves; }
+uint32_t probe_waves_per_block() { return kProbeWaves; }
 
 int probe_blocks_per_cu(int codec, uint32_t seg)
 {
