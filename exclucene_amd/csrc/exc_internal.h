@@ -264,6 +264,7 @@ hipError_t launch_hs_lists(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_hs_fit(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_hs_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_heads_line(const EncodeArgs& a, hipStream_t s);
 // HS_SegLRC: the per-bucket records and tables of the encoder compacted to the
 // device layout (records of non-empty buckets, occupancy tables; kDenseBuckets)
 struct HsCompactArgs {

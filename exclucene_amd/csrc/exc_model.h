@@ -19,6 +19,36 @@
 
 namespace exc {
 
+// The list-level line of a d-gap (ParaPFD) list, derived from its block heads
+// at upload (the reference's ParaPFD format carries no list fit): docID v sits
+// near position (v - icpt) / slope, slope = docIDs per position between the
+// first and last head, and every head lies within [-rr, +rl] positions of the
+// line, widened by one block.  The probe's head window uses it as it uses the
+// SegLRC list fit, and verifies the window against the heads, so the line only
+// has to be close.  Lists of fewer than two blocks (or a flat line) get the
+// no-window values.  Host and device compute it with the same float steps.
+__host__ __device__ inline void heads_line(const uint32_t* hd, uint32_t nseg, uint32_t seg, float& slope, float& icpt,
+                                           float& rl, float& rr)
+{
+    slope = 1.0f;
+    icpt = 0.0f;
+    rl = rr = 4.0e9f;
+    if (nseg < 2 || hd[nseg - 1] <= hd[0]) return;
+    const float s = (float)((double)(hd[nseg - 1] - hd[0]) / ((double)(nseg - 1) * (double)seg));
+    if (!(s > 0.0f)) return;
+    const float c = (float)hd[0], inv = 1.0f / s;
+    float dl = 0.0f, dr = 0.0f;
+    for (uint32_t b = 0; b < nseg; ++b) {
+        const float d = ((float)hd[b] - c) * inv - (float)((double)b * (double)seg);
+        dl = d > dl ? d : dl;
+        dr = -d > dr ? -d : dr;
+    }
+    slope = s;
+    icpt = c;
+    rl = dl + (float)seg;
+    rr = dr + (float)seg;
+}
+
 constexpr uint32_t kModelPla = 1;
 constexpr uint32_t kModelMlp = 2;
 constexpr uint32_t kModelPlaQ = 3;

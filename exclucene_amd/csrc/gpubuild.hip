@@ -12,6 +12,7 @@
 #include "exc_encode.h"
 #include "exc_format.h"
 #include "exc_internal.h"
+#include "exc_model.h"
 
 namespace exc {
 namespace {
@@ -485,6 +486,18 @@ __global__ __launch_bounds__(256) void k_term_recs(EncodeArgs a)
     a.terms[t] = R;
 }
 
+// Thread per ParaPFD list: the list-level line through its block heads, as
+// the upload derives it (heads_line).
+__global__ __launch_bounds__(256) void k_heads_line(EncodeArgs a)
+{
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.nterms) return;
+    TermRec& R = a.terms[t];
+    float icpt;
+    heads_line(a.head + a.seg_num[t], a.seg_num[t + 1] - a.seg_num[t], a.seg, R.slope, icpt, R.range_l, R.range_r);
+    R.icpt = __float_as_int(icpt);
+}
+
 // ---------------------------------------------------------------- HS_SegLRC compaction
 
 // Thread per bucket (and one past the last): 1 if the bucket holds docIDs.
@@ -691,6 +704,13 @@ hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s)
 {
     if (!a.nterms) return hipSuccess;
     hipLaunchKernelGGL(k_term_recs, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_heads_line(const EncodeArgs& a, hipStream_t s)
+{
+    if (!a.nterms) return hipSuccess;
+    hipLaunchKernelGGL(k_heads_line, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

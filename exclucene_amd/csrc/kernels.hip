@@ -160,6 +160,48 @@ __device__ uint32_t wave_upper_bound(const uint32_t* __restrict__ a, uint32_t n,
     return lo + popc(__ballot(p));
 }
 
+// wave_upper_bound of two values at once: both searches advance a level per
+// load round, so their dependent loads overlap (one chain of rounds, not two).
+__device__ void wave_upper_bound2(const uint32_t* __restrict__ a, uint32_t n, uint32_t va, uint32_t vb, uint32_t& ra,
+                                  uint32_t& rb)
+{
+    const uint32_t ln = lane_id();
+    uint32_t lo_a = 0, hi_a = n, lo_b = 0, hi_b = n;
+    bool done_a = false, done_b = false;
+    ra = rb = 0;
+    while (!done_a || !done_b) {
+        // a side whose range is <= 64 wide takes its last level this round
+        const bool last_a = hi_a - lo_a <= 64, last_b = hi_b - lo_b <= 64;
+        const uint32_t st_a = last_a ? 1u : (hi_a - lo_a + 63) >> 6, st_b = last_b ? 1u : (hi_b - lo_b + 63) >> 6;
+        const uint32_t ia = lo_a + ln * st_a, ib = lo_b + ln * st_b;
+        const bool pa = !done_a && ia < hi_a && a[ia] <= va;
+        const bool pb = !done_b && ib < hi_b && a[ib] <= vb;
+        const uint32_t ca = popc(__ballot(pa)), cb = popc(__ballot(pb));
+        if (!done_a) {
+            if (last_a || ca == 0) {
+                ra = lo_a + (last_a ? ca : 0u);
+                done_a = true;
+            } else {
+                const uint32_t nlo = lo_a + (ca - 1) * st_a + 1, nhi = min(lo_a + ca * st_a, hi_a);
+                lo_a = uni(nlo);
+                hi_a = uni(nhi);
+            }
+        }
+        if (!done_b) {
+            if (last_b || cb == 0) {
+                rb = lo_b + (last_b ? cb : 0u);
+                done_b = true;
+            } else {
+                const uint32_t nlo = lo_b + (cb - 1) * st_b + 1, nhi = min(lo_b + cb * st_b, hi_b);
+                lo_b = uni(nlo);
+                hi_b = uni(nhi);
+            }
+        }
+    }
+    ra = uni(ra);
+    rb = uni(rb);
+}
+
 // Count of lanes of the sorted register array H (kNone padded) that are <= v,
 // v per lane: 6 shuffles of binary lifting plus the 64th lane.
 __device__ __forceinline__ uint32_t lane_count_le(uint32_t H, uint32_t h63, uint32_t v)
@@ -690,8 +732,8 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
         }
     }
     if (!small && pre_count == kUnset) {
-        const uint32_t ub_lo = uni(wave_upper_bound(head, (uint32_t)nseg, vmin));
-        const uint32_t ub_hi = uni(wave_upper_bound(head, (uint32_t)nseg, vmax));
+        uint32_t ub_lo, ub_hi;
+        wave_upper_bound2(head, (uint32_t)nseg, vmin, vmax, ub_lo, ub_hi);
         if (ub_hi == 0) return 0;  // every candidate precedes the list
         hb0 = ub_lo ? ub_lo - 1 : 0;
         rng = (int64_t)ub_hi - hb0;
@@ -1701,9 +1743,9 @@ __global__ __launch_bounds__(256) void k_unit_prologue(ProbeArgs A)
     // once two bracket loads confirm it holds the bound (a window that misses
     // falls back to the whole list, so the fit's rounding never matters)
     uint32_t lo_a = 0, len_a = n, lo_b = 0, len_b = n;
-    if ((A.ix.codec == EXC_SEGLRC || A.ix.codec == EXC_LRC) && n > 64) {
+    if ((A.ix.codec == EXC_SEGLRC || A.ix.codec == EXC_LRC || A.ix.codec == EXC_PARAPFD) && n > 64) {
         int64_t wa0, wb0, wa1, wb1;
-        if (A.ix.codec == EXC_SEGLRC) {
+        if (A.ix.codec != EXC_LRC) {  // SegLRC list fit, ParaPFD heads line: float intercept
             fit_window<EXC_SEGLRC, SEG>(R1, vmin, wa0, wb0);
             fit_window<EXC_SEGLRC, SEG>(R1, vmax, wa1, wb1);
         } else {

@@ -34,6 +34,7 @@
 #include "exc_format.h"
 #include "exc_internal.h"
 #include "exc_model.h"
+#include "exc_model.h"
 
 using namespace exc;
 
@@ -675,6 +676,11 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
             q.head = h->seg_head[g];
             // ParaPFD keeps the block's .segMedian row in rl (no fit ranges)
             if (codec == EXC_PARAPFD && h->seg_median) q.rl = h->seg_median[g];
+        }
+        if (codec == EXC_PARAPFD) {
+            float icpt;
+            heads_line(h->seg_head + R.segnum, R.nseg, seg, recs[t].slope, icpt, recs[t].range_l, recs[t].range_r);
+            std::memcpy(&recs[t].icpt, &icpt, 4);
         }
     }
 
@@ -1952,6 +1958,7 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
         if ((e = (codec == EXC_SEGLRC ? launch_seg_pack(a, c->stream) : launch_blk_pack(a, c->stream))) != hipSuccess)
             break;
     }
+    if (e == hipSuccess && codec == EXC_PARAPFD) e = launch_heads_line(a, c->stream);
     ix->recs.resize(nterms);
     if (e == hipSuccess)
         e = hipMemcpyAsync(ix->recs.data(), p_terms, (size_t)nterms * sizeof(TermRec), hipMemcpyDeviceToHost, c->stream);
