@@ -304,7 +304,8 @@ struct alignas(16) DecUnit {
     uint32_t term;    // term index (the HS decoder reads its bucket table)
     float slope;      // LRC list-level fit
     int32_t icpt;
-    uint32_t pad[2];
+    uint32_t wend;    // the list's words in .ind2 (kUnset: unknown)
+    uint32_t pad;
 };
 static_assert(sizeof(DecUnit) == 48, "DecUnit is 48 bytes");
 hipError_t launch_decompress(const DevIndex& ix, const DecUnit* units, uint32_t n, uint32_t* out, hipStream_t s);

@@ -2038,6 +2038,160 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
     }
 }
 
+// ---- whole-list decompression of the d-gap codecs (ParaPFD, NewPFD)
+// One wave (one workgroup, so its LDS frees with it) per unit of kDecodeUnit
+// docIDs.  The unit's block records come in one lane-parallel load (lane nb:
+// the next block's, whose offset ends the run; the list's last unit ends at
+// the list's word count instead); the packed words of all its blocks -- one
+// contiguous run of the list -- then come in one round of coalesced loads into
+// the wave's LDS, and every block decodes from there: fields and exceptions
+// read from LDS, the patch in a one-block stage, the segHead + scan in
+// registers, the values back through the stage and out in coalesced rows.
+// Two dependent HBM round trips per unit before the stores, and no packed
+// word held in a register while it is in flight.  A run longer than
+// kGapWords words (or a list whose word count is unknown) decodes its blocks
+// with the register path, one block at a time through the stage.
+constexpr uint32_t kGapWords = 1024;
+constexpr uint32_t kGapWaves = 4;  // waves per workgroup of the d-gap decoder
+
+template <int CODEC, int SEG>
+__global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(SEG == 256 ? 8 : 5, SEG == 256 ? 8 : 5))) void k_decode_gap(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
+{
+    static_assert(gap_codec(CODEC), "d-gap codecs only");
+    constexpr uint32_t U = kDecodeUnit / SEG;
+    constexpr int V = SEG / 64;
+    constexpr uint32_t XS = CODEC == EXC_NEWPFD ? 2 * SEG : 0;  // NewPFor exception scratch
+    // + 128: a block's fields are read for every position below SEG (up to
+    // 124 words past a short last block) and a NewPFor stream 64 words at a
+    // time (up to 63 past its end)
+    __shared__ __attribute__((aligned(16))) uint32_t lds_buf[kGapWaves][kGapWords + 128];
+    __shared__ uint32_t lds_stage[kGapWaves][SEG + XS];
+    const uint32_t wid = uni(threadIdx.x >> 6), u = blockIdx.x * kGapWaves + wid, ln = lane_id();
+    uint32_t* const buf = lds_buf[wid];
+    uint32_t* const stage = lds_stage[wid];
+    if (u >= nunits) return;
+    const DecUnit D = units[u];
+    const uint32_t k = uni(D.k), n = uni(D.n);
+    const uint32_t nseg = (n + SEG - 1) / SEG;
+    const uint32_t nb = uni(min(U, nseg - k));
+    const bool has_next = k + nb < nseg;
+    TermRec R;  // the fields seg_len and the register path read
+    R.woff = D.woff;
+    R.n = n;
+    R.segnum = D.segnum;
+    R.nseg = nseg;
+    const SegRec gr = ix.rec[D.segnum + k + min(ln, has_next ? nb : nb - 1u)];
+    const uint32_t w_first = uni(__builtin_amdgcn_readlane(gr.off, 0));
+    const uint32_t w_end = has_next ? uni(__builtin_amdgcn_readlane(gr.off, (int)nb)) : uni(D.wend);
+    uint32_t* o = out + D.out;
+    // the run is copied in 16-byte pieces from the 16-byte boundary at or
+    // before its first word; that word then sits at buf[sh]
+    const uint64_t gfirst = D.woff + w_first;
+    const uint32_t sh = (uint32_t)(gfirst & 3u);
+    const bool lds_run = w_end != kUnset && w_end >= w_first && w_end - w_first + sh <= kGapWords &&
+                         ((uintptr_t)ix.words & 15u) == 0;
+    if (lds_run) {
+        constexpr uint32_t kRounds = kGapWords / 256 + 1;
+        const uint32_t nq = (w_end - w_first + sh + 3) >> 2;
+        const uint4* src = reinterpret_cast<const uint4*>(ix.words + (gfirst - sh));
+        uint4 x[kRounds];
+#pragma unroll
+        for (uint32_t i = 0; i < kRounds; ++i) {
+            const uint32_t q = ln + 64 * i;
+            x[i] = q < nq ? src[q] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kRounds; ++i) {
+            const uint32_t q = ln + 64 * i;
+            if (q < nq) reinterpret_cast<uint4*>(buf)[q] = x[i];
+        }
+        wave_sync();
+    }
+    // ParaPFD with every block's exceptions below 64: the unit's blocks advance
+    // phase by phase, all of them in each phase, so their LDS reads, patches
+    // and scans overlap instead of forming one chain per block.  Fields and
+    // exceptions go to registers first; the run's LDS copy is then dead and
+    // its words become the unit's output staging.
+    const bool few_exc = !__ballot(ln < nb && desc_en(gr.desc) > 64u);
+    if constexpr (CODEC == EXC_PARAPFD && SEG * U <= kGapWords) {
+        if (lds_run && few_exc) {
+            uint32_t f[U][V], xq[U], xh[U], lbv[U], lenv[U];
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g) {
+                const uint32_t gs = min(g, nb - 1u);  // past nb: a copy of the last block, not used
+                const uint32_t off = uni(__builtin_amdgcn_readlane(gr.off, (int)gs));
+                const uint32_t desc = uni(__builtin_amdgcn_readlane(gr.desc, (int)gs));
+                const uint32_t lb = desc_lb(desc), en = desc_en(desc), eb = desc_eb(desc), hb = desc_hb(desc);
+                const uint32_t len = seg_len(R, k + gs, SEG);
+                const uint32_t* data = buf + sh + (off - w_first) + 1;
+                lbv[g] = lb;
+                lenv[g] = len;
+#pragma unroll
+                for (int t = 0; t < V; ++t) f[g][t] = bits_at(data, lb, ln * V + t);
+                const uint32_t* pp = data + words_for(len, lb);
+                const uint32_t e = min(ln, en ? en - 1u : 0u);
+                xq[g] = ln < en ? bits_at(pp, eb, e) : kNone;
+                xh[g] = bits_at(pp + words_for(en, eb), hb, e);
+            }
+            wave_sync();
+            uint32_t* L = buf;  // output staging: block g at L + g * SEG
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g)
+#pragma unroll
+                for (int t = 0; t < V; ++t) L[g * SEG + ln * V + t] = f[g][t];
+            wave_sync();
+            // exception e of block g: field[pos_e] |= high_e << lb (positions are distinct)
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g)
+                if (g < nb && xq[g] < lenv[g]) L[g * SEG + xq[g]] |= xh[g] << lbv[g];
+            wave_sync();
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g) {
+                const uint32_t head = uni(__builtin_amdgcn_readlane(gr.head, (int)min(g, nb - 1u)));
+                uint32_t acc = 0;
+#pragma unroll
+                for (int t = 0; t < V; ++t) {
+                    uint32_t x = L[g * SEG + ln * V + t];
+                    if (ln == 0 && t == 0) x = head;
+                    acc += x;
+                    f[g][t] = acc;
+                }
+                const uint32_t excl = wave_incl_scan(acc) - acc;
+#pragma unroll
+                for (int t = 0; t < V; ++t) f[g][t] += excl;
+            }
+            wave_sync();
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g)
+#pragma unroll
+                for (int t = 0; t < V; ++t) L[g * SEG + ln * V + t] = f[g][t];
+            wave_sync();
+            const uint32_t total = (nb - 1u) * SEG + seg_len(R, k + nb - 1u, SEG);
+            for (uint32_t e = ln; e < total; e += 64) o[e] = L[e];
+            return;
+        }
+    }
+    for (uint32_t g = 0; g < nb; ++g) {
+        SegRec rec;
+        rec.off = uni(__builtin_amdgcn_readlane(gr.off, (int)g));
+        rec.desc = uni(__builtin_amdgcn_readlane(gr.desc, (int)g));
+        rec.head = uni(__builtin_amdgcn_readlane(gr.head, (int)g));
+        rec.slope = 1.0f;
+        rec.icpt = 0.0f;
+        rec.rl = rec.rr = 0;
+        rec.base = 0;
+        const uint32_t s = k + g, len = seg_len(R, s, SEG);
+        // the block's fields: from the LDS copy of the run, or from HBM
+        const uint32_t* data = lds_run ? buf + sh + (rec.off - w_first) + 1 : ix.words + R.woff + rec.off + 1;
+        uint32_t f[V];
+        load_fields<CODEC, SEG>(data, desc_lb(rec.desc), len, f);
+        finish_block<CODEC, SEG>(R, s, rec, len, data, f, stage, stage + SEG);
+        wave_sync();
+        for (uint32_t e = ln; e < len; e += 64) o[g * SEG + e] = stage[e];
+        wave_sync();
+    }
+}
+
 // ---- whole-list decompression (the *_Decompression programs)
 // One wave per unit of kDecodeUnit docIDs: a run of kDecodeUnit/SEG blocks of
 // one list (the HS codec: one chunk of SEG elements).  Every block record and
@@ -2112,6 +2266,14 @@ hipError_t probe_for_seg(const ProbeArgs& a, uint32_t grid, hipStream_t s)
 template <int CODEC>
 hipError_t decomp_for_seg(const DevIndex& ix, const DecUnit* units, uint32_t n, uint32_t* out, hipStream_t s)
 {
+    if constexpr (CODEC == EXC_PARAPFD) {
+        const uint32_t grid = (n + kGapWaves - 1) / kGapWaves;
+        if (ix.seg == 256)
+            hipLaunchKernelGGL((k_decode_gap<CODEC, 256>), dim3(grid), dim3(64 * kGapWaves), 0, s, ix, units, n, out);
+        else
+            hipLaunchKernelGGL((k_decode_gap<CODEC, 128>), dim3(grid), dim3(64 * kGapWaves), 0, s, ix, units, n, out);
+        return hipGetLastError();
+    }
     const uint32_t grid = (n + kWaves - 1) / kWaves;
     if (ix.seg == 256)
         hipLaunchKernelGGL((k_decompress<CODEC, 256>), dim3(grid), dim3(256), 0, s, ix, units, n, out);

@@ -112,6 +112,7 @@ struct exc_index {
     uint64_t hash_words = 0;       // .HashSeg_info words (HS codecs)
     std::vector<uint32_t> n;       // list lengths (host copy for planning)
     std::vector<TermRec> recs;     // host copy of the device term records
+    std::vector<uint32_t> lwords;  // .ind2 words of each list (empty: unknown)
     TermRec* d_terms = nullptr;
     std::vector<Model> models;     // learned models (host copy)
     DevBuf d_models, d_params;
@@ -489,6 +490,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
 
     // ---- validate the layout so that every device read stays inside its array
     std::vector<TermRec> recs(nt);
+    std::vector<uint32_t> lwords(nt, 0);
     std::vector<uint32_t> s16_scratch;  // NewPFor exception streams, decoded for the check
     const uint64_t W = h->ind2_words;
     for (uint32_t t = 0; t < nt; ++t) {
@@ -497,6 +499,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
         const uint64_t w0 = r.m_off / 4;
         const uint64_t w1 = (t + 1 < nt) ? h->ind1[t + 1].m_off / 4 : W;
         if (w0 > w1 || w1 > W) return set_error(EXC_ERR_FORMAT, "term %u: list words outside .ind2", t);
+        lwords[t] = (uint32_t)std::min<uint64_t>(w1 - w0, 0xffffffffull);
         const uint32_t n = r.m_urlcount;
         TermRec& R = recs[t];
         std::memset(&R, 0, sizeof R);
@@ -709,6 +712,7 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
     ix->dix.terms = (const TermRec*)up(recs.data(), recs.size() * sizeof(TermRec));
     ix->d_terms = const_cast<TermRec*>(ix->dix.terms);
     ix->recs = recs;
+    ix->lwords = lwords;
     ix->dix.rec = (const SegRec*)up(srec.data(), srec.size() * sizeof(SegRec));
     if (hs) {
         ix->dix.seg_head = (const uint32_t*)up(chead.data(), chead.size() * 4);
@@ -1479,6 +1483,7 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
             d.term = terms[i];
             d.slope = R.slope;
             d.icpt = R.icpt;
+            d.wend = terms[i] < ix->lwords.size() ? ix->lwords[terms[i]] : kUnset;
             units.push_back(d);
         }
     }
@@ -1967,6 +1972,10 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
     ix->nwords = W;
     ix->n.resize(nterms);
     for (uint32_t t = 0; t < nterms; ++t) ix->n[t] = ix->recs[t].n;
+    // the device builder packs the lists back to back in term order
+    ix->lwords.resize(nterms);
+    for (uint32_t t = 0; t < nterms; ++t)
+        ix->lwords[t] = (uint32_t)((t + 1 < nterms ? ix->recs[t + 1].woff : W) - ix->recs[t].woff);
     ix->dix.codec = codec;
     ix->dix.seg = seg;
     ix->dix.flags = codec == EXC_PARAPFD ? kIxMedians : 0u;  // k_blk_fit keeps each block's median
