@@ -293,7 +293,8 @@ __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint
         const uint32_t e = e0 + ln;
         const uint32_t d = e < en ? xs[e] + (e ? 1u : 0u) : 0u;
         const uint32_t p = wave_incl_scan(d) + carry;
-        if (e < en && p < len) dst[p] += xs[en + e] << lb;
+        if (e < en && p < len)
+            __hip_atomic_fetch_add(dst + p, xs[en + e] << lb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         carry = uni(__builtin_amdgcn_readlane(p, 63));
     }
 }
@@ -338,7 +339,8 @@ __device__ __forceinline__ void pfd_patch_lds(const uint32_t* __restrict__ data,
     for (uint32_t e = lane_id(); e < en; e += 64) {
         const uint32_t p = (PREFETCH && e < 64) ? xp.a : bits_at(pp, eb, e);
         const uint32_t h = (PREFETCH && e < 64) ? xp.b : bits_at(hp, hb, e);
-        if (p < len) dst[p] |= h << lb;
+        // an LDS atomic OR (no return): no read round trip before the write
+        if (p < len) __hip_atomic_fetch_or(dst + p, h << lb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
 }
 
@@ -477,6 +479,70 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
                 finish_block<CODEC, SEG, PREFETCH>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG,
                                                    dst + NB * SEG, xp[g]);
     }
+}
+
+// The blocks s0 .. s0+r-1 (r <= NB) of a ParaPFD list decoded into dst
+// (block g at dst + g * SEG), phase by phase over all of them: every block's
+// fields and first 64 exceptions are requested in one load round, all are
+// staged in dst, patched, read back, scanned and written, so their LDS round
+// trips overlap instead of forming one chain per block (the whole-list
+// decoder showed the same ordering 28 % faster).  Exceptions past the first
+// 64 of a block are read in pfd_patch_lds's loop.
+template <int SEG, int NB>
+__device__ void decode_pfd_phased(const DevIndex& ix, const TermRec& R, uint32_t s0, uint32_t r, uint32_t* dst)
+{
+    constexpr int V = SEG / 64;
+    const uint32_t ln = lane_id();
+    const SegRec gr = ix.rec[R.segnum + s0 + min(ln, r - 1)];
+    uint32_t desc[NB], head[NB], len[NB], f[NB][V];
+    const uint32_t* data[NB];
+    ExcPrefetch xp[NB];
+#pragma unroll
+    for (int g = 0; g < NB; ++g) {
+        const int src = (int)min((uint32_t)g, r - 1);  // past r: a copy of the last block, not used
+        desc[g] = uni(__builtin_amdgcn_readlane(gr.desc, src));
+        head[g] = uni(__builtin_amdgcn_readlane(gr.head, src));
+        len[g] = seg_len(R, s0 + (uint32_t)src, SEG);
+        data[g] = ix.words + R.woff + uni(__builtin_amdgcn_readlane(gr.off, src)) + 1;
+        load_fields<EXC_PARAPFD, SEG, false>(data[g], desc_lb(desc[g]), len[g], f[g]);
+        xp[g] = prefetch_exceptions<EXC_PARAPFD>(data[g], desc[g], len[g]);
+    }
+    bool exc = false;
+#pragma unroll
+    for (int g = 0; g < NB; ++g) exc = exc || ((uint32_t)g < r && desc_en(desc[g]));
+    if (exc) {
+#pragma unroll
+        for (int g = 0; g < NB; ++g)
+#pragma unroll
+            for (int t = 0; t < V; ++t) dst[g * SEG + ln * V + t] = f[g][t];
+        wave_sync();
+#pragma unroll
+        for (int g = 0; g < NB; ++g)
+            if ((uint32_t)g < r && desc_en(desc[g])) pfd_patch_lds<true>(data[g], desc[g], len[g], xp[g], dst + g * SEG);
+        wave_sync();
+#pragma unroll
+        for (int g = 0; g < NB; ++g)
+#pragma unroll
+            for (int t = 0; t < V; ++t) f[g][t] = dst[g * SEG + ln * V + t];
+    }
+#pragma unroll
+    for (int g = 0; g < NB; ++g) {
+        if (ln == 0) f[g][0] = head[g];
+        uint32_t acc = 0;
+#pragma unroll
+        for (int t = 0; t < V; ++t) { acc += f[g][t]; f[g][t] = acc; }
+        const uint32_t excl = wave_incl_scan(acc) - acc;
+#pragma unroll
+        for (int t = 0; t < V; ++t) f[g][t] += excl;
+    }
+    wave_sync();
+#pragma unroll
+    for (int g = 0; g < NB; ++g)
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+            const uint32_t j = ln * V + t;
+            if ((uint32_t)g < r && j < len[g]) dst[g * SEG + j] = f[g][t];
+        }
 }
 
 // ------------------------------------------------------------ random access
@@ -1612,8 +1678,10 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
     } else {
         const uint32_t s0 = k * U;
         const uint32_t nu = uni(min(U, R0.nseg - s0));
-        decode_blocks<CODEC, SEG, U, CODEC == EXC_PARAPFD>(A.ix, R0, nu, [&](int g) { return s0 + (uint32_t)g; },
-                                                           W.cand);
+        if constexpr (CODEC == EXC_PARAPFD)
+            decode_pfd_phased<SEG, (int)U>(A.ix, R0, s0, nu, W.cand);
+        else
+            decode_blocks<CODEC, SEG, U>(A.ix, R0, nu, [&](int g) { return s0 + (uint32_t)g; }, W.cand);
         nlive = (nu - 1) * SEG + seg_len(R0, s0 + nu - 1, SEG);
     }
     nlive = uni(nlive);
@@ -2038,37 +2106,40 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
     }
 }
 
-// ---- whole-list decompression of the d-gap codecs (ParaPFD, NewPFD)
-// One wave (one workgroup, so its LDS frees with it) per unit of kDecodeUnit
-// docIDs.  The unit's block records come in one lane-parallel load (lane nb:
-// the next block's, whose offset ends the run; the list's last unit ends at
-// the list's word count instead); the packed words of all its blocks -- one
-// contiguous run of the list -- then come in one round of coalesced loads into
-// the wave's LDS, and every block decodes from there: fields and exceptions
-// read from LDS, the patch in a one-block stage, the segHead + scan in
-// registers, the values back through the stage and out in coalesced rows.
-// Two dependent HBM round trips per unit before the stores, and no packed
-// word held in a register while it is in flight.  A run longer than
-// kGapWords words (or a list whose word count is unknown) decodes its blocks
-// with the register path, one block at a time through the stage.
+// ---- whole-list decompression of ParaPFD lists
+// One wave per unit of kDecodeUnit docIDs.  The unit's block records come in
+// one lane-parallel load (lane nb: the next block's, whose offset ends the
+// run; the list's last unit ends at the list's word count instead); the packed
+// words of all its blocks -- one contiguous run of the list -- then come in one
+// round of 16-byte loads into the wave's LDS.  All the unit's blocks then
+// advance phase by phase (fields and exceptions from LDS to registers, stage,
+// patch, read back, segHead + scan, values back through LDS, coalesced rows
+// out), so their LDS round trips overlap instead of forming one chain per
+// block: C2 ParaPFD-128 0.826 -> 0.57 ms, ParaPFD-256 0.722 -> 0.42 ms.  A run
+// longer than kGapWords words (or a list whose word count is unknown, or a
+// block with more than 64 exceptions) decodes its blocks from HBM one at a
+// time, staged in the same buffer.  (NewPFD keeps k_decompress: the same phase
+// ordering with the field-parallel Simple16 unpack of two blocks' streams
+// measured 1.17 ms against 0.83.)
 constexpr uint32_t kGapWords = 1024;
 constexpr uint32_t kGapWaves = 4;  // waves per workgroup of the d-gap decoder
+// waves per SIMD each instance is compiled for (its phase path's registers
+// without spills; measured on the C2 lists)
+template <int SEG>
+constexpr int kGapOcc = SEG == 256 ? 7 : 5;
 
 template <int CODEC, int SEG>
-__global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(SEG == 256 ? 8 : 5, SEG == 256 ? 8 : 5))) void k_decode_gap(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
+__global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(kGapOcc<SEG>, kGapOcc<SEG>))) void k_decode_gap(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
 {
-    static_assert(gap_codec(CODEC), "d-gap codecs only");
+    static_assert(CODEC == EXC_PARAPFD, "ParaPFD lists only");
     constexpr uint32_t U = kDecodeUnit / SEG;
     constexpr int V = SEG / 64;
-    constexpr uint32_t XS = CODEC == EXC_NEWPFD ? 2 * SEG : 0;  // NewPFor exception scratch
     // + 128: a block's fields are read for every position below SEG (up to
-    // 124 words past a short last block) and a NewPFor stream 64 words at a
-    // time (up to 63 past its end)
+    // 124 words past a short last block)
+    static_assert(SEG <= kGapWords, "the per-block path stages in the run's buffer");
     __shared__ __attribute__((aligned(16))) uint32_t lds_buf[kGapWaves][kGapWords + 128];
-    __shared__ uint32_t lds_stage[kGapWaves][SEG + XS];
     const uint32_t wid = uni(threadIdx.x >> 6), u = blockIdx.x * kGapWaves + wid, ln = lane_id();
     uint32_t* const buf = lds_buf[wid];
-    uint32_t* const stage = lds_stage[wid];
     if (u >= nunits) return;
     const DecUnit D = units[u];
     const uint32_t k = uni(D.k), n = uni(D.n);
@@ -2107,13 +2178,14 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
         }
         wave_sync();
     }
-    // ParaPFD with every block's exceptions below 64: the unit's blocks advance
-    // phase by phase, all of them in each phase, so their LDS reads, patches
-    // and scans overlap instead of forming one chain per block.  Fields and
-    // exceptions go to registers first; the run's LDS copy is then dead and
-    // its words become the unit's output staging.
+    // The unit's blocks advance phase by phase, all of them in each phase, so
+    // their LDS reads, patches and scans overlap instead of forming one chain
+    // per block.  Fields and each block's exceptions (at most 64) go to
+    // registers first; the run's LDS copy is then dead and its words become
+    // the unit's output staging.  Units with more exceptions in a block take
+    // the per-block path below.
     const bool few_exc = !__ballot(ln < nb && desc_en(gr.desc) > 64u);
-    if constexpr (CODEC == EXC_PARAPFD && SEG * U <= kGapWords) {
+    if constexpr (SEG * U <= kGapWords) {
         if (lds_run && few_exc) {
             uint32_t f[U][V], xq[U], xh[U], lbv[U], lenv[U];
 #pragma unroll
@@ -2121,7 +2193,7 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
                 const uint32_t gs = min(g, nb - 1u);  // past nb: a copy of the last block, not used
                 const uint32_t off = uni(__builtin_amdgcn_readlane(gr.off, (int)gs));
                 const uint32_t desc = uni(__builtin_amdgcn_readlane(gr.desc, (int)gs));
-                const uint32_t lb = desc_lb(desc), en = desc_en(desc), eb = desc_eb(desc), hb = desc_hb(desc);
+                const uint32_t lb = desc_lb(desc), en = desc_en(desc);
                 const uint32_t len = seg_len(R, k + gs, SEG);
                 const uint32_t* data = buf + sh + (off - w_first) + 1;
                 lbv[g] = lb;
@@ -2129,46 +2201,50 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                 for (int t = 0; t < V; ++t) f[g][t] = bits_at(data, lb, ln * V + t);
                 const uint32_t* pp = data + words_for(len, lb);
-                const uint32_t e = min(ln, en ? en - 1u : 0u);
+                const uint32_t eb = desc_eb(desc), hb = desc_hb(desc), e = min(ln, en ? en - 1u : 0u);
                 xq[g] = ln < en ? bits_at(pp, eb, e) : kNone;
                 xh[g] = bits_at(pp + words_for(en, eb), hb, e);
             }
-            wave_sync();
-            uint32_t* L = buf;  // output staging: block g at L + g * SEG
+            {
+                wave_sync();
+                uint32_t* L = buf;  // output staging: block g at L + g * SEG
 #pragma unroll
-            for (uint32_t g = 0; g < U; ++g)
+                for (uint32_t g = 0; g < U; ++g)
 #pragma unroll
-                for (int t = 0; t < V; ++t) L[g * SEG + ln * V + t] = f[g][t];
-            wave_sync();
-            // exception e of block g: field[pos_e] |= high_e << lb (positions are distinct)
+                    for (int t = 0; t < V; ++t) L[g * SEG + ln * V + t] = f[g][t];
+                wave_sync();
+                // exception e of block g: field[pos_e] |= high_e << lb (positions are distinct)
 #pragma unroll
-            for (uint32_t g = 0; g < U; ++g)
-                if (g < nb && xq[g] < lenv[g]) L[g * SEG + xq[g]] |= xh[g] << lbv[g];
-            wave_sync();
+                for (uint32_t g = 0; g < U; ++g)
+                    if (g < nb && xq[g] < lenv[g])
+                        __hip_atomic_fetch_or(L + g * SEG + xq[g], xh[g] << lbv[g], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WAVEFRONT);
+                wave_sync();
 #pragma unroll
-            for (uint32_t g = 0; g < U; ++g) {
-                const uint32_t head = uni(__builtin_amdgcn_readlane(gr.head, (int)min(g, nb - 1u)));
-                uint32_t acc = 0;
+                for (uint32_t g = 0; g < U; ++g) {
+                    const uint32_t head = uni(__builtin_amdgcn_readlane(gr.head, (int)min(g, nb - 1u)));
+                    uint32_t acc = 0;
 #pragma unroll
-                for (int t = 0; t < V; ++t) {
-                    uint32_t x = L[g * SEG + ln * V + t];
-                    if (ln == 0 && t == 0) x = head;
-                    acc += x;
-                    f[g][t] = acc;
+                    for (int t = 0; t < V; ++t) {
+                        uint32_t x = L[g * SEG + ln * V + t];
+                        if (ln == 0 && t == 0) x = head;
+                        acc += x;
+                        f[g][t] = acc;
+                    }
+                    const uint32_t excl = wave_incl_scan(acc) - acc;
+#pragma unroll
+                    for (int t = 0; t < V; ++t) f[g][t] += excl;
                 }
-                const uint32_t excl = wave_incl_scan(acc) - acc;
+                wave_sync();
 #pragma unroll
-                for (int t = 0; t < V; ++t) f[g][t] += excl;
+                for (uint32_t g = 0; g < U; ++g)
+#pragma unroll
+                    for (int t = 0; t < V; ++t) L[g * SEG + ln * V + t] = f[g][t];
+                wave_sync();
+                const uint32_t total = (nb - 1u) * SEG + seg_len(R, k + nb - 1u, SEG);
+                for (uint32_t e = ln; e < total; e += 64) o[e] = L[e];
+                return;
             }
-            wave_sync();
-#pragma unroll
-            for (uint32_t g = 0; g < U; ++g)
-#pragma unroll
-                for (int t = 0; t < V; ++t) L[g * SEG + ln * V + t] = f[g][t];
-            wave_sync();
-            const uint32_t total = (nb - 1u) * SEG + seg_len(R, k + nb - 1u, SEG);
-            for (uint32_t e = ln; e < total; e += 64) o[e] = L[e];
-            return;
         }
     }
     for (uint32_t g = 0; g < nb; ++g) {
@@ -2181,11 +2257,13 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
         rec.rl = rec.rr = 0;
         rec.base = 0;
         const uint32_t s = k + g, len = seg_len(R, s, SEG);
-        // the block's fields: from the LDS copy of the run, or from HBM
-        const uint32_t* data = lds_run ? buf + sh + (rec.off - w_first) + 1 : ix.words + R.woff + rec.off + 1;
+        // the block from HBM, staged in the buffer
+        const uint32_t* data = ix.words + R.woff + rec.off + 1;
+        uint32_t* const stage = buf;
         uint32_t f[V];
         load_fields<CODEC, SEG>(data, desc_lb(rec.desc), len, f);
-        finish_block<CODEC, SEG>(R, s, rec, len, data, f, stage, stage + SEG);
+        wave_sync();  // the previous block's (or the phase path's) reads of the buffer come first
+        finish_block<CODEC, SEG>(R, s, rec, len, data, f, stage, nullptr);
         wave_sync();
         for (uint32_t e = ln; e < len; e += 64) o[g * SEG + e] = stage[e];
         wave_sync();
