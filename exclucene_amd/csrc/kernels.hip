@@ -1083,14 +1083,20 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             if (!__ballot(any)) break;
         }
     } else {
-        const uint32_t h63 = uni(__builtin_amdgcn_readlane(H, 63));
+        // beyond 64 heads: every stride-th head of the range in one load round
+        // (lane l: head l * stride), shared by all candidate rows; a candidate
+        // then binary-searches only the stride heads between two samples
+        const uint32_t stride = small ? 1u : uni((uint32_t)((rng + 63) >> 6));
+        const uint32_t G = small ? H : (ln * stride < (uint32_t)rng ? head[hb0 + ln * stride] : kNone);
+        const uint32_t h63 = uni(__builtin_amdgcn_readlane(G, 63));
         // segment of a candidate (-1: precedes the list, -2: no candidate)
         const auto seg_of = [&](uint32_t vi, bool live) -> int32_t {
             int64_t cnt;
             if (small) {
                 cnt = lane_count_le(H, h63, vi);
             } else {
-                uint32_t lo = 0, hi = (uint32_t)rng;
+                const uint32_t c = lane_count_le(G, h63, vi);  // samples <= vi
+                uint32_t lo = c ? (c - 1) * stride + 1 : 0u, hi = c ? min(c * stride, (uint32_t)rng) : 0u;
                 if (live)
                     while (lo < hi) {
                         const uint32_t m = (lo + hi) >> 1;
