@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # k_probe<SegLRC,128> HBM bytes per launch, in KiB, from rocprofv3 --pmc FETCH_SIZE then WRITE_SIZE
 # runs of this bench (FETCH x1.99996 and WRITE x1.0, a 1 GiB copy kernel's calibration); they are
 # profile figures reported beside this run's, not measured in it
-TRAFFIC_C2_KIB = 1302880.0
+TRAFFIC_C2_KIB = 1469502.0
 TRAFFIC_C2_SOURCE = "profiles/k_probe_seglrc128_c2.md (rocprofv3 --pmc on the C2 batch)"
-TRAFFIC_C5_KIB = 5091503.0
+TRAFFIC_C5_KIB = 5950743.0
 TRAFFIC_C5_SOURCE = "profiles/k_probe_seglrc128_c5.md (rocprofv3 --pmc on the C5 batch)"
 META = {0: 8, 1: 28, 2: 32, 3: 8, 4: 8}  # per-block side data bytes: LRC, SegLRC, HS, ParaPFD, NewPFD
 CODEC_NAME = {0: "LRC", 1: "SegLRC", 2: "HS_SegLRC", 3: "ParaPFD", 4: "NewPFD"}
@@ -281,7 +281,8 @@ def decode_leg(ea, ctx, dix, built, docs, reps=5):
     alg = len(built.ind2) * 4 + nblocks * META[built.codec] + total * 4
     same = bool(np.array_equal(ctx.from_device(out, total), docs))
     ctx.device_free(out)
-    return {"kernel": "k_decompress<%s,%d>" % (CODEC_NAME[built.codec], built.seg), "docids": total,
+    kernel = "k_decode_gap" if built.codec == 3 else "k_decompress"  # ParaPFD lists: the phase-ordered decoder
+    return {"kernel": "%s<%s,%d>" % (kernel, CODEC_NAME[built.codec], built.seg), "docids": total,
             "bits_per_docid": 8.0 * (len(built.ind2) * 4 + nblocks * META[built.codec]) / total,
             "kernel_ms": float(ms.mean()),
             "algorithmic_bytes": int(alg), "achieved_GBps": alg / (ms.mean() * 1e-3) / 1e9,
