@@ -607,6 +607,42 @@ __device__ __forceinline__ uint32_t block_value(const TermRec& R, uint32_t s, co
         return f + seg_pred(j, rec.slope, rec.icpt, rec.base);
 }
 
+// The blocks s0 .. s0+r-1 (r <= NB) of an LRC / SegLRC list decoded into dst
+// (block g at dst + g * SEG), dst holding NB * SEG words: every lane stores
+// every position of every block, past a block's length and past r too (a
+// copy of the last block), so no store waits behind a per-lane branch; the
+// words past the run are never read as docIDs (the probe's candidate count
+// stops before them).  Records in scalar registers, one load each, all
+// requested before the first field.
+template <int CODEC, int SEG, int NB>
+__device__ void decode_ra_padded(const DevIndex& ix, const TermRec& R, uint32_t s0, uint32_t r, uint32_t* dst)
+{
+    constexpr int V = SEG / 64;
+    const uint32_t ln = lane_id();
+    SegRec rec[NB];
+    uint32_t f[NB][V];
+#pragma unroll
+    for (int g = 0; g < NB; ++g) rec[g] = ix.rec[R.segnum + s0 + min((uint32_t)g, r - 1)];
+#pragma unroll
+    for (int g = 0; g < NB; ++g)
+        load_fields<CODEC, SEG, false>(ix.words + R.woff + rec[g].off + 1, desc_lb(rec[g].desc), SEG, f[g]);
+#pragma unroll
+    for (int g = 0; g < NB; ++g) {
+        const uint32_t s = s0 + min((uint32_t)g, r - 1);
+        const float icv = vgpr_copy(rec[g].icpt);
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+            const uint32_t j = ln + 64 * t;
+            uint32_t x;
+            if constexpr (CODEC == EXC_SEGLRC)
+                x = f[g][t] + seg_pred(j, rec[g].slope, icv, rec[g].base);
+            else
+                x = block_value<CODEC, SEG>(R, s, rec[g], j, f[g][t]);
+            dst[g * SEG + j] = x;
+        }
+    }
+}
+
 // Value j of a SegLRC block whose packed words from w0 on are staged in LDS,
 // word k at col[64 k] (one column per lane).
 __device__ __forceinline__ uint32_t win_value(const uint32_t* col, uint32_t lb, uint32_t w0, uint32_t j,
@@ -822,9 +858,12 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
 
     // candidates are re-read from LDS where needed instead of being held in
     // registers through the bitmap path (its register peak)
+    // (the read is unconditional -- every slot below kChunk is a valid LDS
+    // word -- and the result selected, so no lane branches round it)
     const auto cand_at = [&](int i) {
         const uint32_t e = ln + 64 * i;
-        return e < nlive ? cand[e] : kNone;
+        const uint32_t x = cand[e];
+        return e < nlive ? x : kNone;
     };
     bool found[V];
 #pragma unroll
@@ -1118,7 +1157,7 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             for (int i = 0; i < V; ++i) {
                 const uint32_t e = ln + 64u * (uint32_t)i;
                 if (!__ballot(e < nlive)) break;  // rows fill in order
-                const uint32_t vi = e < nlive ? cand[e] : kNone;
+                const uint32_t vi = cand_at(i);
                 const int32_t sc = seg_of(vi, e < nlive);
                 if (!__ballot(sc >= 0)) continue;
                 const uint32_t si = (uint32_t)(sc >= 0 ? sc : (int32_t)sf);
@@ -1292,7 +1331,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
     for (int i = 0; i < V; ++i) {
         const uint64_t m = __ballot(found[i]);
         const uint32_t vi = cand_at(i);
-        if (found[i]) cand[cnt + below_me(m)] = vi;
+        // lanes without a survivor write into the buffer behind the candidates
+        // (free between lists) instead of branching round the store
+        cand[found[i] ? cnt + below_me(m) : kChunk + ln] = vi;
         cnt += popc(m);
     }
     wave_sync();
@@ -1687,7 +1728,7 @@ __device__ void run_unit(const ProbeArgs& A, uint32_t u, uint32_t* L)
         if constexpr (CODEC == EXC_PARAPFD)
             decode_pfd_phased<SEG, (int)U>(A.ix, R0, s0, nu, W.cand);
         else
-            decode_blocks<CODEC, SEG, U>(A.ix, R0, nu, [&](int g) { return s0 + (uint32_t)g; }, W.cand);
+            decode_ra_padded<CODEC, SEG, (int)U>(A.ix, R0, s0, nu, W.cand);
         nlive = (nu - 1) * SEG + seg_len(R0, s0 + nu - 1, SEG);
     }
     nlive = uni(nlive);
