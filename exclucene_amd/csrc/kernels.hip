@@ -905,8 +905,11 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                 const uint32_t b0 = c_lo > hoff ? c_lo - 1 - hoff : 0;
                 const uint32_t b1 = min(c_hi > hoff ? c_hi - 1 - hoff : 0, r - 1);
                 const uint32_t span = hi - lo;
-                const uint32_t nw = min(kBmWords, span + 1);  // offset x sits in word x % kBmWords
-                for (uint32_t e = ln; e < nw; e += 64) bm[e] = 0;
+                // the whole bitmap cleared in two 16-byte stores a lane (a narrow
+                // window uses fewer words, but a fixed count needs no loop)
+                static_assert(kBmWords == 2 * 64 * 4, "two uint4 stores a lane clear the bitmap");
+                reinterpret_cast<uint4*>(bm)[ln] = make_uint4(0, 0, 0, 0);
+                reinterpret_cast<uint4*>(bm)[64 + ln] = make_uint4(0, 0, 0, 0);
                 wave_sync();
                 for (uint32_t p0 = b0; p0 <= b1; p0 += NBB) {
                     const uint32_t np = min((uint32_t)NBB, b1 + 1 - p0);
