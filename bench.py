@@ -281,7 +281,8 @@ def decode_leg(ea, ctx, dix, built, docs, reps=5):
     alg = len(built.ind2) * 4 + nblocks * META[built.codec] + total * 4
     same = bool(np.array_equal(ctx.from_device(out, total), docs))
     ctx.device_free(out)
-    kernel = "k_decode_gap" if built.codec == 3 else "k_decompress"  # ParaPFD lists: the phase-ordered decoder
+    # ParaPFD, SegLRC and LRC lists: the run decoder (packed run via LDS); the others k_decompress
+    kernel = "k_decode_run" if built.codec in (0, 1, 3) else "k_decompress"
     return {"kernel": "%s<%s,%d>" % (kernel, CODEC_NAME[built.codec], built.seg), "docids": total,
             "bits_per_docid": 8.0 * (len(built.ind2) * 4 + nblocks * META[built.codec]) / total,
             "kernel_ms": float(ms.mean()),

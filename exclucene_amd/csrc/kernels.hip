@@ -2156,32 +2156,34 @@ __global__ __launch_bounds__(256) void k_scatter(ScanArgs a)
     }
 }
 
-// ---- whole-list decompression of ParaPFD lists
+// ---- whole-list decompression of the ParaPFor-layout codecs (ParaPFD, SegLRC, LRC)
 // One wave per unit of kDecodeUnit docIDs.  The unit's block records come in
 // one lane-parallel load (lane nb: the next block's, whose offset ends the
 // run; the list's last unit ends at the list's word count instead); the packed
 // words of all its blocks -- one contiguous run of the list -- then come in one
-// round of 16-byte loads into the wave's LDS.  All the unit's blocks then
-// advance phase by phase (fields and exceptions from LDS to registers, stage,
-// patch, read back, segHead + scan, values back through LDS, coalesced rows
-// out), so their LDS round trips overlap instead of forming one chain per
+// round of 16-byte loads into the wave's LDS.  LRC / SegLRC: every block's
+// fields go from LDS to registers and each value is formed and stored
+// straight to HBM (C2 SegLRC-128 0.561 -> 0.405 ms).  ParaPFD: all the unit's
+// blocks advance phase by phase (fields and exceptions from LDS to registers,
+// stage, patch, read back, segHead + scan, values back through LDS, coalesced
+// rows out), so their LDS round trips overlap instead of forming one chain per
 // block: C2 ParaPFD-128 0.826 -> 0.57 ms, ParaPFD-256 0.722 -> 0.42 ms.  A run
 // longer than kGapWords words (or a list whose word count is unknown, or a
-// block with more than 64 exceptions) decodes its blocks from HBM one at a
-// time, staged in the same buffer.  (NewPFD keeps k_decompress: the same phase
-// ordering with the field-parallel Simple16 unpack of two blocks' streams
-// measured 1.17 ms against 0.83.)
+// ParaPFD block with more than 64 exceptions) decodes its blocks from HBM one
+// at a time, staged in the same buffer.  (NewPFD keeps k_decompress: the same
+// phase ordering with the field-parallel Simple16 unpack of two blocks'
+// streams measured 1.17 ms against 0.83.)
 constexpr uint32_t kGapWords = 1024;
-constexpr uint32_t kGapWaves = 4;  // waves per workgroup of the d-gap decoder
+constexpr uint32_t kGapWaves = 4;  // waves per workgroup of the run decoder
 // waves per SIMD each instance is compiled for (its phase path's registers
 // without spills; measured on the C2 lists)
-template <int SEG>
-constexpr int kGapOcc = SEG == 256 ? 7 : 5;
+template <int CODEC, int SEG>
+constexpr int kGapOcc = !gap_codec(CODEC) ? 8 : SEG == 256 ? 7 : 5;
 
 template <int CODEC, int SEG>
-__global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(kGapOcc<SEG>, kGapOcc<SEG>))) void k_decode_gap(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
+__global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(kGapOcc<CODEC, SEG>, kGapOcc<CODEC, SEG>))) void k_decode_run(DevIndex ix, const DecUnit* units, uint32_t nunits, uint32_t* out)
 {
-    static_assert(CODEC == EXC_PARAPFD, "ParaPFD lists only");
+    static_assert(CODEC == EXC_PARAPFD || CODEC == EXC_SEGLRC || CODEC == EXC_LRC, "block codecs with a ParaPFor layout");
     constexpr uint32_t U = kDecodeUnit / SEG;
     constexpr int V = SEG / 64;
     // + 128: a block's fields are read for every position below SEG (up to
@@ -2196,11 +2198,13 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
     const uint32_t nseg = (n + SEG - 1) / SEG;
     const uint32_t nb = uni(min(U, nseg - k));
     const bool has_next = k + nb < nseg;
-    TermRec R;  // the fields seg_len and the register path read
+    TermRec R;  // the fields seg_len, the LRC fit and the per-block path read
     R.woff = D.woff;
     R.n = n;
     R.segnum = D.segnum;
     R.nseg = nseg;
+    R.slope = D.slope;
+    R.icpt = D.icpt;
     const SegRec gr = ix.rec[D.segnum + k + min(ln, has_next ? nb : nb - 1u)];
     const uint32_t w_first = uni(__builtin_amdgcn_readlane(gr.off, 0));
     const uint32_t w_end = has_next ? uni(__builtin_amdgcn_readlane(gr.off, (int)nb)) : uni(D.wend);
@@ -2234,8 +2238,46 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
     // registers first; the run's LDS copy is then dead and its words become
     // the unit's output staging.  Units with more exceptions in a block take
     // the per-block path below.
+    // LRC / SegLRC: every block's fields from the LDS copy to registers, then
+    // each value formed and stored straight to HBM (lane l holds positions l,
+    // l + 64, ...: coalesced rows)
+    if constexpr (!gap_codec(CODEC)) {
+        if (lds_run) {
+            uint32_t f[U][V];
+            SegRec rec[U];
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g) {
+                const int gs = (int)min(g, nb - 1u);  // past nb: a copy of the last block, not used
+                rec[g].off = uni(__builtin_amdgcn_readlane(gr.off, gs));
+                rec[g].desc = uni(__builtin_amdgcn_readlane(gr.desc, gs));
+                rec[g].slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), gs)));
+                rec[g].icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), gs)));
+                rec[g].base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, gs));
+                const uint32_t* data = buf + sh + (rec[g].off - w_first) + 1;
+                load_fields<CODEC, SEG, false>(data, desc_lb(rec[g].desc), SEG, f[g]);
+            }
+#pragma unroll
+            for (uint32_t g = 0; g < U; ++g) {
+                if (g < nb) {
+                    const uint32_t s = k + g, len = seg_len(R, s, SEG);
+                    const float icv = vgpr_copy(rec[g].icpt);
+#pragma unroll
+                    for (int t = 0; t < V; ++t) {
+                        const uint32_t j = ln + 64 * t;
+                        uint32_t x;
+                        if constexpr (CODEC == EXC_SEGLRC)
+                            x = f[g][t] + seg_pred(j, rec[g].slope, icv, rec[g].base);
+                        else
+                            x = f[g][t] + lrc_pred(s * SEG + j, R.slope, R.icpt);
+                        if (j < len) o[g * SEG + j] = x;
+                    }
+                }
+            }
+            return;
+        }
+    }
     const bool few_exc = !__ballot(ln < nb && desc_en(gr.desc) > 64u);
-    if constexpr (SEG * U <= kGapWords) {
+    if constexpr (gap_codec(CODEC) && SEG * U <= kGapWords) {
         if (lds_run && few_exc) {
             uint32_t f[U][V], xq[U], xh[U], lbv[U], lenv[U];
 #pragma unroll
@@ -2302,10 +2344,10 @@ __global__ __launch_bounds__(64 * kGapWaves) __attribute__((amdgpu_waves_per_eu(
         rec.off = uni(__builtin_amdgcn_readlane(gr.off, (int)g));
         rec.desc = uni(__builtin_amdgcn_readlane(gr.desc, (int)g));
         rec.head = uni(__builtin_amdgcn_readlane(gr.head, (int)g));
-        rec.slope = 1.0f;
-        rec.icpt = 0.0f;
+        rec.slope = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.slope), (int)g)));
+        rec.icpt = __uint_as_float(uni(__builtin_amdgcn_readlane(__float_as_uint(gr.icpt), (int)g)));
         rec.rl = rec.rr = 0;
-        rec.base = 0;
+        rec.base = (int32_t)uni(__builtin_amdgcn_readlane((uint32_t)gr.base, (int)g));
         const uint32_t s = k + g, len = seg_len(R, s, SEG);
         // the block from HBM, staged in the buffer
         const uint32_t* data = ix.words + R.woff + rec.off + 1;
@@ -2394,12 +2436,12 @@ hipError_t probe_for_seg(const ProbeArgs& a, uint32_t grid, hipStream_t s)
 template <int CODEC>
 hipError_t decomp_for_seg(const DevIndex& ix, const DecUnit* units, uint32_t n, uint32_t* out, hipStream_t s)
 {
-    if constexpr (CODEC == EXC_PARAPFD) {
+    if constexpr (CODEC == EXC_PARAPFD || CODEC == EXC_SEGLRC || CODEC == EXC_LRC) {
         const uint32_t grid = (n + kGapWaves - 1) / kGapWaves;
         if (ix.seg == 256)
-            hipLaunchKernelGGL((k_decode_gap<CODEC, 256>), dim3(grid), dim3(64 * kGapWaves), 0, s, ix, units, n, out);
+            hipLaunchKernelGGL((k_decode_run<CODEC, 256>), dim3(grid), dim3(64 * kGapWaves), 0, s, ix, units, n, out);
         else
-            hipLaunchKernelGGL((k_decode_gap<CODEC, 128>), dim3(grid), dim3(64 * kGapWaves), 0, s, ix, units, n, out);
+            hipLaunchKernelGGL((k_decode_run<CODEC, 128>), dim3(grid), dim3(64 * kGapWaves), 0, s, ix, units, n, out);
         return hipGetLastError();
     }
     const uint32_t grid = (n + kWaves - 1) / kWaves;
