@@ -77,7 +77,7 @@ constexpr uint32_t s16_runs(uint32_t c0, uint32_t w0, uint32_t c1 = 0, uint32_t 
     return c0 | w0 << 5 | c1 << 10 | w1 << 15 | c2 << 20 | w2 << 25;
 }
 
-EXC_HD uint32_t s16_layout(uint32_t k)
+EXC_HD constexpr uint32_t s16_layout(uint32_t k)
 {
     switch (k & 15u) {
     case 0: return s16_runs(28, 1);
@@ -99,18 +99,14 @@ EXC_HD uint32_t s16_layout(uint32_t k)
     }
 }
 
-// number of fields of a Simple16 word (all are decoded, used or not)
-EXC_HD uint32_t s16_count(uint32_t word)
-{
-    const uint32_t L = s16_layout(word >> 28);
-    return (L & 31u) + ((L >> 10) & 31u) + ((L >> 20) & 31u);
-}
+// number of fields of a Simple16 word of layout L (all are decoded, used or not)
+EXC_HD uint32_t s16_count_l(uint32_t L) { return (L & 31u) + ((L >> 10) & 31u) + ((L >> 20) & 31u); }
+EXC_HD uint32_t s16_count(uint32_t word) { return s16_count_l(s16_layout(word >> 28)); }
 
-// field k of a Simple16 word, k < s16_count(word) (larger k: some value,
-// never a shift of 32 or more)
-EXC_HD uint32_t s16_field(uint32_t word, uint32_t k)
+// field k of a Simple16 word of layout L, k < its count (larger k: some
+// value, never a shift of 32 or more)
+EXC_HD uint32_t s16_field_l(uint32_t word, uint32_t k, uint32_t L)
 {
-    const uint32_t L = s16_layout(word >> 28);
     const uint32_t c0 = L & 31u, w0 = (L >> 5) & 31u, c1 = (L >> 10) & 31u, w1 = (L >> 15) & 31u;
     const uint32_t w2 = (L >> 25) & 31u;
     uint32_t sh, w;
@@ -126,6 +122,7 @@ EXC_HD uint32_t s16_field(uint32_t word, uint32_t k)
     }
     return (word >> (sh < 31u ? sh : 31u)) & ((1u << w) - 1u);
 }
+EXC_HD uint32_t s16_field(uint32_t word, uint32_t k) { return s16_field_l(word, k, s16_layout(word >> 28)); }
 
 // the fields of one Simple16 word into out[0 .. min(count, lim))
 EXC_HD void s16_unpack(uint32_t word, uint32_t* out, uint32_t lim)

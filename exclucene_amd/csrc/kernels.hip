@@ -249,6 +249,12 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
     }
 }
 
+// The 16 Simple16 layouts (s16_layout), for the lane-shuffle lookup
+__constant__ uint32_t kS16Layouts[16] = {s16_layout(0),  s16_layout(1),  s16_layout(2),  s16_layout(3),
+                                         s16_layout(4),  s16_layout(5),  s16_layout(6),  s16_layout(7),
+                                         s16_layout(8),  s16_layout(9),  s16_layout(10), s16_layout(11),
+                                         s16_layout(12), s16_layout(13), s16_layout(14), s16_layout(15)};
+
 // The exceptions of a NewPFor block patched into dst[0..len) (low parts
 // already there).  The reference decodes the Simple16 stream and patches in
 // thread 0 alone (GC/NewPFD/NewPFD_Decompression/NewPFD_Decompression_kernel.cu:420-447);
@@ -260,13 +266,17 @@ __device__ __forceinline__ void load_fields(const uint32_t* __restrict__ data, u
 // Stream words are read 64 at a time: past the stream's end a round reads at
 // most 63 words, inside the list's successors or the array's zero padding.
 // w_first: this lane's word of the first round, loaded with the block's fields.
+// s16t: this lane's entry of the Simple16 layout table (selector lane % 16,
+// kS16Layouts); a word's layout is then one lane shuffle, where a per-lane
+// switch on the selector compiles to a divergent decision tree of some 60
+// branch-masked instructions.
 __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint32_t en, uint32_t lb, uint32_t len,
-                                           uint32_t w_first, uint32_t* xs, uint32_t* dst)
+                                           uint32_t w_first, uint32_t* xs, uint32_t* dst, uint32_t s16t)
 {
     const uint32_t ln = lane_id(), need = 2 * en;
     for (uint32_t w0 = 0, got = 0; got < need; w0 += 64) {
         const uint32_t word = w0 ? sp[w0 + ln] : w_first;
-        const uint32_t c = s16_count(word);
+        const uint32_t c = s16_count_l((uint32_t)__shfl((int)s16t, (int)(word >> 28)));
         const uint32_t incl = wave_incl_scan(c);
         const uint32_t ex = incl - c;  // the round's first field in this lane's word
         const uint32_t tot = uni(__builtin_amdgcn_readlane(incl, 63));
@@ -282,7 +292,7 @@ __device__ __forceinline__ void npfd_patch(const uint32_t* __restrict__ sp, uint
                 if ((uint32_t)__shfl((int)ex, (int)(o + step)) <= f) o += step;
             const uint32_t wo = (uint32_t)__shfl((int)word, (int)o);
             const uint32_t k = f - (uint32_t)__shfl((int)ex, (int)o);
-            const uint32_t v = s16_field(wo, k);
+            const uint32_t v = s16_field_l(wo, k, (uint32_t)__shfl((int)s16t, (int)(wo >> 28)));
             if (f < nf) xs[got + f] = v;
         }
         got += tot;
@@ -349,7 +359,8 @@ __device__ __forceinline__ void pfd_patch_lds(const uint32_t* __restrict__ data,
 template <int CODEC, int SEG, bool PREFETCH = false>
 __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const SegRec& rec, uint32_t len,
                                              const uint32_t* __restrict__ data, uint32_t (&f)[SEG / 64],
-                                             uint32_t* dst, uint32_t* xs, ExcPrefetch xp = ExcPrefetch())
+                                             uint32_t* dst, uint32_t* xs, ExcPrefetch xp = ExcPrefetch(),
+                                             uint32_t s16t = 0)
 {
     constexpr int V = SEG / 64;
     const uint32_t ln = lane_id();
@@ -385,7 +396,7 @@ __device__ __forceinline__ void finish_block(const TermRec& R, uint32_t s, const
             wave_sync();
             if constexpr (CODEC == EXC_NEWPFD) {
                 const uint32_t* sp = data + words_for(len, lb);
-                npfd_patch(sp, en, lb, len, PREFETCH ? xp.a : sp[ln], xs, dst);
+                npfd_patch(sp, en, lb, len, PREFETCH ? xp.a : sp[ln], xs, dst, s16t);
             } else {
                 pfd_patch_lds<PREFETCH>(data, rec.desc, len, xp, dst);
             }
@@ -434,6 +445,8 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
         sl_all = sid((int)min(min(lane_id(), (uint32_t)(NB - 1)), r - 1));
         gr_all = ix.rec[R.segnum + sl_all];
     }
+    // NewPFD: this lane's Simple16 layout-table entry, once per call
+    const uint32_t s16t = CODEC == EXC_NEWPFD ? kS16Layouts[lane_id() & 15u] : 0u;
     for (uint32_t g0 = 0; g0 < r; g0 += G) {
         SegRec rec[G];
         uint32_t len[G], sg[G];
@@ -477,7 +490,7 @@ __device__ void decode_blocks(const DevIndex& ix, const TermRec& R, uint32_t r, 
         for (int g = 0; g < G; ++g)
             if (g0 + g < r)
                 finish_block<CODEC, SEG, PREFETCH>(R, sg[g], rec[g], len[g], data[g], f[g], dst + (g0 + g) * SEG,
-                                                   dst + NB * SEG, xp[g]);
+                                                   dst + NB * SEG, xp[g], s16t);
     }
 }
 
