@@ -346,9 +346,19 @@ __device__ __forceinline__ void pfd_patch_lds(const uint32_t* __restrict__ data,
     const uint32_t lb = desc_lb(desc), en = desc_en(desc), eb = desc_eb(desc), hb = desc_hb(desc);
     const uint32_t* pp = data + words_for(len, lb);
     const uint32_t* hp = pp + words_for(en, eb);
-    for (uint32_t e = lane_id(); e < en; e += 64) {
-        const uint32_t p = (PREFETCH && e < 64) ? xp.a : bits_at(pp, eb, e);
-        const uint32_t h = (PREFETCH && e < 64) ? xp.b : bits_at(hp, hb, e);
+    const uint32_t ln = lane_id();
+    uint32_t e0 = 0;
+    if constexpr (PREFETCH) {
+        // exceptions 0..63 came with the fields: one round with no lane
+        // branching, lanes without an exception ORing 0 into their own word
+        const bool act = ln < en && xp.a < len;
+        __hip_atomic_fetch_or(dst + (act ? xp.a : ln), act ? xp.b << lb : 0u, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WAVEFRONT);
+        e0 = 64;
+        if (en <= 64) return;
+    }
+    for (uint32_t e = e0 + ln; e < en; e += 64) {
+        const uint32_t p = bits_at(pp, eb, e), h = bits_at(hp, hb, e);
         // an LDS atomic OR (no return): no read round trip before the write
         if (p < len) __hip_atomic_fetch_or(dst + p, h << lb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
