@@ -339,6 +339,7 @@ __device__ __forceinline__ ExcPrefetch prefetch_exceptions(const uint32_t* __res
 // high_e << lb; positions are distinct, so no two lanes write one word).
 // PREFETCH: exception e < 64's position and high part arrived in xp with the
 // fields.  Positions at or past len (none in a valid block) are ignored.
+// Every caller's dst holds at least 64 words (a whole block's stage).
 template <bool PREFETCH>
 __device__ __forceinline__ void pfd_patch_lds(const uint32_t* __restrict__ data, uint32_t desc, uint32_t len,
                                               ExcPrefetch xp, uint32_t* dst)
@@ -347,17 +348,20 @@ __device__ __forceinline__ void pfd_patch_lds(const uint32_t* __restrict__ data,
     const uint32_t* pp = data + words_for(len, lb);
     const uint32_t* hp = pp + words_for(en, eb);
     const uint32_t ln = lane_id();
-    uint32_t e0 = 0;
-    if constexpr (PREFETCH) {
-        // exceptions 0..63 came with the fields: one round with no lane
-        // branching, lanes without an exception ORing 0 into their own word
-        const bool act = ln < en && xp.a < len;
-        __hip_atomic_fetch_or(dst + (act ? xp.a : ln), act ? xp.b << lb : 0u, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_WAVEFRONT);
-        e0 = 64;
-        if (en <= 64) return;
+    if (!en) return;
+    // exceptions 0..63 (prefetched with the fields, or read here): one round
+    // with no lane branching, lanes without an exception ORing 0 into their
+    // own word
+    if constexpr (!PREFETCH) {
+        const uint32_t e = min(ln, en - 1u);
+        xp.a = bits_at(pp, eb, e);
+        xp.b = bits_at(hp, hb, e);
     }
-    for (uint32_t e = e0 + ln; e < en; e += 64) {
+    const bool act = ln < en && xp.a < len;
+    __hip_atomic_fetch_or(dst + (act ? xp.a : ln), act ? xp.b << lb : 0u, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (en <= 64) return;
+    for (uint32_t e = 64 + ln; e < en; e += 64) {
         const uint32_t p = bits_at(pp, eb, e), h = bits_at(hp, hb, e);
         // an LDS atomic OR (no return): no read round trip before the write
         if (p < len) __hip_atomic_fetch_or(dst + p, h << lb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -1262,7 +1266,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                         const uint32_t* hp = pp + words_for(en, eb);
                         for (uint32_t e = k; e < en; e += 16) {
                             const uint32_t q = bits_at(pp, eb, e) - p0;
-                            if (q < pn) st[q] |= bits_at(hp, hb, e) << lb;
+                            if (q < pn)
+                                __hip_atomic_fetch_or(st + q, bits_at(hp, hb, e) << lb, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
                         }
                         wave_sync();
 #pragma unroll
