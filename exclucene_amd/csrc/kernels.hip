@@ -704,16 +704,25 @@ __device__ bool win_search(uint32_t* buf, const uint32_t* data, uint32_t len, co
     if (!act || len == 0) return false;
     if (!ok) return seg_find(data, len, r, v, search != SEARCH_BINARY);
     const uint32_t vlo = win_value(buf + ln, lb, w0, (uint32_t)wl, r), vhi = win_value(buf + ln, lb, w0, (uint32_t)wh, r);
-    if (v < vlo) return wl > 0 && seg_bsearch(data, lb, r, 0, wl - 1, v);
-    if (v > vhi) return wh < last && seg_bsearch(data, lb, r, wh + 1, last, v);
-    int a0 = wl, a1 = wh;
-    while (a0 <= a1) {
-        const int m = (a0 + a1) >> 1;
-        const uint32_t q = win_value(buf + ln, lb, w0, (uint32_t)m, r);
-        if (q == v) return true;
-        if (q < v) a0 = m + 1; else a1 = m - 1;
+    // v inside the window's values: the last position with a value <= v, in
+    // straight-line steps of halving powers of two (the wave's widest window
+    // sets the first), so that no lane branches out of a search loop
+    const bool inwin = v >= vlo && v <= vhi;
+    const uint32_t n = inwin ? (uint32_t)(wh - wl) : 0u;
+    uint32_t step = 128;  // windows span at most 2 x 64 + 3 positions
+    while (step > 1 && !__ballot(n >= step)) step >>= 1;
+    int pos = wl;
+    uint32_t vp = vlo;
+    for (; step; step >>= 1) {
+        const int m = pos + (int)step;
+        const uint32_t q = win_value(buf + ln, lb, w0, (uint32_t)min(m, wh), r);
+        const bool adv = m <= wh && q <= v;
+        pos = adv ? m : pos;
+        vp = adv ? q : vp;
     }
-    return false;
+    if (inwin) return vp == v;
+    if (v < vlo) return wl > 0 && seg_bsearch(data, lb, r, 0, wl - 1, v);
+    return wh < last && seg_bsearch(data, lb, r, wh + 1, last, v);
 }
 
 // Branchless search of v in the sorted LDS run buf[0..n) (n uniform).
