@@ -10,20 +10,25 @@
 //
 // Execution model (CDNA4, wave64):
 //  * one wave owns one work unit = (query, 512 consecutive docIDs of its
-//    shortest list, i.e. 4 segments of 128 or 2 of 256);
-//    a 256-thread workgroup hosts 4 independent waves, each with a private LDS
-//    slice, synchronised with wave barriers only.  Persistent launches claim
-//    chunks of units for the workgroup from one device counter; its waves
-//    take them one at a time through a shared LDS word.
+//    shortest list, i.e. 4 segments of 128 or 2 of 256); the probe launches
+//    one 64-thread workgroup per unit (units of the highest length-ratio class
+//    first), each wave with its own LDS slice, synchronised with wave barriers
+//    only.  Persistent launches claim chunks of units for the workgroup from
+//    one device counter and take them one at a time through an LDS word.
 //  * the unit's docIDs are decoded cooperatively into LDS and stay there as the
 //    candidate set; each longer list (ascending length) filters it.
-//  * per list, one gather of <= 64 segment heads around the window that the
-//    list-level fit predicts (checked against the heads, exact fallback to a
-//    64-ary search) gives every candidate its segment by register shuffles;
-//  * then either the wave decodes the contiguous run of needed segments into
-//    LDS with coalesced loads (low length ratios) and candidates search LDS
-//    branch-free, or each lane fetches its segment record, extracts the value
-//    at the position the segment fit predicts and walks to v (high ratios);
+//  * per list, the heads of the blocks that can hold a candidate: for the
+//    second list the prologue kernel's bounds, otherwise one gather of <= 64
+//    heads around the window the list-level fit predicts (checked against the
+//    heads, exact fallback to a 64-ary search);
+//  * then either the wave decodes that run of blocks straight from registers
+//    into a bitmap of a docID window in LDS and every candidate tests its bit
+//    (low length ratios), or each row of 64 candidates finds its blocks among
+//    the heads, stages the words of the positions each block's fit allows in
+//    LDS and binary-searches them there (high ratios); ParaPFD, which has no
+//    random access, decodes the run into the bitmap the same way, or a
+//    candidate's half block by one row of 16 lanes (the block's median picks
+//    the half);
 //  * survivors are compacted with ballot + mbcnt (order preserving), so the
 //    next list only probes live candidates and no 0/1 flag array exists;
 //  * each unit writes its survivors into the stage slot of its shortest-list
@@ -43,7 +48,7 @@ constexpr uint32_t kClaim = 8;        // units per workgroup claim on the device
 // ParaPFD probes take the sparse path (one candidate's half block per row of
 // 16 lanes) when the candidates average at most this many per distinct block
 constexpr uint32_t kPfdSparse = 4;
-constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (LRC/SegLRC: a bitmap of 24,576 docIDs)
+constexpr uint32_t kBufWords = 768;  // per-wave LDS decode buffer (holds the LRC/SegLRC bitmap of 16,384 docIDs)
 constexpr uint32_t kMaxWindows = 8;   // bitmap windows per cooperative ParaPFD probe
 constexpr uint32_t kMaxWindowsSeg = 12;  // bitmap windows per cooperative LRC/SegLRC probe (16,384 docIDs each)
 constexpr uint32_t kBitmapPiece = 768; // docIDs decoded per load round of the bitmap path
@@ -1847,12 +1852,12 @@ __device__ uint32_t claim_unit(const ProbeArgs& A, ClaimLds* q)
     return uni(__shfl(u, 0));
 }
 
-// Eight waves per SIMD for the random-access codecs (64 VGPRs, 20 KB of LDS
-// per workgroup): the probe is bound by load latency, so resident waves pay
-// as long as the spilled registers stay few (C2 batch: 7 waves 0.88 ms,
-// 8 waves 0.85 ms with 32 spilled bytes per lane).  The persistent kernel's
-// 16-byte claim record takes it to 7 workgroups per CU: eight would need
-// 8 x 20,496 bytes of the 160 KiB.
+// Eight waves per SIMD (64 VGPRs; one 64-thread workgroup with 5,120 bytes
+// of LDS per unit): resident waves hide the load rounds as long as the
+// spilled registers stay few (C2 batch, an earlier build: 7 waves 0.88 ms,
+// 8 waves 0.85 ms with 32 spilled bytes per lane; ParaPFD now spills 72
+// bytes and still measured faster at 8 than at 7 or 6).  The persistent
+// kernel is compiled for 7, its 72 VGPRs holding the claim loop's state.
 template <int CODEC, int SEG, bool PERSIST>
 __global__ __launch_bounds__(64 * kProbeWaves) __attribute__((amdgpu_waves_per_eu(PERSIST ? 7 : 8, PERSIST ? 7 : 8))) void k_probe(ProbeArgs A)
 {
