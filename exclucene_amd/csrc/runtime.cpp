@@ -2908,7 +2908,8 @@ int slot_copy_results(exc_server* sv, exc_server::Slot& sl, bool& started)
             }
     }
     if (!b.p) {
-        b.cap = std::max<size_t>(total + total / 2, 1u << 20);
+        // twice the batch (at least 16 Mi results): a pinned allocation stalls the loop
+        b.cap = std::max<size_t>(2 * total, 1u << 24);
         HIP_TRY(hipHostMalloc((void**)&b.p, b.cap * 4, hipHostMallocDefault));
     }
     std::weak_ptr<exc_server::Pool> wp = sv->pool;
@@ -3099,8 +3100,13 @@ extern "C" int exc_server_wait(exc_server* sv, uint64_t ticket, uint32_t* result
     // a too-small buffer keeps the request so the caller can retry with *n
     if (!(sv->flags & EXC_SERVER_COUNTS) && r.count > cap)
         return set_error(EXC_ERR_CAPACITY, "result buffer too small");
-    if (results && r.buf && r.count) std::memcpy(results, r.buf->p + r.off, r.count * 4);
+    // the copy runs outside the lock (the server loop and the submitters take
+    // it), on a reference that keeps the batch's pinned buffer alive
+    const std::shared_ptr<exc_server::Pinned> buf = r.buf;
+    const uint64_t off = r.off, count = r.count;
     sv->reqs.erase(ticket);
+    lk.unlock();
+    if (results && buf && count) std::memcpy(results, buf->p + off, count * 4);
     return EXC_OK;
 }
 
@@ -3132,18 +3138,24 @@ extern "C" int exc_server_replay(exc_server* sv, const uint32_t* qterms, const u
             }
             if (lat_ms) lat_ms[q] = r.latency_ms;
             if (counts) counts[q] = (uint32_t)r.count;
-            if (results && r.buf && r.count) {
-                if (pos + r.count > res_cap) {
+            // the results are copied after the lock is released (the server
+            // loop and the submitter take it); the reference keeps the batch's
+            // pinned buffer alive
+            const std::shared_ptr<exc_server::Pinned> buf = r.buf;
+            const uint64_t off = r.off, count = r.count;
+            sv->reqs.erase(tickets[q]);
+            lk.unlock();
+            if (results && buf && count) {
+                if (pos + count > res_cap) {
                     if (!crc) {
                         crc = EXC_ERR_CAPACITY;
                         cq = -1;
                     }
                 } else {
-                    std::memcpy(results + pos, r.buf->p + r.off, r.count * 4);
+                    std::memcpy(results + pos, buf->p + off, count * 4);
                 }
             }
-            pos += r.buf ? r.count : 0;
-            sv->reqs.erase(tickets[q]);
+            pos += buf ? count : 0;
         }
     });
     const auto t0 = std::chrono::steady_clock::now();
