@@ -65,6 +65,11 @@ EXC_HD uint32_t desc_eb(uint32_t d) { return (d >> 21) & 31u; }
 EXC_HD uint32_t desc_hb(uint32_t d) { return (d >> 16) & 31u; }
 EXC_HD uint32_t desc_en(uint32_t d) { return d & 65535u; }
 EXC_HD uint32_t words_for(uint32_t n, uint32_t b) { return (uint32_t)(((uint64_t)n * b + 31) >> 5); }
+// Quarter boundaries of a ParaPFD block of len values: m1, m2 = len / 2 (the
+// .segMedian position) and m3; the device records carry the values at m1 and
+// m3 besides the median, so a probe decodes at most a quarter of a block.
+EXC_HD uint32_t pfd_m1(uint32_t len) { return len >> 2; }
+EXC_HD uint32_t pfd_m3(uint32_t len) { return (len >> 1) + ((len - (len >> 1)) >> 1); }
 EXC_HD uint32_t bits_of(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
 
 // Simple16 (Zhang, Long and Suel, WWW 2008; the tables of GI/Simple16.h:13-48):

@@ -89,10 +89,22 @@ struct DevIndex {
     const float* mparams;
     int32_t codec;
     uint32_t seg;
-    uint32_t flags;  // kIxMedians
+    uint32_t flags;  // kIxMedians | kIxQuarters
 };
 // DevIndex::flags: ParaPFD records carry the block medians (.segMedian) in rl
 constexpr uint32_t kIxMedians = 1;
+// ... and the values at the block's quarter positions pfd_m1 (in base) and
+// pfd_m3 (in rr), derived on the device when the index is built or uploaded
+constexpr uint32_t kIxQuarters = 2;
+
+// One ParaPFD block whose quarter values k_pfd_anchors takes from a decoded
+// run: its record, its first value's position in the run and its length.
+struct AnchorJob {
+    uint64_t rec;
+    uint64_t pos;
+    uint32_t len;
+    uint32_t pad;
+};
 
 struct QueryRec {
     uint32_t toff;        // first sorted term of the query in qterms
@@ -265,6 +277,7 @@ hipError_t launch_hs_fit(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_hs_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_heads_line(const EncodeArgs& a, hipStream_t s);
+hipError_t launch_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec, hipStream_t s);
 // HS_SegLRC: the per-bucket records and tables of the encoder compacted to the
 // device layout (records of non-empty buckets, occupancy tables; kDenseBuckets)
 struct HsCompactArgs {

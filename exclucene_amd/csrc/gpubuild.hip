@@ -325,8 +325,9 @@ __global__ __launch_bounds__(256) void k_blk_fit(EncodeArgs a)
     q.slope = 1.0f;
     q.icpt = 0.0f;
     q.rl = a.codec == EXC_PARAPFD ? val.L[val.st + n / 2] : 0u;
-    q.rr = 0;
-    q.base = 0;
+    // ParaPFD: the values at the other two quarter positions
+    q.rr = a.codec == EXC_PARAPFD ? val.L[val.st + pfd_m3(n)] : 0u;
+    q.base = a.codec == EXC_PARAPFD ? (int32_t)val.L[val.st + pfd_m1(n)] : 0;
     q.head = val.L[val.st];
     a.rec[g] = q;
     a.head[g] = q.head;
@@ -496,6 +497,17 @@ __global__ __launch_bounds__(256) void k_heads_line(EncodeArgs a)
     float icpt;
     heads_line(a.head + a.seg_num[t], a.seg_num[t + 1] - a.seg_num[t], a.seg, R.slope, icpt, R.range_l, R.range_r);
     R.icpt = __float_as_int(icpt);
+}
+
+// Thread per ParaPFD block of a decoded run: the values at its quarter
+// positions into its record (base: pfd_m1, rr: pfd_m3).
+__global__ __launch_bounds__(256) void k_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const AnchorJob j = jobs[i];
+    rec[j.rec].base = (int32_t)dec[j.pos + pfd_m1(j.len)];
+    rec[j.rec].rr = dec[j.pos + pfd_m3(j.len)];
 }
 
 // ---------------------------------------------------------------- HS_SegLRC compaction
@@ -711,6 +723,13 @@ hipError_t launch_heads_line(const EncodeArgs& a, hipStream_t s)
 {
     if (!a.nterms) return hipSuccess;
     hipLaunchKernelGGL(k_heads_line, dim3((a.nterms + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_pfd_anchors, dim3((n + 255) / 256), dim3(256), 0, s, jobs, n, dec, rec);
     return hipGetLastError();
 }
 

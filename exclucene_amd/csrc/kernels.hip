@@ -1235,50 +1235,57 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             }
             nd = uni(nd);
             bool sparse = false;
-            if constexpr (SEG == 128) sparse = (A.ix.flags & kIxMedians) && nlive <= kPfdSparse * nd;
+            if constexpr (SEG == 128) sparse = (A.ix.flags & kIxQuarters) && nlive <= kPfdSparse * nd;
             if (sparse) {
                 if constexpr (SEG == 128) {
                 // ---- sparse: few candidates per block.  Each candidate's
-                // half block, the one its value falls in by the block's median
-                // (.segMedian, the reference's half-select,
-                // GC/ParaPFD/ParaPFD_Intersection_FRAC/ParaPFD_Intersection_kernel.cu:322-455),
-                // is decoded by one row of 16 lanes, 4 positions a lane: fields,
-                // exceptions ORed into the row's LDS stage, prefix from the
-                // block head or the median, compare.  Four candidates a pass;
-                // their segments wait in the buffer's first kChunk words.
-                constexpr uint32_t HP = SEG / 2, PL = HP / 16;
+                // quarter block, the one its value falls in by the block's
+                // quarter values (the head, the value at pfd_m1, the median of
+                // .segMedian -- the reference's half-select,
+                // GC/ParaPFD/ParaPFD_Intersection_FRAC/ParaPFD_Intersection_kernel.cu:322-455
+                // -- and the value at pfd_m3), is decoded by one row of 8
+                // lanes, 4 positions a lane: fields, exceptions ORed into the
+                // row's LDS stage, prefix from the quarter's first value,
+                // compare.  Eight candidates a pass; their segments wait in
+                // the buffer's first kChunk words.
+                constexpr uint32_t QP = 32, PL = QP / 8;
                 uint32_t* segs = W.buf;
                 uint32_t* stage = W.buf + kChunk;
-                static_assert(kChunk + 4 * HP <= kBufWords, "segments and four stages fit the buffer");
+                static_assert(kChunk + 8 * QP <= kBufWords, "segments and eight stages fit the buffer");
 #pragma unroll
                 for (int i = 0; i < V; ++i) segs[ln + 64 * i] = (uint32_t)s[i];
                 wave_sync();
-                const uint32_t r = ln >> 4, k = ln & 15u;
+                const uint32_t r = ln >> 3, k = ln & 7u;
                 uint32_t hitm = 0;  // bit i: this lane's row-i candidate was found
-                for (uint32_t c0 = 0; c0 < nlive; c0 += 4) {
+                for (uint32_t c0 = 0; c0 < nlive; c0 += 8) {
                     const uint32_t c = c0 + r;
                     const int32_t sc = c < nlive ? (int32_t)segs[c] : -1;
                     const bool ok = sc >= 0;
                     const uint32_t vc = ok ? cand[c] : 0u;
                     const uint32_t si = ok ? (uint32_t)sc : sf;
                     const SegRec rc = A.ix.rec[R.segnum + si];
-                    const uint32_t len = seg_len(R, si, SEG), mid = len >> 1;
-                    const bool upper = vc >= rc.rl;
-                    const uint32_t p0 = upper ? mid : 0u, pn = ok ? (upper ? len - mid : mid) : 0u;
+                    const uint32_t len = seg_len(R, si, SEG);
+                    const uint32_t m1 = pfd_m1(len), m2 = len >> 1, m3 = pfd_m3(len);
+                    const bool up = vc >= rc.rl;
+                    const bool hi = up ? vc >= rc.rr : vc >= (uint32_t)rc.base;
+                    const uint32_t p0 = up ? (hi ? m3 : m2) : (hi ? m1 : 0u);
+                    const uint32_t pe = up ? (hi ? len : m3) : (hi ? m2 : m1);
+                    const uint32_t pn = ok ? pe - p0 : 0u;
+                    const uint32_t first = up ? (hi ? rc.rr : rc.rl) : (hi ? (uint32_t)rc.base : rc.head);
                     const uint32_t* data = A.ix.words + R.woff + rc.off + 1;
                     const uint32_t lb = desc_lb(rc.desc), en = ok ? desc_en(rc.desc) : 0u;
                     uint32_t f[PL];
 #pragma unroll
                     for (uint32_t t = 0; t < PL; ++t) f[t] = bits_at(data, lb, p0 + k * PL + t);
                     if (__ballot(en != 0)) {
-                        uint32_t* st = stage + r * HP;
+                        uint32_t* st = stage + r * QP;
 #pragma unroll
                         for (uint32_t t = 0; t < PL; ++t) st[k * PL + t] = f[t];
                         wave_sync();
                         const uint32_t eb = desc_eb(rc.desc), hb = desc_hb(rc.desc);
                         const uint32_t* pp = data + words_for(len, lb);
                         const uint32_t* hp = pp + words_for(en, eb);
-                        for (uint32_t e = k; e < en; e += 16) {
+                        for (uint32_t e = k; e < en; e += 8) {
                             const uint32_t q = bits_at(pp, eb, e) - p0;
                             if (q < pn)
                                 __hip_atomic_fetch_or(st + q, bits_at(hp, hb, e) << lb, __ATOMIC_RELAXED,
@@ -1289,26 +1296,29 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                         for (uint32_t t = 0; t < PL; ++t) f[t] = st[k * PL + t];
                         wave_sync();
                     }
-                    // values: the head or the median, then the half's gaps
-                    // (its first gap stands for that value)
+                    // values: the quarter's first value, then its gaps (its
+                    // first gap stands for that value); the lanes' sums
+                    // scanned within each row of 8 by DPP row shifts
                     if (k == 0) f[0] = 0;
 #pragma unroll
                     for (uint32_t t = 1; t < PL; ++t) f[t] += f[t - 1];
-                    uint32_t x = f[PL - 1];
-                    x = dpp_add<0x111, 0xf>(x);
-                    x = dpp_add<0x112, 0xf>(x);
-                    x = dpp_add<0x114, 0xf>(x);
-                    x = dpp_add<0x118, 0xf>(x);
-                    const uint32_t off = x - f[PL - 1] + (upper ? rc.rl : rc.head);
+                    uint32_t x = f[PL - 1], y;
+                    y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);
+                    x += k >= 1 ? y : 0u;
+                    y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
+                    x += k >= 2 ? y : 0u;
+                    y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
+                    x += k >= 4 ? y : 0u;
+                    const uint32_t off = x - f[PL - 1] + first;
                     bool hit = false;
 #pragma unroll
                     for (uint32_t t = 0; t < PL; ++t) hit = hit || (k * PL + t < pn && f[t] + off == vc);
                     const uint64_t m = __ballot(hit);
-                    uint32_t hit4 = 0;
+                    uint32_t hit8 = 0;
 #pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q) hit4 |= ((m >> (16 * q)) & 0xffffull) ? 1u << q : 0u;
-                    // candidates c0 .. c0 + 3 belong to lanes c0 % 64 .. + 3 of row c0 / 64
-                    if ((ln >> 2) == ((c0 & 63u) >> 2)) hitm |= ((hit4 >> (ln & 3u)) & 1u) << (c0 >> 6);
+                    for (uint32_t q = 0; q < 8; ++q) hit8 |= ((m >> (8 * q)) & 0xffull) ? 1u << q : 0u;
+                    // candidates c0 .. c0 + 7 belong to lanes c0 % 64 .. + 7 of row c0 / 64
+                    if ((ln >> 3) == ((c0 & 63u) >> 3)) hitm |= ((hit8 >> (ln & 7u)) & 1u) << (c0 >> 6);
                 }
 #pragma unroll
                 for (int i = 0; i < V; ++i) found[i] = (hitm >> i) & 1u;

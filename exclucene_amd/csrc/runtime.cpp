@@ -463,6 +463,10 @@ extern "C" int exc_ctx_last_times(exc_ctx* c, float* probe_ms, float* invoke_ms)
 
 // ---------------------------------------------------------------- index
 
+namespace {
+int fill_pfd_anchors(exc_ctx* c, exc_index* ix);
+}
+
 extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index** out)
 {
     if (!c || !h || !out) return set_error(EXC_ERR_ARG, "exc_index_upload: null argument");
@@ -722,6 +726,12 @@ extern "C" int exc_index_upload(exc_ctx* c, const exc_index_host* h, exc_index**
         if (raw_hs) ix->dix.hash = (const uint32_t*)up(h->hash_info, h->hash_words * 4);
     }
     if (rc) {
+        exc_index_free(ix);
+        return rc;
+    }
+    // ParaPFD-128: each block's quarter values for the sparse probe (without
+    // room in the context's budget the probe keeps its other paths)
+    if ((rc = fill_pfd_anchors(c, ix)) && rc != EXC_ERR_BUDGET) {
         exc_index_free(ix);
         return rc;
     }
@@ -1507,6 +1517,54 @@ int decode_to_device(exc_ctx* c, exc_index* ix, const std::vector<uint32_t>& ter
     return EXC_OK;
 }
 
+// ParaPFD-128 indexes uploaded from host arrays or files: the values at each
+// block's quarter positions (kIxQuarters), taken from the lists decoded on the
+// device, at most 2^28 docIDs at a time.
+int fill_pfd_anchors(exc_ctx* c, exc_index* ix)
+{
+    if (ix->codec != EXC_PARAPFD || ix->seg != 128 || !(ix->dix.flags & kIxMedians)) return EXC_OK;
+    SegRec* rec = const_cast<SegRec*>(ix->dix.rec);
+    DevBuf dec, dj;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> terms;
+    std::vector<AnchorJob> jobs;
+    int rc = EXC_OK;
+    for (uint32_t t0 = 0; t0 < ix->nterms && !rc;) {
+        terms.clear();
+        uint64_t tot = 0;
+        uint32_t t = t0;
+        for (; t < ix->nterms && (terms.empty() || tot + ix->n[t] <= (1ull << 28)); ++t) {
+            terms.push_back(t);
+            tot += ix->n[t];
+        }
+        t0 = t;
+        if (!tot) continue;
+        if ((rc = decode_to_device(c, ix, terms, dec, off))) break;
+        jobs.clear();
+        for (size_t i = 0; i < terms.size(); ++i) {
+            const TermRec& R = ix->recs[terms[i]];
+            const uint32_t n = ix->n[terms[i]];
+            for (uint32_t s = 0; s < R.nseg; ++s)
+                jobs.push_back(AnchorJob{(uint64_t)R.segnum + s, off[i] + (uint64_t)s * 128u,
+                                         std::min<uint32_t>(128u, n - s * 128u), 0u});
+        }
+        if (jobs.size() > 0xffffffffull) {
+            rc = set_error(EXC_ERR_CAPACITY, "more than 2^32 blocks in one decoded run");
+            break;
+        }
+        if ((rc = dev_alloc(c, dj, jobs.size() * sizeof(AnchorJob)))) break;
+        hipError_t e = hipMemcpy(dj.p, jobs.data(), jobs.size() * sizeof(AnchorJob), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = launch_pfd_anchors(dptr<AnchorJob>(dj), (uint32_t)jobs.size(), dptr<uint32_t>(dec), rec, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = set_error(EXC_ERR_DEVICE, "ParaPFD quarter values: %s", hipGetErrorString(e));
+    }
+    dev_free(c, dec);
+    dev_free(c, dj);
+    if (!rc) ix->dix.flags |= kIxQuarters;
+    return rc;
+}
+
 }  // namespace
 
 static int check_terms(exc_index* ix, const uint32_t* terms, uint32_t nt, uint64_t* total)
@@ -1978,7 +2036,8 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
         ix->lwords[t] = (uint32_t)((t + 1 < nterms ? ix->recs[t + 1].woff : W) - ix->recs[t].woff);
     ix->dix.codec = codec;
     ix->dix.seg = seg;
-    ix->dix.flags = codec == EXC_PARAPFD ? kIxMedians : 0u;  // k_blk_fit keeps each block's median
+    // k_blk_fit keeps each block's median and its other quarter values
+    ix->dix.flags = codec == EXC_PARAPFD ? kIxMedians | kIxQuarters : 0u;
     ix->dix.words = a.words;
     ix->dix.terms = a.terms;
     ix->d_terms = a.terms;
