@@ -70,6 +70,16 @@ EXC_HD uint32_t words_for(uint32_t n, uint32_t b) { return (uint32_t)(((uint64_t
 // m3 besides the median, so a probe decodes at most a quarter of a block.
 EXC_HD uint32_t pfd_m1(uint32_t len) { return len >> 2; }
 EXC_HD uint32_t pfd_m3(uint32_t len) { return (len >> 1) + ((len - (len >> 1)) >> 1); }
+// Start of quarter q (0..4, 4 = len) and the midpoint of quarter q (0..3):
+// the eighth boundaries whose values a device index keeps per block.
+EXC_HD uint32_t pfd_qb(uint32_t len, uint32_t q)
+{
+    return q == 0 ? 0u : q == 1 ? pfd_m1(len) : q == 2 ? (len >> 1) : q == 3 ? pfd_m3(len) : len;
+}
+EXC_HD uint32_t pfd_oct_pos(uint32_t len, uint32_t q)
+{
+    return pfd_qb(len, q) + ((pfd_qb(len, q + 1) - pfd_qb(len, q)) >> 1);
+}
 EXC_HD uint32_t bits_of(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
 
 // Simple16 (Zhang, Long and Suel, WWW 2008; the tables of GI/Simple16.h:13-48):

@@ -89,13 +89,16 @@ struct DevIndex {
     const float* mparams;
     int32_t codec;
     uint32_t seg;
-    uint32_t flags;  // kIxMedians | kIxQuarters
+    uint32_t flags;  // kIxMedians | kIxQuarters | kIxOctants
+    const uint32_t* pfd_oct;  // ParaPFD (kIxOctants): per block the values at pfd_oct_pos(len, 0..3)
 };
 // DevIndex::flags: ParaPFD records carry the block medians (.segMedian) in rl
 constexpr uint32_t kIxMedians = 1;
 // ... and the values at the block's quarter positions pfd_m1 (in base) and
 // pfd_m3 (in rr), derived on the device when the index is built or uploaded
 constexpr uint32_t kIxQuarters = 2;
+// ... and the values at the midpoints of the quarters in pfd_oct
+constexpr uint32_t kIxOctants = 4;
 
 // One ParaPFD block whose quarter values k_pfd_anchors takes from a decoded
 // run: its record, its first value's position in the run and its length.
@@ -261,6 +264,7 @@ struct EncodeArgs {
     const uint64_t* hs_off;
     uint32_t* hash;
     uint32_t hs_target, hs_fixed;
+    uint32_t* pfd_oct;         // ParaPFD: 4 values a block at pfd_oct_pos (nullptr: none)
 };
 hipError_t launch_synth(const SynthArgs& a, bool write, hipStream_t s);
 hipError_t launch_seg_fit(const EncodeArgs& a, hipStream_t s);
@@ -277,7 +281,8 @@ hipError_t launch_hs_fit(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_hs_pack(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_term_recs(const EncodeArgs& a, hipStream_t s);
 hipError_t launch_heads_line(const EncodeArgs& a, hipStream_t s);
-hipError_t launch_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec, hipStream_t s);
+hipError_t launch_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec, uint32_t* oct,
+                              hipStream_t s);
 // HS_SegLRC: the per-bucket records and tables of the encoder compacted to the
 // device layout (records of non-empty buckets, occupancy tables; kDenseBuckets)
 struct HsCompactArgs {

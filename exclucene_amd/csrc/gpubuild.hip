@@ -328,6 +328,8 @@ __global__ __launch_bounds__(256) void k_blk_fit(EncodeArgs a)
     // ParaPFD: the values at the other two quarter positions
     q.rr = a.codec == EXC_PARAPFD ? val.L[val.st + pfd_m3(n)] : 0u;
     q.base = a.codec == EXC_PARAPFD ? (int32_t)val.L[val.st + pfd_m1(n)] : 0;
+    if (a.codec == EXC_PARAPFD && a.pfd_oct)
+        for (uint32_t k = 0; k < 4; ++k) a.pfd_oct[4ull * g + k] = val.L[val.st + pfd_oct_pos(n, k)];
     q.head = val.L[val.st];
     a.rec[g] = q;
     a.head[g] = q.head;
@@ -500,14 +502,17 @@ __global__ __launch_bounds__(256) void k_heads_line(EncodeArgs a)
 }
 
 // Thread per ParaPFD block of a decoded run: the values at its quarter
-// positions into its record (base: pfd_m1, rr: pfd_m3).
-__global__ __launch_bounds__(256) void k_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec)
+// positions into its record (base: pfd_m1, rr: pfd_m3) and those at the
+// quarters' midpoints into oct.
+__global__ __launch_bounds__(256) void k_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec,
+                                                     uint32_t* oct)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const AnchorJob j = jobs[i];
     rec[j.rec].base = (int32_t)dec[j.pos + pfd_m1(j.len)];
     rec[j.rec].rr = dec[j.pos + pfd_m3(j.len)];
+    for (uint32_t k = 0; k < 4; ++k) oct[4 * j.rec + k] = dec[j.pos + pfd_oct_pos(j.len, k)];
 }
 
 // ---------------------------------------------------------------- HS_SegLRC compaction
@@ -726,10 +731,11 @@ hipError_t launch_heads_line(const EncodeArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec, hipStream_t s)
+hipError_t launch_pfd_anchors(const AnchorJob* jobs, uint32_t n, const uint32_t* dec, SegRec* rec, uint32_t* oct,
+                              hipStream_t s)
 {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_pfd_anchors, dim3((n + 255) / 256), dim3(256), 0, s, jobs, n, dec, rec);
+    hipLaunchKernelGGL(k_pfd_anchors, dim3((n + 255) / 256), dim3(256), 0, s, jobs, n, dec, rec, oct);
     return hipGetLastError();
 }
 

@@ -1235,60 +1235,66 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
             }
             nd = uni(nd);
             bool sparse = false;
-            if constexpr (SEG == 128) sparse = (A.ix.flags & kIxQuarters) && nlive <= kPfdSparse * nd;
+            if constexpr (SEG == 128) sparse = (A.ix.flags & kIxOctants) && nlive <= kPfdSparse * nd;
             if (sparse) {
                 if constexpr (SEG == 128) {
                 // ---- sparse: few candidates per block.  Each candidate's
-                // quarter block, the one its value falls in by the block's
-                // quarter values (the head, the value at pfd_m1, the median of
-                // .segMedian -- the reference's half-select,
-                // GC/ParaPFD/ParaPFD_Intersection_FRAC/ParaPFD_Intersection_kernel.cu:322-455
-                // -- and the value at pfd_m3), is decoded by one row of 8
-                // lanes, 4 positions a lane: fields, exceptions ORed into the
-                // row's LDS stage, prefix from the quarter's first value,
-                // compare.  Eight candidates a pass; their segments wait in
-                // the buffer's first kChunk words.
-                constexpr uint32_t QP = 32, PL = QP / 8;
+                // eighth of a block, the one its value falls in by the block's
+                // values at the eighth boundaries (the head; the median of
+                // .segMedian, the reference's half-select,
+                // GC/ParaPFD/ParaPFD_Intersection_FRAC/ParaPFD_Intersection_kernel.cu:322-455;
+                // the values at the quarter positions in the record and at
+                // the quarters' midpoints in pfd_oct), is decoded by one row
+                // of 4 lanes, 4 positions a lane: fields, exceptions ORed
+                // into the row's LDS stage, prefix from the eighth's first
+                // value, compare.  Sixteen candidates a pass; their segments
+                // wait in the buffer's first kChunk words.
+                constexpr uint32_t EP = 16, PL = EP / 4;
                 uint32_t* segs = W.buf;
                 uint32_t* stage = W.buf + kChunk;
-                static_assert(kChunk + 8 * QP <= kBufWords, "segments and eight stages fit the buffer");
+                static_assert(kChunk + 16 * EP <= kBufWords, "segments and sixteen stages fit the buffer");
 #pragma unroll
                 for (int i = 0; i < V; ++i) segs[ln + 64 * i] = (uint32_t)s[i];
                 wave_sync();
-                const uint32_t r = ln >> 3, k = ln & 7u;
+                const uint32_t r = ln >> 2, k = ln & 3u;
                 uint32_t hitm = 0;  // bit i: this lane's row-i candidate was found
-                for (uint32_t c0 = 0; c0 < nlive; c0 += 8) {
+                for (uint32_t c0 = 0; c0 < nlive; c0 += 16) {
                     const uint32_t c = c0 + r;
                     const int32_t sc = c < nlive ? (int32_t)segs[c] : -1;
                     const bool ok = sc >= 0;
                     const uint32_t vc = ok ? cand[c] : 0u;
                     const uint32_t si = ok ? (uint32_t)sc : sf;
                     const SegRec rc = A.ix.rec[R.segnum + si];
+                    const uint4 oc = reinterpret_cast<const uint4*>(A.ix.pfd_oct)[R.segnum + si];
                     const uint32_t len = seg_len(R, si, SEG);
-                    const uint32_t m1 = pfd_m1(len), m2 = len >> 1, m3 = pfd_m3(len);
+                    // quarter, then its half
                     const bool up = vc >= rc.rl;
-                    const bool hi = up ? vc >= rc.rr : vc >= (uint32_t)rc.base;
-                    const uint32_t p0 = up ? (hi ? m3 : m2) : (hi ? m1 : 0u);
-                    const uint32_t pe = up ? (hi ? len : m3) : (hi ? m2 : m1);
+                    const bool q_hi = up ? vc >= rc.rr : vc >= (uint32_t)rc.base;
+                    const uint32_t q = (up ? 2u : 0u) + (q_hi ? 1u : 0u);
+                    const uint32_t qfirst = up ? (q_hi ? rc.rr : rc.rl) : (q_hi ? (uint32_t)rc.base : rc.head);
+                    const uint32_t mid = q == 0 ? oc.x : q == 1 ? oc.y : q == 2 ? oc.z : oc.w;
+                    const bool h_hi = vc >= mid;
+                    const uint32_t qs = pfd_qb(len, q), qe = pfd_qb(len, q + 1), qm = qs + ((qe - qs) >> 1);
+                    const uint32_t p0 = h_hi ? qm : qs, pe = h_hi ? qe : qm;
                     const uint32_t pn = ok ? pe - p0 : 0u;
-                    const uint32_t first = up ? (hi ? rc.rr : rc.rl) : (hi ? (uint32_t)rc.base : rc.head);
+                    const uint32_t first = h_hi ? mid : qfirst;
                     const uint32_t* data = A.ix.words + R.woff + rc.off + 1;
                     const uint32_t lb = desc_lb(rc.desc), en = ok ? desc_en(rc.desc) : 0u;
                     uint32_t f[PL];
 #pragma unroll
                     for (uint32_t t = 0; t < PL; ++t) f[t] = bits_at(data, lb, p0 + k * PL + t);
                     if (__ballot(en != 0)) {
-                        uint32_t* st = stage + r * QP;
+                        uint32_t* st = stage + r * EP;
 #pragma unroll
                         for (uint32_t t = 0; t < PL; ++t) st[k * PL + t] = f[t];
                         wave_sync();
                         const uint32_t eb = desc_eb(rc.desc), hb = desc_hb(rc.desc);
                         const uint32_t* pp = data + words_for(len, lb);
                         const uint32_t* hp = pp + words_for(en, eb);
-                        for (uint32_t e = k; e < en; e += 8) {
-                            const uint32_t q = bits_at(pp, eb, e) - p0;
-                            if (q < pn)
-                                __hip_atomic_fetch_or(st + q, bits_at(hp, hb, e) << lb, __ATOMIC_RELAXED,
+                        for (uint32_t e = k; e < en; e += 4) {
+                            const uint32_t qq = bits_at(pp, eb, e) - p0;
+                            if (qq < pn)
+                                __hip_atomic_fetch_or(st + qq, bits_at(hp, hb, e) << lb, __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
                         }
                         wave_sync();
@@ -1296,9 +1302,9 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                         for (uint32_t t = 0; t < PL; ++t) f[t] = st[k * PL + t];
                         wave_sync();
                     }
-                    // values: the quarter's first value, then its gaps (its
+                    // values: the eighth's first value, then its gaps (its
                     // first gap stands for that value); the lanes' sums
-                    // scanned within each row of 8 by DPP row shifts
+                    // scanned within each row of 4 by DPP row shifts
                     if (k == 0) f[0] = 0;
 #pragma unroll
                     for (uint32_t t = 1; t < PL; ++t) f[t] += f[t - 1];
@@ -1307,18 +1313,16 @@ __device__ uint32_t probe_seg(const ProbeArgs& A, const TermRec& R, const LdsWav
                     x += k >= 1 ? y : 0u;
                     y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);
                     x += k >= 2 ? y : 0u;
-                    y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);
-                    x += k >= 4 ? y : 0u;
                     const uint32_t off = x - f[PL - 1] + first;
                     bool hit = false;
 #pragma unroll
                     for (uint32_t t = 0; t < PL; ++t) hit = hit || (k * PL + t < pn && f[t] + off == vc);
                     const uint64_t m = __ballot(hit);
-                    uint32_t hit8 = 0;
+                    uint32_t hit16 = 0;
 #pragma unroll
-                    for (uint32_t q = 0; q < 8; ++q) hit8 |= ((m >> (8 * q)) & 0xffull) ? 1u << q : 0u;
-                    // candidates c0 .. c0 + 7 belong to lanes c0 % 64 .. + 7 of row c0 / 64
-                    if ((ln >> 3) == ((c0 & 63u) >> 3)) hitm |= ((hit8 >> (ln & 7u)) & 1u) << (c0 >> 6);
+                    for (uint32_t g = 0; g < 16; ++g) hit16 |= ((m >> (4 * g)) & 0xfull) ? 1u << g : 0u;
+                    // candidates c0 .. c0 + 15 belong to lanes c0 % 64 .. + 15 of row c0 / 64
+                    if ((ln >> 4) == ((c0 & 63u) >> 4)) hitm |= ((hit16 >> (ln & 15u)) & 1u) << (c0 >> 6);
                 }
 #pragma unroll
                 for (int i = 0; i < V; ++i) found[i] = (hitm >> i) & 1u;

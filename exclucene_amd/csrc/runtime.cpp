@@ -1524,6 +1524,11 @@ int fill_pfd_anchors(exc_ctx* c, exc_index* ix)
 {
     if (ix->codec != EXC_PARAPFD || ix->seg != 128 || !(ix->dix.flags & kIxMedians)) return EXC_OK;
     SegRec* rec = const_cast<SegRec*>(ix->dix.rec);
+    DevBuf ob;  // the quarters' midpoint values, 4 a block, kept with the index
+    if (int rc = dev_alloc(c, ob, std::max<uint64_t>(1, ix->nseg) * 16)) return rc;
+    ix->bufs.push_back(ob);
+    ix->bytes += ob.bytes;
+    uint32_t* oct = static_cast<uint32_t*>(ob.p);
     DevBuf dec, dj;
     std::vector<uint64_t> off;
     std::vector<uint32_t> terms;
@@ -1555,13 +1560,16 @@ int fill_pfd_anchors(exc_ctx* c, exc_index* ix)
         if ((rc = dev_alloc(c, dj, jobs.size() * sizeof(AnchorJob)))) break;
         hipError_t e = hipMemcpy(dj.p, jobs.data(), jobs.size() * sizeof(AnchorJob), hipMemcpyHostToDevice);
         if (e == hipSuccess)
-            e = launch_pfd_anchors(dptr<AnchorJob>(dj), (uint32_t)jobs.size(), dptr<uint32_t>(dec), rec, c->stream);
+            e = launch_pfd_anchors(dptr<AnchorJob>(dj), (uint32_t)jobs.size(), dptr<uint32_t>(dec), rec, oct, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = set_error(EXC_ERR_DEVICE, "ParaPFD quarter values: %s", hipGetErrorString(e));
     }
     dev_free(c, dec);
     dev_free(c, dj);
-    if (!rc) ix->dix.flags |= kIxQuarters;
+    if (!rc) {
+        ix->dix.pfd_oct = oct;
+        ix->dix.flags |= kIxQuarters | kIxOctants;
+    }
     return rc;
 }
 
@@ -1952,10 +1960,11 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
     uint32_t *d_snum, *d_nw;
     uint64_t* d_goff;
     uint32_t* d_err;
-    void *p_rec, *p_head, *p_terms;
+    void *p_rec, *p_head, *p_terms, *p_oct = nullptr;
     if ((rc = tmp.get(nterms + 1, &d_loff)) || (rc = tmp.get(nterms + 1, &d_snum)) || (rc = tmp.get(S + 1, &d_nw)) ||
         (rc = tmp.get(S + 1, &d_goff)) || (rc = tmp.get(1, &d_err)) || (rc = alloc_padded(ix, S * sizeof(SegRec), &p_rec)) ||
-        (rc = alloc_padded(ix, S * 4, &p_head)) || (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)))
+        (rc = alloc_padded(ix, S * 4, &p_head)) || (rc = alloc_padded(ix, (size_t)nterms * sizeof(TermRec), &p_terms)) ||
+        (codec == EXC_PARAPFD && (rc = alloc_padded(ix, S * 16, &p_oct))))
         return fail(rc);
     hipError_t e = hipMemcpyAsync(d_loff, list_off.data(), (nterms + 1ull) * 8, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_snum, seg_num.data(), (nterms + 1ull) * 4, hipMemcpyHostToDevice, c->stream);
@@ -1973,6 +1982,7 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
     a.nwords = d_nw;
     a.goff = d_goff;
     a.terms = static_cast<TermRec*>(p_terms);
+    a.pfd_oct = static_cast<uint32_t*>(p_oct);
     a.codec = codec;
     a.frac = frac;
     a.err = d_err;
@@ -2036,8 +2046,10 @@ int build_index_on_device(exc_ctx* c, int codec, uint32_t seg, float frac, uint3
         ix->lwords[t] = (uint32_t)((t + 1 < nterms ? ix->recs[t + 1].woff : W) - ix->recs[t].woff);
     ix->dix.codec = codec;
     ix->dix.seg = seg;
-    // k_blk_fit keeps each block's median and its other quarter values
-    ix->dix.flags = codec == EXC_PARAPFD ? kIxMedians | kIxQuarters : 0u;
+    // k_blk_fit keeps each block's median, its other quarter values and the
+    // quarters' midpoint values
+    ix->dix.flags = codec == EXC_PARAPFD ? kIxMedians | kIxQuarters | kIxOctants : 0u;
+    ix->dix.pfd_oct = a.pfd_oct;
     ix->dix.words = a.words;
     ix->dix.terms = a.terms;
     ix->d_terms = a.terms;
