@@ -158,7 +158,11 @@ int exc_ctx_set_option(exc_ctx *ctx, int key, int64_t value);
 int exc_ctx_last_times(exc_ctx *ctx, float *probe_ms, float *invoke_ms);
 
 /* ---- index residency (replaces as_load_atind + the whole-.ind2 cudaMemcpy,
- *      GC/LRC/LRC_Intersection/LRC_Intersection.cu:50-74,94-106) */
+ *      GC/LRC/LRC_Intersection/LRC_Intersection.cu:50-74,94-106)
+ * A ParaPFD index of 128-docID blocks with medians also gets, per block, its
+ * values at the other eighth boundaries for the sparse probe: its lists are
+ * decoded on the device once, 2^28 docIDs at a time, through ctx's memory;
+ * when ctx's budget leaves no room the index is uploaded without them. */
 int exc_index_upload(exc_ctx *ctx, const exc_index_host *h, exc_index **out);
 int exc_index_load_files(exc_ctx *ctx, int codec, uint32_t seg_size, const char *dir,
                          const char *name, exc_index **out);
