@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # k_probe<SegLRC,128> HBM bytes per launch, in KiB, from rocprofv3 --pmc FETCH_SIZE then WRITE_SIZE
 # runs of this bench (FETCH x1.99996 and WRITE x1.0, a 1 GiB copy kernel's calibration); they are
 # profile figures reported beside this run's, not measured in it
-TRAFFIC_C2_KIB = 1464794.0
+TRAFFIC_C2_KIB = 1464610.0
 TRAFFIC_C2_SOURCE = "profiles/k_probe_seglrc128_c2.md (rocprofv3 --pmc on the C2 batch)"
 TRAFFIC_C5_KIB = 5952507.0
 TRAFFIC_C5_SOURCE = "profiles/k_probe_seglrc128_c5.md (rocprofv3 --pmc on the C5 batch)"
